@@ -1,0 +1,358 @@
+/*
+ * srs_abi.h -- C ABI of the MI355X spark time-step (srs_amd).
+ *
+ * The reference (mwalker174/srs) has no plugin API: its seam is a set of
+ * plain C-style functions over C structs, declared in cpu/parameters.h:252-275
+ * (CPU path) and gpu/DeviceStructs.h:60-71 (CUDA path).  Each entry point
+ * below names the reference function(s) it replaces.  Everything is plain C:
+ * pointers, sizes and status codes; no C++ or torch types cross this line.
+ *
+ * Conventions
+ *  - Every call returns SRS_OK (0) or a negative SRS_E_* code; the message of
+ *    the last failure on the calling thread is returned by srs_last_error().
+ *  - Element indices, face lists and state arrays crossing the ABI are in the
+ *    reference's numbering (after the reference's BFS renumbering,
+ *    cpu/InitFuncs.cpp:679-793).  A context may store elements in another
+ *    order internally; that permutation never leaks out.
+ *  - State arrays are species-major double[6][n_ele] in the reference species
+ *    order Ca, ATP, CaM, TnC, CSQ, Fluo (cpu/parameters.h:136-141).
+ *  - Parameter vectors are int32[19] and double[56] with the reference's
+ *    index layout (cpu/parameters.h:18-92); SRS_I_* / SRS_R_* below are those
+ *    indices.  t_step is stored in ms, as ReadParameters leaves it.
+ *  - A context owns all device memory; caller buffers are copied, never kept.
+ *    One context per host thread; each context has its own HIP stream.
+ */
+#ifndef SRS_ABI_H_
+#define SRS_ABI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define SRS_OK 0
+#define SRS_E_ARG -1      /* invalid argument / size */
+#define SRS_E_IO -2       /* file missing, truncated or malformed */
+#define SRS_E_HIP -3      /* HIP runtime error (message has the HIP string) */
+#define SRS_E_STATE -4    /* call out of order (e.g. before finalize) */
+#define SRS_E_DIVERGED -5 /* non-finite or negative field detected */
+#define SRS_E_NODEV -6    /* no usable gfx950 device */
+
+const char* srs_last_error(void);
+const char* srs_version(void);
+
+/* ------------------------------------------------- parameter vector layout */
+/* cpu/parameters.h:18-36 */
+#define SRS_N_INT 19
+#define SRS_I_OUTPUT_GRID_FLAG 0
+#define SRS_I_FLAG_FIDELITY_SIM 1
+#define SRS_I_FLAG_CLOCK_SEED 2
+#define SRS_I_RAND_SEED_0 3
+#define SRS_I_RAND_ADD_A 4
+#define SRS_I_RAND_ADD_B 5
+#define SRS_I_INITIALIZE_FROM_FILE 6
+#define SRS_I_CUDA_DEVICE 7
+#define SRS_I_FLAG_SECONDARY_GRADIENT 8
+#define SRS_I_FLAG_NO_FLUX_BOUNDARY 9
+#define SRS_I_N_RYR_FIDELITY 10
+#define SRS_I_FLAG_LCC 11
+#define SRS_I_LCC_INDEX 12
+#define SRS_I_FLAG_V_CLAMP 13
+#define SRS_I_FLAG_NO_GATING 14
+#define SRS_I_GATING_FFWD 15
+#define SRS_I_FLAG_RYR_OUT 16
+#define SRS_I_N_SIMS 17
+#define SRS_I_INIT_RYR 18
+/* cpu/parameters.h:37-92 */
+#define SRS_N_REAL 56
+#define SRS_R_T_FINAL 0
+#define SRS_R_T_STEP 1
+#define SRS_R_OUTPUT_STATES_INTERVAL 2
+#define SRS_R_OUTPUT_GRID_INTERVAL 3
+#define SRS_R_C_MIN 4
+#define SRS_R_D_CA 5
+#define SRS_R_D_CA_JSR 6
+#define SRS_R_D_ATP 7
+#define SRS_R_D_CMDN 8
+#define SRS_R_D_TRPN 9
+#define SRS_R_D_CSQN 10
+#define SRS_R_D_DYE 11
+#define SRS_R_D_EGTA 12
+#define SRS_R_C_0_CA 13
+#define SRS_R_C_0_CA_JSR 14
+#define SRS_R_DEFAULT_VM 15
+#define SRS_R_DEFAULT_CA_NSR 16
+#define SRS_R_CA_0 17
+#define SRS_R_B_TOT_ATP 18
+#define SRS_R_K_OFF_ATP 19
+#define SRS_R_K_ON_ATP 20
+#define SRS_R_B_TOT_CMDN 21
+#define SRS_R_K_OFF_CMDN 22
+#define SRS_R_K_ON_CMDN 23
+#define SRS_R_B_TOT_TRPN 24
+#define SRS_R_K_OFF_TRPN 25
+#define SRS_R_K_ON_TRPN 26
+#define SRS_R_B_TOT_SL_HI 27
+#define SRS_R_K_D_SL_HI 28
+#define SRS_R_B_TOT_SL_LO 29
+#define SRS_R_K_D_SL_LO 30
+#define SRS_R_B_TOT_DYE 31
+#define SRS_R_K_OFF_DYE 32
+#define SRS_R_K_ON_DYE 33
+#define SRS_R_B_T_JSR 34
+#define SRS_R_K_M_JSR 35
+#define SRS_R_V_REFILL 36
+#define SRS_R_V_RYR 37
+#define SRS_R_V_CELL 38
+#define SRS_R_K_ON_CSQN 39
+#define SRS_R_K_OFF_CSQN 40
+#define SRS_R_RYR_ETA 41
+#define SRS_R_RYR_A_STAR 42
+#define SRS_R_RYR_EPS_CC 43
+#define SRS_R_RYR_EPS_OO 44
+#define SRS_R_RYR_K_PLUS 45
+#define SRS_R_RYR_K_MINUS 46
+#define SRS_R_RYR_PHI_M 47
+#define SRS_R_RYR_PHI_B 48
+#define SRS_R_K_D_I 49
+#define SRS_R_K_D_SR 50
+#define SRS_R_A_P 51
+#define SRS_R_LCC_DURATION 52
+#define SRS_R_T_CLAMP 53
+#define SRS_R_V_CLAMP 54
+#define SRS_R_INIT_RYR_HOLD 55
+
+#define SRS_N_SPECIES 6
+#define SRS_TITLE_LEN 256
+
+/* ======================================================= host-side layer */
+
+/* Replaces ReadParameters (cpu/InitFuncs.cpp:25-381): defaults (:28-108),
+ * then every <parameter><symbol/><value/> element of the file, mapped as the
+ * reference maps them (:191-349; atoi/atof semantics, t_step ns -> ms).
+ * Unknown symbols are ignored.  A file with no <parameter> is SRS_E_IO, as
+ * the reference throws at :164-166.  path == NULL returns the defaults. */
+int srs_read_parameters(const char* path, int32_t ints[SRS_N_INT], double reals[SRS_N_REAL],
+                        char title[SRS_TITLE_LEN]);
+
+/* Mesh as loaded and renumbered by ReadGrid (cpu/InitFuncs.cpp:384-813,
+ * GridStruct cpu/parameters.h:161-206, channel arrays of SimData :216-226).
+ * Face types are already mapped to simulation codes 0 default, 1 bound,
+ * 2 no-flux (cpu/InitFuncs.cpp:494-514). */
+typedef struct srs_grid {
+  int32_t n_ele, n_face, n_nodes;
+  double *V0, *tt_area, *sr_area;       /* [n_ele] */
+  int32_t *domain, *tropc;              /* [n_ele] */
+  int32_t *ele_nodes;                   /* [4*n_ele], 0-based */
+  double *node_xyz;                     /* [3*n_nodes] */
+  int32_t *ele1, *ele2, *face_type;     /* [n_face] */
+  double *face_c;                       /* [n_face] */
+  int32_t n_ryr, n_lcc;
+  int32_t *ryr_ele, *ryr_jsr, *ryr_nb;  /* [n_ryr], [n_ryr], [4*n_ryr] */
+  int32_t *lcc_ele;                     /* [n_lcc] */
+  int32_t *perm;                        /* [n_ele] new -> file index (the BFS order) */
+} srs_grid;
+
+/* Replaces ReadGrid.  Reads base_{tet,face}_properties.txt, base.node,
+ * base.ele, base_{ryr,lcc}.txt, maps face types with the no-flux flag, and
+ * renumbers by the reference's BFS from element 0 (in O(N) rather than the
+ * reference's O(N_e*N_f)).  Fails with SRS_E_IO on a missing or truncated
+ * file (the reference prints and continues with uninitialised arrays) and
+ * with SRS_E_ARG on a disconnected face graph (where the reference reads out
+ * of bounds).  Duplicate channel elements are reported through *warnings. */
+int srs_grid_read(const char* base, int no_flux_boundary, srs_grid** out, int32_t* warnings);
+void srs_grid_free(srs_grid* g);
+
+/* The reference BFS order alone (cpu/InitFuncs.cpp:679-731): visited[i] is the
+ * original index of new element i.  Returns SRS_E_ARG if not connected. */
+int srs_bfs_order(int32_t n_ele, int32_t n_face, const int32_t* ele1, const int32_t* ele2,
+                  int32_t* visited);
+
+/* Replaces InitializeExplicitSparseMatrixFast (cpu/InitFuncs.cpp:1058-1112,
+ * 1273-1322): the explicit FV operator packed as the reference packs it,
+ * aij[5*i] diagonal then up to four off-diagonals with ascending columns,
+ * nij[4*i] their columns, self and 0.0 where absent. */
+int srs_assemble_operator(int32_t n_ele, int32_t n_face, const int32_t* ele1, const int32_t* ele2,
+                          const int32_t* face_type, const double* face_c, const double* V0,
+                          double* aij, int32_t* nij);
+
+/* InitializeDomainStruct boundary rates (cpu/InitFuncs.cpp:1012-1026), br out
+ * double[6][n_ele]; and ResetDefaultGridStates (cpu/InitFuncs.cpp:890-954),
+ * X out double[6][n_ele], boundaries out double[6] (may be NULL). */
+int srs_boundary_rates(int32_t n_ele, int32_t n_face, const int32_t* ele1, const int32_t* face_type,
+                       const double* face_c, const double* V0, const double reals[SRS_N_REAL],
+                       const double boundaries[SRS_N_SPECIES], double* br);
+int srs_equilibrium_state(int32_t n_ele, const int32_t* domain, const int32_t* tropc,
+                          const double reals[SRS_N_REAL], double* X, double boundaries[SRS_N_SPECIES]);
+
+/* Synthetic geometry.  Writes base.{node,ele,face}, base_{tet,face}_properties.txt
+ * and base_{ryr,lcc}.txt in the formats of mesh/main.cpp:499-567, 1305-1364 for
+ * a rectilinear box split into 6 Kuhn tets per cell, with optional jSR slab,
+ * sarcolemmal face and T-tubule cylinder.  Face coefficients follow
+ * mesh/main.cpp:334-482.  No reference counterpart (the reference meshes with
+ * tetgen, which is not rebuilt). */
+typedef struct srs_box_spec {
+  int32_t nx, ny, nz;
+  const double *xs, *ys, *zs;   /* plane coordinates, nx+1 / ny+1 / nz+1 values, um */
+  double jsr_lo[3], jsr_hi[3];  /* jSR slab box (elements whose centroid lies inside), hi<lo: none */
+  int32_t sl_face_z0;           /* 1: the z = zs[0] face is sarcolemma (face type 2) */
+  double tt_axis_x, tt_axis_z, tt_radius; /* T-tubule cylinder along y; radius <= 0: none */
+  double trpn_radius;           /* TnC in cytosol elements farther than this from the TT axis */
+  int32_t ryr_rows, ryr_cols;   /* RyR lattice (rows along y, cols along x) */
+  double ryr_spacing, ryr_center[2], ryr_z_cyt, ryr_z_jsr;
+  const uint8_t* ryr_mask;      /* rows*cols occupancy, row-major; NULL = all */
+  int32_t n_lcc;
+  const double* lcc_xyz;        /* [3*n_lcc] LCC positions (nearest cytosol element) */
+} srs_box_spec;
+int srs_box_mesh_write(const srs_box_spec* spec, const char* base);
+/* The same mesh built in memory and returned as srs_grid_read would return it
+ * after reading the files (face types mapped, BFS-renumbered), for meshes too
+ * large to round-trip through text files. */
+int srs_box_grid(const srs_box_spec* spec, int no_flux_boundary, srs_grid** out);
+
+/* States record of WriteToStateFiles (cpu/OutputFuncs.cpp:223-350): t, I_RyR,
+ * N_RyR open, N_LCC open, I_LCC, mean Ca_jSR at RyRs, mean Ca_cyt at RyRs,
+ * [gpu_layout: Ca_cyt at each RyR, gpu/OutputFuncs.cpp:277-284],
+ * [ryr_out: RyR states].  ca_ryr_cyt/ca_ryr_jsr/ca_lcc are Ca at the channel
+ * elements.  Returns the number of floats written (> 255 is SRS_E_ARG, the
+ * reference's buffer limit, cpu/OutputFuncs.cpp:441-444). */
+int srs_states_record(double t, int32_t n_ryr, const int32_t* ryr_state, const double* ca_ryr_cyt,
+                      const double* ca_ryr_jsr, int32_t n_lcc, const int32_t* lcc_state,
+                      const int32_t* lccv_state, const double* ca_lcc, double vm,
+                      const double reals[SRS_N_REAL], int32_t ryr_out, int32_t gpu_layout, float* rec);
+
+/* Replaces main + RunSimulation (cpu/main.cpp:18-274, cpu/RunSimulation.cpp:402-589):
+ * the cru3d driver.  Reads the parameter file and mesh, runs sims_per_proc
+ * realizations from start_index on the GPU, writes states_<rank>_<run>.out
+ * and the VTK grids into out_dir (created if needed).  rank plays the role of
+ * the MPI rank in seeding and file names.  The states file keeps the CPU
+ * record layout unless gpu_layout is set. */
+int srs_run_cru3d(const char* param_path, const char* mesh_base, const char* out_dir, int32_t rank,
+                  int32_t start_index, int32_t gpu_layout, double* wall_seconds);
+
+/* ======================================================= device layer */
+
+typedef struct srs_ctx srs_ctx;
+
+/* Element order inside a context (never visible through the ABI). */
+#define SRS_ORDER_AS_GIVEN 0
+#define SRS_ORDER_RCM 1
+
+/* Replaces InitializeDeviceConstants / cudaSetDevice (gpu/RunSimulation.cu:503-506, 869-872). */
+int srs_ctx_create(int device, srs_ctx** out);
+/* Replaces FreeDomainStruct / FreeGlobalStruct (gpu/RunSimulation.cu:874-901). */
+int srs_ctx_destroy(srs_ctx* ctx);
+
+/* Geometry of one mesh, shared by all realizations on it.  Replaces the
+ * DeviceDomainStruct upload of InitializeDomainStruct (gpu/InitFuncs.cu:1001-1093,
+ * gpu/DeviceStructs.h:19-38).  aij/nij as from srs_assemble_operator; br may
+ * be NULL (all zero, as with flag_noflux).  order: SRS_ORDER_*. */
+typedef struct srs_mesh_desc {
+  int32_t n_ele;
+  const double* aij;       /* [5*n_ele] */
+  const int32_t* nij;      /* [4*n_ele] */
+  const double* br;        /* [6*n_ele] or NULL */
+  const int32_t* domain;   /* [n_ele] */
+  const int32_t* tropc;    /* [n_ele] */
+  const double* V0;        /* [n_ele] */
+  const double* tt_area;   /* [n_ele] */
+  int32_t order;
+} srs_mesh_desc;
+int srs_ctx_add_mesh(srs_ctx* ctx, const srs_mesh_desc* mesh, int32_t* mesh_id);
+
+/* One realization: a channel set on a mesh plus its own parameter vector.
+ * Replaces InitializeDeviceGlobalStruct (gpu/InitFuncs.cu:1736-1782,
+ * gpu/DeviceStructs.h:40-58).  Channel element indices are mesh indices. */
+typedef struct srs_chan_desc {
+  int32_t n_ryr;
+  const int32_t* ryr_ele;  /* [n_ryr] cleft element */
+  const int32_t* ryr_jsr;  /* [n_ryr] jSR element */
+  const int32_t* ryr_nb;   /* [4*n_ryr] lattice neighbours, -1 none */
+  int32_t n_lcc;
+  const int32_t* lcc_ele;  /* [n_lcc] */
+} srs_chan_desc;
+/* seed_key: the Philox key for this realization's gating draws; stream: the
+ * realization's counter word (distinct streams for distinct realizations). */
+int srs_ctx_add_realization(srs_ctx* ctx, int32_t mesh_id, const srs_chan_desc* ch,
+                            const int32_t ints[SRS_N_INT], const double reals[SRS_N_REAL],
+                            uint64_t seed_key, uint64_t stream, int32_t* real_id);
+
+/* Allocates and uploads everything; after this the context is runnable.
+ * record_capacity: states records kept on the device per realization
+ * between srs_records_read calls (0: no device-side records). */
+int srs_ctx_finalize(srs_ctx* ctx, int32_t record_capacity);
+
+/* Field and channel state transfer (the cudaMemcpy calls of
+ * gpu/RunSimulation.cu:712-723, 755-763).  X is double[6][n_ele] of the
+ * realization's mesh, reference numbering. */
+int srs_state_set(srs_ctx* ctx, int32_t real, const double* X);
+int srs_state_get(srs_ctx* ctx, int32_t real, double* X);
+int srs_channels_set(srs_ctx* ctx, int32_t real, const int32_t* ryr_state, const int32_t* lcc_state,
+                     const int32_t* lccv_state, const double* R);
+int srs_channels_get(srs_ctx* ctx, int32_t real, int32_t* ryr_state, int32_t* lcc_state,
+                     int32_t* lccv_state, double* R);
+int srs_globals_set(srs_ctx* ctx, int32_t real, double vm, double ca_nsr);
+int srs_globals_get(srs_ctx* ctx, int32_t real, double* vm, double* ca_nsr);
+
+/* Loop state of a realization as RunSimulation keeps it (cpu/RunSimulation.cpp:410-460):
+ * simulated time, output bookkeeping and protocol flags. */
+typedef struct srs_loop_state {
+  double t, dt_last;
+  int64_t step;              /* Philox counter word: number of steps taken */
+  int32_t done;              /* 1 once the reference's while-condition failed */
+  int32_t lcc_open;          /* bLCC_Open */
+  double t_lcc_close;        /* T_LCC_Close */
+  int32_t n_ryr_open_last;   /* N_RyR_Open as of the last states output */
+  double ca_ss_max_last;     /* ca_ss_max as of the last states output */
+} srs_loop_state;
+int srs_loop_set(srs_ctx* ctx, int32_t real, const srs_loop_state* ls);
+int srs_loop_get(srs_ctx* ctx, int32_t real, srs_loop_state* ls);
+
+/* Step flags */
+#define SRS_STEP_PLAIN 0u      /* exactly ExplicitStep: sweep, then gating unless flag_nogating */
+#define SRS_STEP_PROTOCOL 1u   /* also run the RunSimulation loop body on the device: time
+                                  accumulation, while-condition, LCC trigger, voltage clamp,
+                                  RyR hold and states-record output (cpu/RunSimulation.cpp:522-573) */
+#define SRS_STEP_GRAPH 2u      /* replay the step block as a captured hipGraph */
+
+/* Replaces ExplicitStep (cpu/RunSimulation.cpp:591-654; gpu/RunSimulation.cu:781-852),
+ * called n_steps times for every realization that is not done, with each
+ * realization's own dt = reals[SRS_R_T_STEP]. */
+int srs_explicit_step(srs_ctx* ctx, int64_t n_steps, uint32_t flags);
+
+/* Replaces UpdateChannelGating (cpu/RunSimulation.cpp:656-674; gpu/RunSimulation.cu:854-866)
+ * alone, for every realization: the gating-only loop of flag_ffwd. */
+int srs_channel_gating(srs_ctx* ctx);
+
+/* Deterministic replay: n_steps ExplicitStep calls with gating off and the RyR
+ * states of realization `real` overwritten before step k by
+ * schedule[k*n_ryr + i] (a recorded open/closed schedule). */
+int srs_explicit_step_replay(srs_ctx* ctx, int32_t real, int64_t n_steps, const int32_t* schedule);
+
+/* Ca at the channel elements (for the states record on the host). */
+int srs_channel_ca(srs_ctx* ctx, int32_t real, double* ca_ryr_cyt, double* ca_ryr_jsr, double* ca_lcc);
+
+/* Device-side states records written by SRS_STEP_PROTOCOL steps since the last
+ * call; copies up to max_records records of rec_len floats. */
+int srs_records_read(srs_ctx* ctx, int32_t real, float* out, int32_t max_records, int32_t* n_records,
+                     int32_t* rec_len);
+
+/* Waits for all work of the context. */
+int srs_ctx_sync(srs_ctx* ctx);
+
+/* Sizes of a realization. */
+int srs_real_info(srs_ctx* ctx, int32_t real, int32_t* n_ele, int32_t* n_ryr, int32_t* n_lcc);
+
+/* Timing of the last srs_explicit_step call, measured with HIP events on the
+ * context's stream: total milliseconds and the mean duration of one sweep
+ * kernel (ms, from a separately timed single-step launch when requested). */
+int srs_last_timing(srs_ctx* ctx, double* total_ms, double* sweep_ms);
+int srs_time_sweep(srs_ctx* ctx, int32_t reps, double* sweep_ms, double* gating_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SRS_ABI_H_ */

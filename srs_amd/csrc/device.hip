@@ -1,0 +1,1039 @@
+// device.hip -- gfx950 kernels and the device half of the C ABI.
+//
+// One spark time-step (cpu/RunSimulation.cpp:591-654) is two kernels:
+//   k_sweep : every element of every live realization, fused: 5-point ELL
+//             diffusion of Ca/Fluo/ATP/CaM, binding reactions, SERCA, jSR
+//             refill, RyR/LCC flux injection, rapid SL buffer and the forward
+//             Euler update, old state -> new state (ping-pong buffers).
+//   k_gate  : one workgroup per realization: RyR and LCC Markov gating on the
+//             new Ca (integrated-hazard residuals, Philox draws), then,
+//             optionally, the RunSimulation loop body for the next time
+//             (while-condition, LCC trigger, voltage clamp, RyR hold,
+//             states-record output).
+// The reference runs the same step as nine kernels with a state-sized rate
+// buffer (gpu/RunSimulation.cu:781-852); here the rates never leave registers.
+//
+// Arithmetic keeps the reference's operation order and this file is compiled
+// without FP contraction, so the field update is bit-identical to the
+// reference CPU path for the same row order of each element's neighbours.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "kinetics.h"
+#include "philox.h"
+#include "srs_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int TILE = 256;      // elements per workgroup in k_sweep
+constexpr int GATE_THREADS = 256;
+constexpr int MAX_RYR = 1024;  // the reference GPU path launches <<<1, N_RyR>>>
+constexpr int MAX_LCC = 1024;
+constexpr int NXCD = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+enum : uint8_t { F_JSR = 1, F_TROPC = 2, F_SL = 4, F_BR = 8, F_CHAN = 16 };
+
+// Immutable per-realization descriptor (read with scalar loads).
+struct RealDesc {
+  const int32_t* nbr;     // [4][np]
+  const double* coef;     // [5][np]
+  const uint8_t* flags;   // [np]
+  const double* V0;       // [np]
+  const double* tt;       // [np]
+  const double* br;       // [4][np] Ca, Fluo, ATP, CaM boundary rates, or null
+  double* X[2];           // [6][np] ping-pong state
+  int32_t n, np;
+  int32_t n_ryr, n_lcc;
+  const int32_t* ryr_ele;
+  const int32_t* ryr_jsr;
+  const int32_t* ryr_nb;
+  int32_t* ryr_state;
+  const int32_t* lcc_ele;
+  int32_t* lcc_state;
+  int32_t* lccv_state;
+  double* R;
+  int32_t n_celem;
+  const int32_t* celem;   // sorted element indices that carry channel flux
+  const int32_t* cstart;  // [n_celem+1]
+  const int32_t* centry;  // >=0: 2*ryr + side(0 cleft, 1 jSR); <0: -(lcc+1)
+  const int32_t* sched;   // replay schedule [n_sched][n_ryr] or null
+  int64_t n_sched;
+  float* rec;
+  int32_t rec_cap, rec_len, ryr_out, gpu_layout;
+  uint64_t key, stream;
+  double P[SRS_N_REAL];
+  int32_t I[SRS_N_INT];
+};
+
+// Mutable per-realization loop state.
+struct RealLive {
+  double t, dt_last, vm, ca_nsr, t_lcc_close, ca_max_last;
+  int64_t step;
+  int64_t sched_base;
+  int32_t done, lcc_open, n_open_last, cur, rec_count, rec_overflow;
+};
+
+enum : uint32_t {
+  G_AFTER_SWEEP = 1,  // new state is X[1-cur]; flip cur afterwards
+  G_PROTOCOL = 2,     // run the loop body for the next time
+  G_NO_GATING = 4,    // skip gating regardless of the parameter flag
+  G_REPLAY = 8,       // load RyR states for the next step from the schedule
+};
+
+__device__ __forceinline__ int xcd_tile(int bid, int grid) {
+  // Consecutive tiles of a realization land on one XCD, so neighbour gathers
+  // that cross a tile boundary hit that XCD's L2.
+  return (bid % NXCD) * (grid / NXCD) + bid / NXCD;
+}
+
+__global__ __launch_bounds__(TILE) void k_sweep(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                                                const int2* __restrict__ tiles, int n_tiles) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  if (b >= n_tiles) return;
+  const int2 tl = tiles[b];
+  const RealDesc& d = D[tl.x];
+  const RealLive& l = L[tl.x];
+  if (l.done) return;
+  const int j = tl.y * TILE + threadIdx.x;
+  if (j >= d.n) return;
+
+  const int np = d.np;
+  const int cur = l.cur;
+  const double* __restrict__ Xo = d.X[cur];
+  double* __restrict__ Xn = d.X[cur ^ 1];
+  const double* P = d.P;
+  const double dt = P[SRS_R_T_STEP];
+  const uint8_t f = d.flags[j];
+
+  const double a0 = d.coef[j], a1 = d.coef[np + j], a2 = d.coef[2 * np + j], a3 = d.coef[3 * np + j],
+               a4 = d.coef[4 * np + j];
+  const int n1 = d.nbr[j], n2 = d.nbr[np + j], n3 = d.nbr[2 * np + j], n4 = d.nbr[3 * np + j];
+
+  const double* Sca = Xo + srs::CA * np;
+  const double* Sdy = Xo + srs::DYE * np;
+  const double* Sat = Xo + srs::ATP * np;
+  const double* Scm = Xo + srs::CMDN * np;
+  const double c_ca = Sca[j], c_dy = Sdy[j], c_at = Sat[j], c_cm = Scm[j];
+  const double c_tr = Xo[srs::TRPN * np + j], c_cs = Xo[srs::CSQN * np + j];
+
+  double br_ca = 0.0, br_dy = 0.0, br_at = 0.0, br_cm = 0.0;
+  if (f & F_BR) {
+    br_ca = d.br[j];
+    br_dy = d.br[np + j];
+    br_at = d.br[2 * np + j];
+    br_cm = d.br[3 * np + j];
+  }
+  // cpu/RunSimulation.cpp:61-71 (Ca uses D_CA in both domains)
+  double rca = P[SRS_R_D_CA] * (c_ca * a0 + Sca[n1] * a1 + Sca[n2] * a2 + Sca[n3] * a3 + Sca[n4] * a4) + br_ca;
+  double rdy = P[SRS_R_D_DYE] * (c_dy * a0 + Sdy[n1] * a1 + Sdy[n2] * a2 + Sdy[n3] * a3 + Sdy[n4] * a4) + br_dy;
+  double rat = P[SRS_R_D_ATP] * (c_at * a0 + Sat[n1] * a1 + Sat[n2] * a2 + Sat[n3] * a3 + Sat[n4] * a4) + br_at;
+  double rcm = P[SRS_R_D_CMDN] * (c_cm * a0 + Scm[n1] * a1 + Scm[n2] * a2 + Scm[n3] * a3 + Scm[n4] * a4) + br_cm;
+  double rtr = 0, rcs = 0;
+  const double nsr = l.ca_nsr;
+
+  if (!(f & F_JSR)) {
+    if (f & F_TROPC) {  // :75-91
+      const double r = P[SRS_R_K_ON_TRPN] * (P[SRS_R_B_TOT_TRPN] - c_tr) * c_ca - P[SRS_R_K_OFF_TRPN] * c_tr;
+      rca += -r;
+      rtr = r;
+      double K_i = c_ca / P[SRS_R_K_D_I];
+      K_i = K_i * K_i;
+      double K_sr = nsr / P[SRS_R_K_D_SR];
+      K_sr = K_sr * K_sr;
+      const double D_cycle = 0.104217 + 17.923 * K_sr + K_i * (1.75583e6 + K_sr * 7.61673e6) +
+                             K_i * K_i * (6.08463e11 + K_sr * 4.50544e11);
+      const double v_cycle =
+          (K_i * K_i * 3.24873e12 + K_i * (9.17846e6 - 11478.2 * K_sr) - 0.329904 * K_sr) / D_cycle;
+      const double d_ca = (2e-3) * v_cycle * P[SRS_R_A_P];
+      rca -= d_ca;
+    }
+    double r = P[SRS_R_K_ON_DYE] * (P[SRS_R_B_TOT_DYE] - c_dy) * c_ca - P[SRS_R_K_OFF_DYE] * c_dy;
+    rca += -r;
+    rdy += r;
+    r = P[SRS_R_K_ON_ATP] * (P[SRS_R_B_TOT_ATP] - c_at) * c_ca - P[SRS_R_K_OFF_ATP] * c_at;
+    rca += -r;
+    rat += r;
+    r = P[SRS_R_K_ON_CMDN] * (P[SRS_R_B_TOT_CMDN] - c_cm) * c_ca - P[SRS_R_K_OFF_CMDN] * c_cm;
+    rca += -r;
+    rcm += r;
+  } else {  // :113-121
+    const double r = P[SRS_R_K_ON_CSQN] * (P[SRS_R_B_T_JSR] - c_cs) * c_ca - P[SRS_R_K_OFF_CSQN] * c_cs;
+    rca += -r;
+    rcs = r;
+    rca += (nsr - c_ca) * P[SRS_R_V_REFILL];
+  }
+
+  if (f & F_CHAN) {  // :619-632, contributions in the reference's channel order
+    int lo = 0, hi = d.n_celem;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (d.celem[mid] < j) lo = mid + 1; else hi = mid;
+    }
+    if (lo < d.n_celem && d.celem[lo] == j) {
+      const double v0 = d.V0[j];
+      for (int e = d.cstart[lo]; e < d.cstart[lo + 1]; ++e) {
+        const int c = d.centry[e];
+        if (c >= 0) {
+          const int i = c >> 1;
+          if (d.ryr_state[i]) {
+            const double J = P[SRS_R_V_RYR] * (Sca[d.ryr_jsr[i]] - Sca[d.ryr_ele[i]]) * P[SRS_R_V_CELL];
+            if (c & 1) rca -= J / v0; else rca += J / v0;
+          }
+        } else {
+          const int i = -c - 1;
+          if (srs_lcc_open(d.lcc_state[i], d.lccv_state[i])) rca += srs_lcc_flux_raw(l.vm, c_ca, P[SRS_R_CA_0]) / v0;
+        }
+      }
+    }
+  }
+
+  if (f & F_SL) {  // :15-22; B == 1 exactly where the TT area is zero
+    const double btothi = d.tt[j] * P[SRS_R_B_TOT_SL_HI] / (d.V0[j] * (1e-15));
+    const double btotlo = d.tt[j] * P[SRS_R_B_TOT_SL_LO] / (d.V0[j] * (1e-15));
+    const double kh = P[SRS_R_K_D_SL_HI], kl = P[SRS_R_K_D_SL_LO];
+    const double B = 1 + (btothi * kh / ((kh + c_ca) * (kh + c_ca))) + (btotlo * kl / ((kl + c_ca) * (kl + c_ca)));
+    rca /= B;
+  }
+
+  // :640-644 forward Euler
+  Xn[srs::CA * np + j] = c_ca + rca * dt;
+  Xn[srs::ATP * np + j] = c_at + rat * dt;
+  Xn[srs::CMDN * np + j] = c_cm + rcm * dt;
+  Xn[srs::TRPN * np + j] = c_tr + rtr * dt;
+  Xn[srs::CSQN * np + j] = c_cs + rcs * dt;
+  Xn[srs::DYE * np + j] = c_dy + rdy * dt;
+}
+
+__device__ void write_record(const RealDesc& d, RealLive& l, const double* Ca, double t) {
+  // Called by one thread: sums run sequentially, in the reference's order.
+  if (l.rec_count >= d.rec_cap) { l.rec_overflow = 1; return; }
+  float* rec = d.rec + (size_t)l.rec_count * d.rec_len;
+  int k = 0;
+  rec[k++] = (float)t;
+  double i_ryr = 0;
+  for (int i = 0; i < d.n_ryr; ++i)
+    i_ryr += d.ryr_state[i] * d.P[SRS_R_V_RYR] * (Ca[d.ryr_jsr[i]] - Ca[d.ryr_ele[i]]) * (d.P[SRS_R_V_CELL]) * 96485 *
+             2 * (1e-6);
+  rec[k++] = (float)i_ryr;
+  int n_open = 0;
+  for (int i = 0; i < d.n_ryr; ++i) n_open += d.ryr_state[i];
+  rec[k++] = (float)n_open;
+  int n_open_lcc = 0;
+  for (int i = 0; i < d.n_lcc; ++i)
+    if (d.lccv_state[i] == 2 && (d.lcc_state[i] == 6 || d.lcc_state[i] == 12)) n_open_lcc++;
+  rec[k++] = (float)n_open_lcc;
+  double i_lcc = 0;
+  for (int i = 0; i < d.n_lcc; ++i)
+    if (srs_lcc_open(d.lcc_state[i], d.lccv_state[i]))
+      i_lcc += srs_lcc_flux_raw(l.vm, Ca[d.lcc_ele[i]], d.P[SRS_R_CA_0]) * 96485 * 2 * (1e-6);
+  rec[k++] = (float)i_lcc;
+  double m = 0;
+  for (int i = 0; i < d.n_ryr; ++i) m += Ca[d.ryr_jsr[i]];
+  m /= d.n_ryr;
+  rec[k++] = (float)m;
+  m = 0;
+  for (int i = 0; i < d.n_ryr; ++i) m += Ca[d.ryr_ele[i]];
+  m /= d.n_ryr;
+  rec[k++] = (float)m;
+  if (d.gpu_layout)
+    for (int i = 0; i < d.n_ryr; ++i) rec[k++] = (float)Ca[d.ryr_ele[i]];
+  if (d.ryr_out)
+    for (int i = 0; i < d.n_ryr; ++i) rec[k++] = (float)d.ryr_state[i];
+  l.rec_count++;
+}
+
+__global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restrict__ D, RealLive* __restrict__ L,
+                                                       uint32_t mode) {
+  const RealDesc& d = D[blockIdx.x];
+  RealLive& l = L[blockIdx.x];
+  if (l.done) return;
+  __shared__ int s_old[MAX_RYR];
+  __shared__ int s_a[MAX_RYR];
+  __shared__ int s_b[MAX_RYR];
+  const int tid = threadIdx.x;
+  const int cur = l.cur;
+  const double* Xn = (mode & G_AFTER_SWEEP) ? d.X[cur ^ 1] : d.X[cur];
+  const double* Ca = Xn;  // species 0 plane
+  const double* P = d.P;
+  const double dt = P[SRS_R_T_STEP];
+  const int64_t step = l.step;
+  const bool gating = !(mode & G_NO_GATING) && !d.I[SRS_I_FLAG_NO_GATING];
+  const int nr = d.n_ryr;
+
+  if (gating) {
+    // RyRs: the reference updates them one after another (later RyRs see
+    // earlier flips through the neighbour counts, cpu/RunSimulation.cpp:661-666).
+    // Each sweep below recomputes every RyR with lower-index neighbours taken
+    // from the previous sweep's result and higher-index ones from the old
+    // states; the first sweep that changes nothing is the sequential result.
+    for (int i = tid; i < nr; i += GATE_THREADS) { s_old[i] = d.ryr_state[i]; s_a[i] = s_old[i]; }
+    __syncthreads();
+    const bool coupled = P[SRS_R_RYR_A_STAR] != 0.0;
+    int* prev = s_a;
+    int* next = s_b;
+    if (coupled) {
+      for (int it = 0; it <= nr; ++it) {
+        int changed = 0;
+        for (int i = tid; i < nr; i += GATE_THREADS) {
+          int no = 0, ncl = 0;
+          for (int q = 0; q < 4; ++q) {
+            const int nb = d.ryr_nb[4 * i + q];
+            if (nb >= 0) {
+              const int s = (nb < i) ? prev[nb] : s_old[nb];
+              if (s) no++; else ncl++;
+            }
+          }
+          const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[d.ryr_ele[i]], Ca[d.ryr_jsr[i]], P);
+          const double Rn = d.R[i] + rate * dt;
+          const int ns = (Rn > 0) ? (s_old[i] == 1 ? 0 : 1) : s_old[i];
+          next[i] = ns;
+          changed |= (ns != prev[i]);
+        }
+        changed = __syncthreads_or(changed);
+        int* tmp = prev; prev = next; next = tmp;
+        if (!changed) break;
+      }
+    }
+    // Commit with neighbour states from the fixed point (for uncoupled RyRs
+    // the neighbour counts do not enter the rate, A* = 0 gives X = 1).
+    for (int i = tid; i < nr; i += GATE_THREADS) {
+      int no = 0, ncl = 0;
+      for (int q = 0; q < 4; ++q) {
+        const int nb = d.ryr_nb[4 * i + q];
+        if (nb >= 0) {
+          const int s = (nb < i) ? prev[nb] : s_old[nb];
+          if (s) no++; else ncl++;
+        }
+      }
+      const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[d.ryr_ele[i]], Ca[d.ryr_jsr[i]], P);
+      double Rn = d.R[i] + rate * dt;
+      int ns = s_old[i];
+      if (Rn > 0) {
+        uint64_t o[4];
+        srs_philox4x64_10((uint64_t)step, (uint64_t)i, d.stream, SRS_PURPOSE_GATING, d.key, SRS_PHILOX_KEY1, o);
+        Rn = log(srs_open01(o[0]));
+        ns = (s_old[i] == 1) ? 0 : 1;
+      }
+      d.R[i] = Rn;
+      d.ryr_state[i] = ns;
+    }
+    // LCCs, cpu/RunSimulation.cpp:229-400
+    for (int i = tid; i < d.n_lcc; i += GATE_THREADS) {
+      const int s = d.lcc_state[i], vs = d.lccv_state[i];
+      double tr[3];
+      int32_t tg[3];
+      const double lr = srs_lcc_rates(s, vs, l.vm, Ca[d.lcc_ele[i]], tr, tg);
+      const int ic = nr + i;
+      double Rn = d.R[ic] + (tr[0] + tr[1] + tr[2] + lr) * dt;
+      if (Rn > 0) {
+        uint64_t o[4];
+        srs_philox4x64_10((uint64_t)step, (uint64_t)ic, d.stream, SRS_PURPOSE_GATING, d.key, SRS_PHILOX_KEY1, o);
+        Rn = log(srs_open01(o[0]));
+        const double u = srs_open01(o[1]) * (tr[0] + tr[1] + tr[2] + lr);
+        if (u < tr[0]) d.lcc_state[i] = tg[0];
+        else if (u < tr[0] + tr[1]) d.lcc_state[i] = tg[1];
+        else if (u < tr[0] + tr[1] + tr[2]) d.lcc_state[i] = tg[2];
+        else d.lccv_state[i] = (vs == 1) ? 2 : 1;
+      }
+      d.R[ic] = Rn;
+    }
+  }
+  __syncthreads();
+
+  if (mode & G_REPLAY) {
+    const int64_t k = step + 1 - l.sched_base;  // schedule row of the next step
+    if (k < d.n_sched)
+      for (int i = tid; i < nr; i += GATE_THREADS) d.ryr_state[i] = d.sched[k * nr + i];
+  }
+
+  if (tid != 0) return;
+  if (mode & G_AFTER_SWEEP) l.cur = cur ^ 1;
+  l.step = step + 1;
+  const double t = l.t + dt;  // cpu/RunSimulation.cpp:572-573
+  l.t = t;
+  l.dt_last = dt;
+  if (!(mode & G_PROTOCOL)) return;
+
+  // While-condition at the top of the next iteration (:522).
+  const bool fid = d.I[SRS_I_FLAG_FIDELITY_SIM] != 0;
+  const bool cont = t <= P[SRS_R_T_FINAL] &&
+                    (!fid || ((l.lcc_open || l.n_open_last > 0 || l.ca_max_last > P[SRS_R_C_MIN]) &&
+                              l.n_open_last < d.I[SRS_I_N_RYR_FIDELITY]));
+  if (!cont) {
+    l.done = 1;
+    write_record(d, l, Ca, t);  // final states output, :584
+    return;
+  }
+  if (d.I[SRS_I_FLAG_LCC] && l.lcc_open && t >= l.t_lcc_close) {  // :525-528
+    l.lcc_open = 0;
+    d.lcc_state[d.I[SRS_I_LCC_INDEX]] = 1;
+  }
+  if (d.I[SRS_I_FLAG_V_CLAMP]) l.vm = (t <= P[SRS_R_T_CLAMP]) ? P[SRS_R_V_CLAMP] : P[SRS_R_DEFAULT_VM];  // :531-537
+  const int ir = d.I[SRS_I_INIT_RYR];
+  if (ir != -1 && t < P[SRS_R_INIT_RYR_HOLD] && !d.ryr_state[ir]) d.ryr_state[ir] = 1;  // :540-542
+  const double iv = P[SRS_R_OUTPUT_STATES_INTERVAL];
+  if (floor(t / iv) > floor((t - dt) / iv)) {  // :552-566
+    int n_open = 0;
+    double cmax = 0;
+    for (int i = 0; i < nr; ++i) {
+      n_open += d.ryr_state[i];
+      const double c = Ca[d.ryr_ele[i]];
+      if (c > cmax) cmax = c;
+    }
+    l.n_open_last = n_open;
+    l.ca_max_last = cmax;
+    write_record(d, l, Ca, t);
+  }
+}
+
+// Gathers Ca at the channel elements of one realization.
+__global__ void k_channel_ca(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int r, double* out) {
+  const RealDesc& d = D[r];
+  const double* Ca = d.X[L[r].cur];
+  for (int i = threadIdx.x; i < d.n_ryr; i += blockDim.x) {
+    out[i] = Ca[d.ryr_ele[i]];
+    out[d.n_ryr + i] = Ca[d.ryr_jsr[i]];
+  }
+  for (int i = threadIdx.x; i < d.n_lcc; i += blockDim.x) out[2 * d.n_ryr + i] = Ca[d.lcc_ele[i]];
+}
+
+// ----------------------------------------------------------------- host side
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) return srs::fail(SRS_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct HostMesh {
+  int32_t n = 0, np = 0;
+  std::vector<int32_t> perm;  // dev -> ref
+  std::vector<int32_t> inv;   // ref -> dev
+  std::vector<int32_t> nbr;   // [4][np] dev numbering
+  std::vector<double> coef;   // [5][np]
+  std::vector<uint8_t> flags;
+  std::vector<double> V0, tt, br;  // br [4][np] or empty
+  int32_t *d_nbr = nullptr;
+  double *d_coef = nullptr, *d_V0 = nullptr, *d_tt = nullptr, *d_br = nullptr;
+  uint8_t* d_flags = nullptr;
+};
+
+struct HostReal {
+  int32_t mesh = 0;
+  std::vector<int32_t> ryr_ele, ryr_jsr, ryr_nb, lcc_ele;  // dev numbering
+  std::vector<int32_t> celem, cstart, centry;
+  int32_t ints[SRS_N_INT];
+  double reals[SRS_N_REAL];
+  uint64_t key = 0, stream = 0;
+  RealDesc desc{};
+};
+
+}  // namespace
+
+struct srs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool finalized = false;
+  std::vector<std::unique_ptr<HostMesh>> meshes;
+  std::vector<HostReal> reals;
+  std::vector<void*> allocs;
+  RealDesc* d_desc = nullptr;
+  RealLive* d_live = nullptr;
+  int2* d_tiles = nullptr;
+  int n_tiles = 0, grid_sweep = 0;
+  double last_total_ms = 0, last_sweep_ms = 0;
+  // captured step blocks, keyed by (steps, gate mode)
+  std::map<std::pair<int64_t, uint32_t>, hipGraphExec_t> graphs;
+  int32_t* d_sched = nullptr;
+};
+
+namespace {
+
+template <class T>
+int dev_alloc(srs_ctx* c, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return SRS_OK;
+  HIPCHK(hipMalloc((void**)p, count * sizeof(T)));
+  c->allocs.push_back((void*)*p);
+  return SRS_OK;
+}
+
+template <class T>
+int dev_upload(srs_ctx* c, T** p, const std::vector<T>& v) {
+  int rc = dev_alloc(c, p, v.size());
+  if (rc) return rc;
+  if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return SRS_OK;
+}
+
+int check_real(srs_ctx* c, int32_t r) {
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  if (!c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
+  if (r < 0 || r >= (int32_t)c->reals.size()) return srs::fail(SRS_E_ARG, "realization %d out of range", r);
+  return SRS_OK;
+}
+
+int launch_step(srs_ctx* c, uint32_t gmode) {
+  hipLaunchKernelGGL(k_sweep, dim3(c->grid_sweep), dim3(TILE), 0, c->stream, c->d_desc, c->d_live, c->d_tiles,
+                     c->n_tiles);
+  hipLaunchKernelGGL(k_gate, dim3((unsigned)c->reals.size()), dim3(GATE_THREADS), 0, c->stream, c->d_desc,
+                     c->d_live, gmode);
+  return SRS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int srs_ctx_create(int device, srs_ctx** out) {
+  srs::clear_error();
+  if (!out) return srs::fail(SRS_E_ARG, "null out");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return srs::fail(SRS_E_NODEV, "no HIP device");
+  if (device < 0 || device >= n) return srs::fail(SRS_E_ARG, "device %d out of range (%d)", device, n);
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return srs::fail(SRS_E_NODEV, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+  HIPCHK(hipSetDevice(device));
+  auto c = new srs_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return srs::fail(SRS_E_HIP, "stream/event creation failed");
+  }
+  *out = c;
+  return SRS_OK;
+}
+
+int srs_ctx_destroy(srs_ctx* c) {
+  if (!c) return SRS_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (void* p : c->allocs) (void)hipFree(p);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SRS_OK;
+}
+
+int srs_ctx_add_mesh(srs_ctx* c, const srs_mesh_desc* m, int32_t* mesh_id) {
+  srs::clear_error();
+  if (!c || !m || !mesh_id) return srs::fail(SRS_E_ARG, "null argument");
+  if (c->finalized) return srs::fail(SRS_E_STATE, "context already finalized");
+  const int32_t n = m->n_ele;
+  if (n <= 0 || !m->aij || !m->nij || !m->domain || !m->tropc || !m->V0 || !m->tt_area)
+    return srs::fail(SRS_E_ARG, "mesh descriptor incomplete");
+  for (int64_t k = 0; k < 4LL * n; ++k)
+    if (m->nij[k] < 0 || m->nij[k] >= n) return srs::fail(SRS_E_ARG, "nij[%lld] = %d out of range", (long long)k, m->nij[k]);
+  auto hm = std::make_unique<HostMesh>();
+  hm->n = n;
+  hm->np = (n + TILE - 1) / TILE * TILE;
+  const int np = hm->np;
+  // element order inside the context
+  if (m->order == SRS_ORDER_RCM) {
+    srs::rcm_order(n, m->nij, hm->perm);
+  } else {
+    hm->perm.resize(n);
+    std::iota(hm->perm.begin(), hm->perm.end(), 0);
+  }
+  hm->inv.assign(n, 0);
+  for (int i = 0; i < n; ++i) hm->inv[hm->perm[i]] = i;
+  hm->nbr.assign(4 * (size_t)np, 0);
+  hm->coef.assign(5 * (size_t)np, 0.0);
+  hm->flags.assign(np, 0);
+  hm->V0.assign(np, 1.0);
+  hm->tt.assign(np, 0.0);
+  bool any_br = false;
+  if (m->br)
+    for (int s : {srs::CA, srs::DYE, srs::ATP, srs::CMDN})
+      for (int i = 0; i < n; ++i) any_br |= (m->br[(size_t)s * n + i] != 0.0);
+  if (any_br) hm->br.assign(4 * (size_t)np, 0.0);
+  for (int k = 0; k < np; ++k) {
+    if (k >= n) {  // padding rows point at themselves
+      for (int q = 0; q < 4; ++q) hm->nbr[(size_t)q * np + k] = k < n ? k : 0;
+      continue;
+    }
+    const int i = hm->perm[k];
+    for (int q = 0; q < 5; ++q) hm->coef[(size_t)q * np + k] = m->aij[5 * (size_t)i + q];
+    for (int q = 0; q < 4; ++q) hm->nbr[(size_t)q * np + k] = hm->inv[m->nij[4 * (size_t)i + q]];
+    uint8_t f = 0;
+    if (m->domain[i] != 0) f |= F_JSR;
+    if (m->tropc[i]) f |= F_TROPC;
+    if (m->tt_area[i] != 0.0) f |= F_SL;
+    hm->V0[k] = m->V0[i];
+    hm->tt[k] = m->tt_area[i];
+    if (any_br) {
+      const int sp[4] = {srs::CA, srs::DYE, srs::ATP, srs::CMDN};
+      bool nz = false;
+      for (int q = 0; q < 4; ++q) {
+        const double v = m->br[(size_t)sp[q] * n + i];
+        hm->br[(size_t)q * np + k] = v;
+        nz |= (v != 0.0);
+      }
+      if (nz) f |= F_BR;
+    }
+    hm->flags[k] = f;
+  }
+  *mesh_id = (int32_t)c->meshes.size();
+  c->meshes.push_back(std::move(hm));
+  return SRS_OK;
+}
+
+int srs_ctx_add_realization(srs_ctx* c, int32_t mesh_id, const srs_chan_desc* ch, const int32_t ints[SRS_N_INT],
+                            const double reals[SRS_N_REAL], uint64_t seed_key, uint64_t stream, int32_t* real_id) {
+  srs::clear_error();
+  if (!c || !ch || !ints || !reals || !real_id) return srs::fail(SRS_E_ARG, "null argument");
+  if (c->finalized) return srs::fail(SRS_E_STATE, "context already finalized");
+  if (mesh_id < 0 || mesh_id >= (int32_t)c->meshes.size()) return srs::fail(SRS_E_ARG, "bad mesh id %d", mesh_id);
+  HostMesh& hm = *c->meshes[mesh_id];
+  if (ch->n_ryr < 0 || ch->n_ryr > MAX_RYR || ch->n_lcc < 0 || ch->n_lcc > MAX_LCC)
+    return srs::fail(SRS_E_ARG, "channel counts %d/%d exceed %d/%d", ch->n_ryr, ch->n_lcc, MAX_RYR, MAX_LCC);
+  if (!(reals[SRS_R_T_STEP] > 0)) return srs::fail(SRS_E_ARG, "t_step must be > 0");
+  HostReal hr;
+  hr.mesh = mesh_id;
+  memcpy(hr.ints, ints, sizeof(hr.ints));
+  memcpy(hr.reals, reals, sizeof(hr.reals));
+  hr.key = seed_key;
+  hr.stream = stream;
+  std::vector<std::pair<int32_t, int32_t>> entries;  // (dev element, code) in reference order
+  for (int i = 0; i < ch->n_ryr; ++i) {
+    const int ce = ch->ryr_ele[i], je = ch->ryr_jsr[i];
+    if (ce < 0 || ce >= hm.n || je < 0 || je >= hm.n) return srs::fail(SRS_E_ARG, "RyR %d element out of range", i);
+    hr.ryr_ele.push_back(hm.inv[ce]);
+    hr.ryr_jsr.push_back(hm.inv[je]);
+    for (int q = 0; q < 4; ++q) {
+      const int nb = ch->ryr_nb ? ch->ryr_nb[4 * i + q] : -1;
+      if (nb >= ch->n_ryr || nb < -1) return srs::fail(SRS_E_ARG, "RyR %d neighbour %d out of range", i, nb);
+      hr.ryr_nb.push_back(nb);
+    }
+    entries.push_back({hm.inv[ce], 2 * i});
+    entries.push_back({hm.inv[je], 2 * i + 1});
+  }
+  for (int i = 0; i < ch->n_lcc; ++i) {
+    const int e = ch->lcc_ele[i];
+    if (e < 0 || e >= hm.n) return srs::fail(SRS_E_ARG, "LCC %d element out of range", i);
+    hr.lcc_ele.push_back(hm.inv[e]);
+    entries.push_back({hm.inv[e], -(i + 1)});
+  }
+  if (ints[SRS_I_INIT_RYR] != -1 && (ints[SRS_I_INIT_RYR] < 0 || ints[SRS_I_INIT_RYR] >= ch->n_ryr))
+    return srs::fail(SRS_E_ARG, "ryr_open_init %d must be -1 or in [0, %d)", ints[SRS_I_INIT_RYR], ch->n_ryr);
+  if (ints[SRS_I_FLAG_LCC] && (ints[SRS_I_LCC_INDEX] < 0 || ints[SRS_I_LCC_INDEX] >= ch->n_lcc))
+    return srs::fail(SRS_E_ARG, "LCC index %d out of range", ints[SRS_I_LCC_INDEX]);
+  std::stable_sort(entries.begin(), entries.end(),
+                   [](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) { return a.first < b.first; });
+  for (size_t k = 0; k < entries.size(); ++k) {
+    if (k == 0 || entries[k].first != entries[k - 1].first) {
+      hr.celem.push_back(entries[k].first);
+      hr.cstart.push_back((int32_t)k);
+      hm.flags[entries[k].first] |= F_CHAN;
+    }
+    hr.centry.push_back(entries[k].second);
+  }
+  hr.cstart.push_back((int32_t)entries.size());
+  *real_id = (int32_t)c->reals.size();
+  c->reals.push_back(std::move(hr));
+  return SRS_OK;
+}
+
+int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
+  srs::clear_error();
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  if (c->finalized) return srs::fail(SRS_E_STATE, "already finalized");
+  if (c->reals.empty()) return srs::fail(SRS_E_STATE, "no realizations");
+  HIPCHK(hipSetDevice(c->device));
+  int rc;
+  for (auto& hmp : c->meshes) {
+    HostMesh& hm = *hmp;
+    if ((rc = dev_upload(c, &hm.d_nbr, hm.nbr))) return rc;
+    if ((rc = dev_upload(c, &hm.d_coef, hm.coef))) return rc;
+    if ((rc = dev_upload(c, &hm.d_flags, hm.flags))) return rc;
+    if ((rc = dev_upload(c, &hm.d_V0, hm.V0))) return rc;
+    if ((rc = dev_upload(c, &hm.d_tt, hm.tt))) return rc;
+    if (!hm.br.empty() && (rc = dev_upload(c, &hm.d_br, hm.br))) return rc;
+  }
+  std::vector<RealDesc> descs;
+  std::vector<RealLive> lives;
+  std::vector<int2> tiles;
+  for (size_t r = 0; r < c->reals.size(); ++r) {
+    HostReal& hr = c->reals[r];
+    HostMesh& hm = *c->meshes[hr.mesh];
+    RealDesc d{};
+    d.nbr = hm.d_nbr; d.coef = hm.d_coef; d.flags = hm.d_flags; d.V0 = hm.d_V0; d.tt = hm.d_tt; d.br = hm.d_br;
+    d.n = hm.n; d.np = hm.np;
+    double *x0, *x1;
+    if ((rc = dev_alloc(c, &x0, 6 * (size_t)hm.np)) || (rc = dev_alloc(c, &x1, 6 * (size_t)hm.np))) return rc;
+    HIPCHK(hipMemset(x0, 0, 6 * (size_t)hm.np * sizeof(double)));
+    HIPCHK(hipMemset(x1, 0, 6 * (size_t)hm.np * sizeof(double)));
+    d.X[0] = x0; d.X[1] = x1;
+    d.n_ryr = (int32_t)hr.ryr_ele.size();
+    d.n_lcc = (int32_t)hr.lcc_ele.size();
+    int32_t *p;
+    if ((rc = dev_upload(c, &p, hr.ryr_ele))) return rc; d.ryr_ele = p;
+    if ((rc = dev_upload(c, &p, hr.ryr_jsr))) return rc; d.ryr_jsr = p;
+    if ((rc = dev_upload(c, &p, hr.ryr_nb))) return rc; d.ryr_nb = p;
+    if ((rc = dev_upload(c, &p, hr.lcc_ele))) return rc; d.lcc_ele = p;
+    if ((rc = dev_upload(c, &p, hr.celem))) return rc; d.celem = p;
+    if ((rc = dev_upload(c, &p, hr.cstart))) return rc; d.cstart = p;
+    if ((rc = dev_upload(c, &p, hr.centry))) return rc; d.centry = p;
+    d.n_celem = (int32_t)hr.celem.size();
+    std::vector<int32_t> zeros_r(d.n_ryr, 0), ones_l(d.n_lcc, 1), twos_l(d.n_lcc, 2);
+    if ((rc = dev_upload(c, &p, zeros_r))) return rc; d.ryr_state = p;
+    if ((rc = dev_upload(c, &p, ones_l))) return rc; d.lcc_state = p;
+    if ((rc = dev_upload(c, &p, twos_l))) return rc; d.lccv_state = p;
+    std::vector<double> zr(d.n_ryr + d.n_lcc, -1.0);
+    double* pr;
+    if ((rc = dev_upload(c, &pr, zr))) return rc; d.R = pr;
+    d.ryr_out = hr.ints[SRS_I_FLAG_RYR_OUT] ? 1 : 0;
+    d.gpu_layout = 0;
+    d.rec_len = 7 + (d.ryr_out ? d.n_ryr : 0);
+    d.rec_cap = record_capacity;
+    if (record_capacity > 0) {
+      float* fr;
+      if ((rc = dev_alloc(c, &fr, (size_t)record_capacity * d.rec_len))) return rc;
+      d.rec = fr;
+    }
+    d.key = hr.key;
+    d.stream = hr.stream;
+    memcpy(d.P, hr.reals, sizeof(d.P));
+    memcpy(d.I, hr.ints, sizeof(d.I));
+    hr.desc = d;
+    descs.push_back(d);
+    RealLive lv{};
+    lv.vm = hr.reals[SRS_R_DEFAULT_VM];
+    lv.ca_nsr = hr.reals[SRS_R_DEFAULT_CA_NSR];
+    lives.push_back(lv);
+    const int nt = (hm.n + TILE - 1) / TILE;
+    for (int t = 0; t < nt; ++t) tiles.push_back(make_int2((int)r, t));
+  }
+  c->n_tiles = (int)tiles.size();
+  c->grid_sweep = (c->n_tiles + NXCD - 1) / NXCD * NXCD;
+  if ((rc = dev_upload(c, &c->d_desc, descs))) return rc;
+  if ((rc = dev_upload(c, &c->d_live, lives))) return rc;
+  if ((rc = dev_upload(c, &c->d_tiles, tiles))) return rc;
+  // meshes' channel flags were updated by add_realization; refresh them
+  for (auto& hmp : c->meshes)
+    HIPCHK(hipMemcpy(hmp->d_flags, hmp->flags.data(), hmp->flags.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipDeviceSynchronize());
+  c->finalized = true;
+  return SRS_OK;
+}
+
+int srs_state_set(srs_ctx* c, int32_t r, const double* X) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  if (!X) return srs::fail(SRS_E_ARG, "null state");
+  const HostReal& hr = c->reals[r];
+  const HostMesh& hm = *c->meshes[hr.mesh];
+  std::vector<double> buf(6 * (size_t)hm.np, 0.0);
+  for (int s = 0; s < 6; ++s)
+    for (int i = 0; i < hm.n; ++i) buf[(size_t)s * hm.np + hm.inv[i]] = X[(size_t)s * hm.n + i];
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(hr.desc.X[0], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(hr.desc.X[1], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
+  return SRS_OK;
+}
+
+int srs_state_get(srs_ctx* c, int32_t r, double* X) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  if (!X) return srs::fail(SRS_E_ARG, "null state");
+  const HostReal& hr = c->reals[r];
+  const HostMesh& hm = *c->meshes[hr.mesh];
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  RealLive lv;
+  HIPCHK(hipMemcpy(&lv, c->d_live + r, sizeof(lv), hipMemcpyDeviceToHost));
+  std::vector<double> buf(6 * (size_t)hm.np);
+  HIPCHK(hipMemcpy(buf.data(), hr.desc.X[lv.cur], buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int s = 0; s < 6; ++s)
+    for (int i = 0; i < hm.n; ++i) X[(size_t)s * hm.n + i] = buf[(size_t)s * hm.np + hm.inv[i]];
+  return SRS_OK;
+}
+
+int srs_channels_set(srs_ctx* c, int32_t r, const int32_t* ryr_state, const int32_t* lcc_state,
+                     const int32_t* lccv_state, const double* R) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  const RealDesc& d = c->reals[r].desc;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (ryr_state && d.n_ryr) HIPCHK(hipMemcpy(d.ryr_state, ryr_state, d.n_ryr * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (lcc_state && d.n_lcc) HIPCHK(hipMemcpy(d.lcc_state, lcc_state, d.n_lcc * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (lccv_state && d.n_lcc) HIPCHK(hipMemcpy(d.lccv_state, lccv_state, d.n_lcc * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (R && d.n_ryr + d.n_lcc) HIPCHK(hipMemcpy(d.R, R, (d.n_ryr + d.n_lcc) * sizeof(double), hipMemcpyHostToDevice));
+  return SRS_OK;
+}
+
+int srs_channels_get(srs_ctx* c, int32_t r, int32_t* ryr_state, int32_t* lcc_state, int32_t* lccv_state, double* R) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  const RealDesc& d = c->reals[r].desc;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (ryr_state && d.n_ryr) HIPCHK(hipMemcpy(ryr_state, d.ryr_state, d.n_ryr * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (lcc_state && d.n_lcc) HIPCHK(hipMemcpy(lcc_state, d.lcc_state, d.n_lcc * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (lccv_state && d.n_lcc) HIPCHK(hipMemcpy(lccv_state, d.lccv_state, d.n_lcc * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (R && d.n_ryr + d.n_lcc) HIPCHK(hipMemcpy(R, d.R, (d.n_ryr + d.n_lcc) * sizeof(double), hipMemcpyDeviceToHost));
+  return SRS_OK;
+}
+
+static int live_rw(srs_ctx* c, int32_t r, RealLive* lv, bool write) {
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (write) HIPCHK(hipMemcpy(c->d_live + r, lv, sizeof(*lv), hipMemcpyHostToDevice));
+  else HIPCHK(hipMemcpy(lv, c->d_live + r, sizeof(*lv), hipMemcpyDeviceToHost));
+  return SRS_OK;
+}
+
+int srs_globals_set(srs_ctx* c, int32_t r, double vm, double ca_nsr) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  RealLive lv;
+  if ((rc = live_rw(c, r, &lv, false))) return rc;
+  lv.vm = vm;
+  lv.ca_nsr = ca_nsr;
+  return live_rw(c, r, &lv, true);
+}
+
+int srs_globals_get(srs_ctx* c, int32_t r, double* vm, double* ca_nsr) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  RealLive lv;
+  if ((rc = live_rw(c, r, &lv, false))) return rc;
+  if (vm) *vm = lv.vm;
+  if (ca_nsr) *ca_nsr = lv.ca_nsr;
+  return SRS_OK;
+}
+
+int srs_loop_set(srs_ctx* c, int32_t r, const srs_loop_state* ls) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  if (!ls) return srs::fail(SRS_E_ARG, "null loop state");
+  RealLive lv;
+  if ((rc = live_rw(c, r, &lv, false))) return rc;
+  lv.t = ls->t; lv.dt_last = ls->dt_last; lv.step = ls->step; lv.done = ls->done; lv.lcc_open = ls->lcc_open;
+  lv.t_lcc_close = ls->t_lcc_close; lv.n_open_last = ls->n_ryr_open_last; lv.ca_max_last = ls->ca_ss_max_last;
+  return live_rw(c, r, &lv, true);
+}
+
+int srs_loop_get(srs_ctx* c, int32_t r, srs_loop_state* ls) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  if (!ls) return srs::fail(SRS_E_ARG, "null loop state");
+  RealLive lv;
+  if ((rc = live_rw(c, r, &lv, false))) return rc;
+  ls->t = lv.t; ls->dt_last = lv.dt_last; ls->step = lv.step; ls->done = lv.done; ls->lcc_open = lv.lcc_open;
+  ls->t_lcc_close = lv.t_lcc_close; ls->n_ryr_open_last = lv.n_open_last; ls->ca_ss_max_last = lv.ca_max_last;
+  return SRS_OK;
+}
+
+int srs_explicit_step(srs_ctx* c, int64_t n_steps, uint32_t flags) {
+  srs::clear_error();
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  if (!c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
+  if (n_steps < 0) return srs::fail(SRS_E_ARG, "negative step count");
+  HIPCHK(hipSetDevice(c->device));
+  const uint32_t gmode = G_AFTER_SWEEP | ((flags & SRS_STEP_PROTOCOL) ? G_PROTOCOL : 0u);
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  int64_t done = 0;
+  if ((flags & SRS_STEP_GRAPH) && n_steps >= 16) {
+    const int64_t block = std::min<int64_t>(n_steps, 128);
+    auto key = std::make_pair(block, gmode);
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (int64_t k = 0; k < block; ++k) launch_step(c, gmode);
+      HIPCHK(hipStreamEndCapture(c->stream, &g));
+      hipGraphExec_t ge;
+      HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(g));
+      it = c->graphs.emplace(key, ge).first;
+    }
+    while (n_steps - done >= block) {
+      HIPCHK(hipGraphLaunch(it->second, c->stream));
+      done += block;
+    }
+  }
+  for (; done < n_steps; ++done) launch_step(c, gmode);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_total_ms = ms;
+  return SRS_OK;
+}
+
+int srs_channel_gating(srs_ctx* c) {
+  srs::clear_error();
+  if (!c || !c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
+  HIPCHK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_gate, dim3((unsigned)c->reals.size()), dim3(GATE_THREADS), 0, c->stream, c->d_desc, c->d_live,
+                     0u);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SRS_OK;
+}
+
+int srs_explicit_step_replay(srs_ctx* c, int32_t r, int64_t n_steps, const int32_t* schedule) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  if (c->reals.size() != 1) return srs::fail(SRS_E_ARG, "replay needs a single-realization context");
+  HostReal& hr = c->reals[r];
+  const int nr = (int)hr.ryr_ele.size();
+  if (n_steps <= 0) return SRS_OK;
+  if (!schedule && nr) return srs::fail(SRS_E_ARG, "null schedule");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int32_t* ds = nullptr;
+  if (nr) {
+    HIPCHK(hipMalloc(&ds, (size_t)n_steps * nr * sizeof(int32_t)));
+    HIPCHK(hipMemcpy(ds, schedule, (size_t)n_steps * nr * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(hr.desc.ryr_state, schedule, nr * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  RealDesc d = hr.desc;
+  d.sched = ds;
+  d.n_sched = n_steps;
+  HIPCHK(hipMemcpy(c->d_desc + r, &d, sizeof(d), hipMemcpyHostToDevice));
+  RealLive lv;
+  HIPCHK(hipMemcpy(&lv, c->d_live + r, sizeof(lv), hipMemcpyDeviceToHost));
+  lv.sched_base = lv.step;
+  HIPCHK(hipMemcpy(c->d_live + r, &lv, sizeof(lv), hipMemcpyHostToDevice));
+  const uint32_t gmode = G_AFTER_SWEEP | G_NO_GATING | G_REPLAY;
+  for (int64_t k = 0; k < n_steps; ++k) launch_step(c, gmode);
+  hipError_t e = hipGetLastError();
+  (void)hipStreamSynchronize(c->stream);
+  HIPCHK(hipMemcpy(c->d_desc + r, &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice));
+  if (ds) (void)hipFree(ds);
+  if (e != hipSuccess) return srs::fail(SRS_E_HIP, "replay launch: %s", hipGetErrorString(e));
+  return SRS_OK;
+}
+
+int srs_channel_ca(srs_ctx* c, int32_t r, double* ca_ryr_cyt, double* ca_ryr_jsr, double* ca_lcc) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  const RealDesc& d = c->reals[r].desc;
+  const int n = 2 * d.n_ryr + d.n_lcc;
+  if (n == 0) return SRS_OK;
+  HIPCHK(hipSetDevice(c->device));
+  double* tmp;
+  HIPCHK(hipMalloc(&tmp, n * sizeof(double)));
+  hipLaunchKernelGGL(k_channel_ca, dim3(1), dim3(256), 0, c->stream, c->d_desc, c->d_live, r, tmp);
+  std::vector<double> h(n);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), tmp, n * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return srs::fail(SRS_E_HIP, "channel Ca: %s", hipGetErrorString(e));
+  if (ca_ryr_cyt) memcpy(ca_ryr_cyt, h.data(), d.n_ryr * sizeof(double));
+  if (ca_ryr_jsr) memcpy(ca_ryr_jsr, h.data() + d.n_ryr, d.n_ryr * sizeof(double));
+  if (ca_lcc) memcpy(ca_lcc, h.data() + 2 * d.n_ryr, d.n_lcc * sizeof(double));
+  return SRS_OK;
+}
+
+int srs_records_read(srs_ctx* c, int32_t r, float* out, int32_t max_records, int32_t* n_records, int32_t* rec_len) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  const RealDesc& d = c->reals[r].desc;
+  RealLive lv;
+  if ((rc = live_rw(c, r, &lv, false))) return rc;
+  if (rec_len) *rec_len = d.rec_len;
+  if (lv.rec_overflow) return srs::fail(SRS_E_STATE, "device record buffer overflowed (capacity %d)", d.rec_cap);
+  const int n = std::min(lv.rec_count, max_records);
+  if (n > 0 && out) HIPCHK(hipMemcpy(out, d.rec, (size_t)n * d.rec_len * sizeof(float), hipMemcpyDeviceToHost));
+  if (n_records) *n_records = n;
+  // keep any records that did not fit
+  const int rest = lv.rec_count - n;
+  if (n == 0) return SRS_OK;
+  if (rest > 0) {
+    std::vector<float> tail((size_t)rest * d.rec_len);
+    HIPCHK(hipMemcpy(tail.data(), d.rec + (size_t)n * d.rec_len, tail.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(d.rec, tail.data(), tail.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  lv.rec_count = rest;
+  return live_rw(c, r, &lv, true);
+}
+
+int srs_ctx_sync(srs_ctx* c) {
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SRS_OK;
+}
+
+int srs_real_info(srs_ctx* c, int32_t r, int32_t* n_ele, int32_t* n_ryr, int32_t* n_lcc) {
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  if (r < 0 || r >= (int32_t)c->reals.size()) return srs::fail(SRS_E_ARG, "realization %d out of range", r);
+  const HostReal& hr = c->reals[r];
+  if (n_ele) *n_ele = c->meshes[hr.mesh]->n;
+  if (n_ryr) *n_ryr = (int32_t)hr.ryr_ele.size();
+  if (n_lcc) *n_lcc = (int32_t)hr.lcc_ele.size();
+  return SRS_OK;
+}
+
+int srs_last_timing(srs_ctx* c, double* total_ms, double* sweep_ms) {
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  if (total_ms) *total_ms = c->last_total_ms;
+  if (sweep_ms) *sweep_ms = c->last_sweep_ms;
+  return SRS_OK;
+}
+
+int srs_time_sweep(srs_ctx* c, int32_t reps, double* sweep_ms, double* gating_ms) {
+  // Real steps (sweep + gating, state advances), with HIP events on the
+  // context's stream bracketing each kernel separately.
+  srs::clear_error();
+  if (!c || !c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
+  if (reps <= 0) return srs::fail(SRS_E_ARG, "reps must be > 0");
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<hipEvent_t> ev(3 * reps);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  const uint32_t gmode = G_AFTER_SWEEP;
+  for (int k = 0; k < reps; ++k) {
+    HIPCHK(hipEventRecord(ev[3 * k], c->stream));
+    hipLaunchKernelGGL(k_sweep, dim3(c->grid_sweep), dim3(TILE), 0, c->stream, c->d_desc, c->d_live, c->d_tiles,
+                       c->n_tiles);
+    HIPCHK(hipEventRecord(ev[3 * k + 1], c->stream));
+    hipLaunchKernelGGL(k_gate, dim3((unsigned)c->reals.size()), dim3(GATE_THREADS), 0, c->stream, c->d_desc,
+                       c->d_live, gmode);
+    HIPCHK(hipEventRecord(ev[3 * k + 2], c->stream));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double s = 0, g = 0;
+  for (int k = 0; k < reps; ++k) {
+    float a = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]));
+    HIPCHK(hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]));
+    s += a;
+    g += b;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  c->last_sweep_ms = s / reps;
+  if (sweep_ms) *sweep_ms = s / reps;
+  if (gating_ms) *gating_ms = g / reps;
+  return SRS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,293 @@
+// host_params.cpp -- the simulation parameter file reader.
+//
+// Replaces ReadParameters (cpu/InitFuncs.cpp:25-381), which parses the file
+// with xerces and collects every element named "parameter", whatever its
+// depth, then the text content of its <symbol> and <value> children.  This
+// reader implements the subset of XML those files use (elements, attributes,
+// comments, processing instructions, CDATA, character and the five named
+// entities) without a library dependency.
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "srs_internal.h"
+
+namespace {
+
+struct Elem {
+  std::string name;
+  std::vector<Elem> kids;
+  std::string text;  // all descendant text, in document order (getTextContent)
+};
+
+struct Parser {
+  const std::string& s;
+  size_t p = 0;
+  std::string err;
+  explicit Parser(const std::string& src) : s(src) {}
+
+  bool starts(const char* lit) const { return s.compare(p, strlen(lit), lit) == 0; }
+
+  static void append_utf8(std::string& out, unsigned long cp) {
+    if (cp < 0x80) out += (char)cp;
+    else if (cp < 0x800) { out += (char)(0xC0 | (cp >> 6)); out += (char)(0x80 | (cp & 0x3F)); }
+    else if (cp < 0x10000) { out += (char)(0xE0 | (cp >> 12)); out += (char)(0x80 | ((cp >> 6) & 0x3F)); out += (char)(0x80 | (cp & 0x3F)); }
+    else { out += (char)(0xF0 | (cp >> 18)); out += (char)(0x80 | ((cp >> 12) & 0x3F)); out += (char)(0x80 | ((cp >> 6) & 0x3F)); out += (char)(0x80 | (cp & 0x3F)); }
+  }
+
+  bool decode(const std::string& raw, std::string& out) {
+    // XML end-of-line handling: CR LF and lone CR become LF.
+    for (size_t i = 0; i < raw.size(); ++i) {
+      char ch = raw[i];
+      if (ch == '\r') { out += '\n'; if (i + 1 < raw.size() && raw[i + 1] == '\n') ++i; continue; }
+      if (ch != '&') { out += ch; continue; }
+      size_t e = raw.find(';', i);
+      if (e == std::string::npos) { err = "unterminated entity"; return false; }
+      std::string ent = raw.substr(i + 1, e - i - 1);
+      if (ent == "amp") out += '&';
+      else if (ent == "lt") out += '<';
+      else if (ent == "gt") out += '>';
+      else if (ent == "quot") out += '"';
+      else if (ent == "apos") out += '\'';
+      else if (!ent.empty() && ent[0] == '#') {
+        unsigned long cp = (ent.size() > 1 && (ent[1] == 'x' || ent[1] == 'X')) ? strtoul(ent.c_str() + 2, nullptr, 16)
+                                                                                 : strtoul(ent.c_str() + 1, nullptr, 10);
+        append_utf8(out, cp);
+      } else { err = "unknown entity &" + ent + ";"; return false; }
+      i = e;
+    }
+    return true;
+  }
+
+  bool skip_misc() {
+    for (;;) {
+      while (p < s.size() && isspace((unsigned char)s[p])) ++p;
+      if (starts("<?")) { size_t e = s.find("?>", p); if (e == std::string::npos) { err = "unterminated PI"; return false; } p = e + 2; }
+      else if (starts("<!--")) { size_t e = s.find("-->", p); if (e == std::string::npos) { err = "unterminated comment"; return false; } p = e + 3; }
+      else if (starts("<!DOCTYPE")) { size_t e = s.find('>', p); if (e == std::string::npos) { err = "bad DOCTYPE"; return false; } p = e + 1; }
+      else return true;
+    }
+  }
+
+  bool element(Elem& el) {
+    if (p >= s.size() || s[p] != '<') { err = "expected element"; return false; }
+    ++p;
+    size_t b = p;
+    while (p < s.size() && !isspace((unsigned char)s[p]) && s[p] != '>' && s[p] != '/') ++p;
+    el.name = s.substr(b, p - b);
+    if (el.name.empty()) { err = "empty element name"; return false; }
+    // attributes are skipped (quoted values may contain '>')
+    while (p < s.size() && s[p] != '>' && !(s[p] == '/' && p + 1 < s.size() && s[p + 1] == '>')) {
+      if (s[p] == '"' || s[p] == '\'') { char q = s[p++]; while (p < s.size() && s[p] != q) ++p; }
+      ++p;
+    }
+    if (p >= s.size()) { err = "unterminated start tag"; return false; }
+    if (s[p] == '/') { p += 2; return true; }
+    ++p;
+    for (;;) {
+      if (p >= s.size()) { err = "unterminated element " + el.name; return false; }
+      if (starts("</")) {
+        p += 2;
+        size_t e = s.find('>', p);
+        if (e == std::string::npos) { err = "bad end tag"; return false; }
+        std::string nm = s.substr(p, e - p);
+        while (!nm.empty() && isspace((unsigned char)nm.back())) nm.pop_back();
+        if (nm != el.name) { err = "mismatched end tag " + nm + " for " + el.name; return false; }
+        p = e + 1;
+        return true;
+      }
+      if (starts("<!--")) { size_t e = s.find("-->", p); if (e == std::string::npos) { err = "unterminated comment"; return false; } p = e + 3; continue; }
+      if (starts("<?")) { size_t e = s.find("?>", p); if (e == std::string::npos) { err = "unterminated PI"; return false; } p = e + 2; continue; }
+      if (starts("<![CDATA[")) {
+        size_t e = s.find("]]>", p);
+        if (e == std::string::npos) { err = "unterminated CDATA"; return false; }
+        el.text += s.substr(p + 9, e - p - 9);
+        p = e + 3;
+        continue;
+      }
+      if (s[p] == '<') {
+        el.kids.emplace_back();
+        if (!element(el.kids.back())) return false;
+        el.text += el.kids.back().text;
+        continue;
+      }
+      size_t e = s.find('<', p);
+      if (e == std::string::npos) e = s.size();
+      if (!decode(s.substr(p, e - p), el.text)) return false;
+      p = e;
+    }
+  }
+};
+
+void collect(const Elem& e, std::vector<const Elem*>& out) {
+  if (e.name == "parameter") out.push_back(&e);
+  for (const Elem& k : e.kids) collect(k, out);
+}
+
+void set_defaults(int32_t* I, double* R, char* title) {
+  // cpu/InitFuncs.cpp:28-108
+  for (int k = 0; k < SRS_N_INT; ++k) I[k] = 0;
+  for (int k = 0; k < SRS_N_REAL; ++k) R[k] = 0;
+  I[SRS_I_OUTPUT_GRID_FLAG] = 0;
+  I[SRS_I_FLAG_FIDELITY_SIM] = 0;
+  I[SRS_I_FLAG_CLOCK_SEED] = 1;
+  I[SRS_I_RAND_SEED_0] = 38376427;
+  I[SRS_I_RAND_ADD_A] = 12783;
+  I[SRS_I_RAND_ADD_B] = 83;
+  I[SRS_I_INITIALIZE_FROM_FILE] = 0;
+  I[SRS_I_CUDA_DEVICE] = 0;
+  I[SRS_I_FLAG_SECONDARY_GRADIENT] = 0;
+  I[SRS_I_FLAG_NO_FLUX_BOUNDARY] = 1;
+  I[SRS_I_N_RYR_FIDELITY] = 1000;
+  I[SRS_I_FLAG_LCC] = 0;
+  I[SRS_I_LCC_INDEX] = 0;
+  I[SRS_I_FLAG_V_CLAMP] = 0;
+  I[SRS_I_FLAG_NO_GATING] = 0;
+  I[SRS_I_GATING_FFWD] = 0;
+  I[SRS_I_FLAG_RYR_OUT] = 0;
+  I[SRS_I_N_SIMS] = 1;
+  I[SRS_I_INIT_RYR] = -1;
+  R[SRS_R_T_FINAL] = 500;
+  R[SRS_R_T_STEP] = 12e-6;
+  R[SRS_R_OUTPUT_STATES_INTERVAL] = 0.01;
+  R[SRS_R_OUTPUT_GRID_INTERVAL] = 1;
+  R[SRS_R_C_MIN] = 1.0;
+  R[SRS_R_D_CA] = 0.25;
+  R[SRS_R_D_CA_JSR] = 0.25;
+  R[SRS_R_D_ATP] = 0.14;
+  R[SRS_R_D_CMDN] = 0.025;
+  R[SRS_R_D_TRPN] = 0;
+  R[SRS_R_D_CSQN] = 0;
+  R[SRS_R_D_DYE] = 0.042;
+  R[SRS_R_D_EGTA] = 0;
+  R[SRS_R_C_0_CA] = 0.1;
+  R[SRS_R_C_0_CA_JSR] = 1000;
+  R[SRS_R_DEFAULT_VM] = -80;
+  R[SRS_R_DEFAULT_CA_NSR] = 1000;
+  R[SRS_R_CA_0] = 2000;
+  R[SRS_R_B_TOT_ATP] = 455;
+  R[SRS_R_K_OFF_ATP] = 45;
+  R[SRS_R_K_ON_ATP] = 0.225;
+  R[SRS_R_B_TOT_CMDN] = 24;
+  R[SRS_R_K_OFF_CMDN] = 0.238;
+  R[SRS_R_K_ON_CMDN] = 0.034;
+  R[SRS_R_B_TOT_TRPN] = 70;
+  R[SRS_R_K_OFF_TRPN] = 0.02;
+  R[SRS_R_K_ON_TRPN] = 0.039;
+  R[SRS_R_B_TOT_SL_HI] = 1.6e-13;
+  R[SRS_R_K_D_SL_HI] = 13;
+  R[SRS_R_B_TOT_SL_LO] = 0;
+  R[SRS_R_K_D_SL_LO] = 1100;
+  R[SRS_R_B_TOT_DYE] = 50;
+  R[SRS_R_K_OFF_DYE] = 0.110;
+  R[SRS_R_K_ON_DYE] = 0.10;
+  R[SRS_R_B_T_JSR] = 30000;
+  R[SRS_R_K_M_JSR] = 638;
+  R[SRS_R_V_REFILL] = 0.095;
+  R[SRS_R_V_RYR] = 3e-8;
+  R[SRS_R_V_CELL] = 25.84e3;
+  R[SRS_R_K_ON_CSQN] = 0.10;
+  R[SRS_R_K_OFF_CSQN] = 63.8;
+  R[SRS_R_RYR_ETA] = 2.1;
+  R[SRS_R_RYR_A_STAR] = 0;
+  R[SRS_R_RYR_EPS_CC] = -0.92;
+  R[SRS_R_RYR_EPS_OO] = -0.85;
+  R[SRS_R_RYR_K_PLUS] = 0.1107e-3;
+  R[SRS_R_RYR_K_MINUS] = 0.500;
+  R[SRS_R_RYR_PHI_M] = 4.0;
+  R[SRS_R_RYR_PHI_B] = 1500;
+  R[SRS_R_K_D_I] = 910;
+  R[SRS_R_K_D_SR] = 2240;
+  R[SRS_R_A_P] = 150;
+  R[SRS_R_LCC_DURATION] = 0.5;
+  R[SRS_R_T_CLAMP] = 200;
+  R[SRS_R_V_CLAMP] = 0;
+  R[SRS_R_INIT_RYR_HOLD] = 0;
+  if (title) snprintf(title, SRS_TITLE_LEN, "Untitled");
+}
+
+// Symbol table of cpu/InitFuncs.cpp:191-349.
+enum Kind { FLAG, INT, REAL, REAL_NS, REAL_SR, PROTOCOL, INIT_RYR, TITLE };
+struct Sym { const char* name; Kind kind; int idx; };
+const Sym kSyms[] = {
+    {"flag_noflux", FLAG, SRS_I_FLAG_NO_FLUX_BOUNDARY}, {"flag_nogating", FLAG, SRS_I_FLAG_NO_GATING},
+    {"flag_ffwd", FLAG, SRS_I_GATING_FFWD}, {"flag_use_clock", FLAG, SRS_I_FLAG_CLOCK_SEED},
+    {"flag_output_grid", FLAG, SRS_I_OUTPUT_GRID_FLAG}, {"protocol", PROTOCOL, 0},
+    {"flag_output_ryr_states", FLAG, SRS_I_FLAG_RYR_OUT}, {"sims_per_proc", INT, SRS_I_N_SIMS},
+    {"ryr_open_init", INIT_RYR, SRS_I_INIT_RYR}, {"c_min", REAL, SRS_R_C_MIN},
+    {"ryr_open_time", REAL, SRS_R_INIT_RYR_HOLD}, {"ryr_open_max", INT, SRS_I_N_RYR_FIDELITY},
+    {"v_clamp", REAL, SRS_R_V_CLAMP}, {"t_clamp", REAL, SRS_R_T_CLAMP}, {"t_final", REAL, SRS_R_T_FINAL},
+    {"t_step", REAL_NS, SRS_R_T_STEP}, {"states_interval", REAL, SRS_R_OUTPUT_STATES_INTERVAL},
+    {"grid_interval", REAL, SRS_R_OUTPUT_GRID_INTERVAL}, {"seed_base", INT, SRS_I_RAND_SEED_0},
+    {"seed_A", INT, SRS_I_RAND_ADD_A}, {"seed_B", INT, SRS_I_RAND_ADD_B}, {"CA_I", REAL, SRS_R_C_0_CA},
+    {"CA_SR", REAL_SR, 0}, {"CA_O", REAL, SRS_R_CA_0}, {"V_M", REAL, SRS_R_DEFAULT_VM},
+    {"D_CA", REAL, SRS_R_D_CA}, {"D_CAJSR", REAL, SRS_R_D_CA_JSR}, {"D_ATP", REAL, SRS_R_D_ATP},
+    {"D_CMDN", REAL, SRS_R_D_CMDN}, {"D_DYE", REAL, SRS_R_D_DYE}, {"B_TOT_ATP", REAL, SRS_R_B_TOT_ATP},
+    {"K_OFF_ATP", REAL, SRS_R_K_OFF_ATP}, {"K_ON_ATP", REAL, SRS_R_K_ON_ATP},
+    {"B_TOT_CMDN", REAL, SRS_R_B_TOT_CMDN}, {"K_OFF_CMDN", REAL, SRS_R_K_OFF_CMDN},
+    {"K_ON_CMDN", REAL, SRS_R_K_ON_CMDN}, {"B_TOT_TRPN", REAL, SRS_R_B_TOT_TRPN},
+    {"K_OFF_TRPN", REAL, SRS_R_K_OFF_TRPN}, {"K_ON_TRPN", REAL, SRS_R_K_ON_TRPN},
+    {"B_TOT_SL", REAL, SRS_R_B_TOT_SL_HI}, {"K_D_SL", REAL, SRS_R_K_D_SL_HI},
+    {"B_TOT_DYE", REAL, SRS_R_B_TOT_DYE}, {"K_OFF_DYE", REAL, SRS_R_K_OFF_DYE},
+    {"K_ON_DYE", REAL, SRS_R_K_ON_DYE}, {"B_TOT_CSQN", REAL, SRS_R_B_T_JSR},
+    {"K_OFF_CSQN", REAL, SRS_R_K_OFF_CSQN}, {"K_ON_CSQN", REAL, SRS_R_K_ON_CSQN},
+    {"V_REFILL", REAL, SRS_R_V_REFILL}, {"V_RYR", REAL, SRS_R_V_RYR}, {"RYR_ETA", REAL, SRS_R_RYR_ETA},
+    {"RYR_A_STAR", REAL, SRS_R_RYR_A_STAR}, {"RYR_EPS_CC", REAL, SRS_R_RYR_EPS_CC},
+    {"RYR_EPS_OO", REAL, SRS_R_RYR_EPS_OO}, {"RYR_K_MINUS", REAL, SRS_R_RYR_K_MINUS},
+    {"RYR_K_PLUS", REAL, SRS_R_RYR_K_PLUS}, {"RYR_PHI_N", REAL, SRS_R_RYR_PHI_M},
+    {"RYR_PHI_KD", REAL, SRS_R_RYR_PHI_B}, {"K_D_I", REAL, SRS_R_K_D_I}, {"K_D_SR", REAL, SRS_R_K_D_SR},
+    {"A_P", REAL, SRS_R_A_P}, {"paramset_title", TITLE, 0},
+};
+
+}  // namespace
+
+extern "C" int srs_read_parameters(const char* path, int32_t ints[SRS_N_INT], double reals[SRS_N_REAL],
+                                   char title[SRS_TITLE_LEN]) {
+  srs::clear_error();
+  if (!ints || !reals) return srs::fail(SRS_E_ARG, "null output vector");
+  set_defaults(ints, reals, title);
+  if (!path) return SRS_OK;
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return srs::fail(SRS_E_IO, "cannot open parameter file %s", path);
+  std::stringstream ss;
+  ss << in.rdbuf();
+  const std::string src = ss.str();
+  Parser ps(src);
+  Elem root;
+  if (!ps.skip_misc() || !ps.element(root)) return srs::fail(SRS_E_IO, "%s: XML error: %s", path, ps.err.c_str());
+  std::vector<const Elem*> params;
+  collect(root, params);
+  if (params.empty()) return srs::fail(SRS_E_IO, "%s: No parameters were found!", path);
+  for (const Elem* pe : params) {
+    const std::string* symbol = nullptr;
+    const std::string* value = nullptr;
+    for (const Elem& k : pe->kids) {
+      if (k.name == "symbol") symbol = &k.text;
+      else if (k.name == "value") value = &k.text;
+    }
+    if (!symbol || !value) continue;  // the reference would reuse stale pointers here
+    const char* v = value->c_str();
+    for (const Sym& sy : kSyms) {
+      if (*symbol != sy.name) continue;
+      switch (sy.kind) {
+        case FLAG: ints[sy.idx] = strcmp(v, "true") == 0 ? 1 : 0; break;
+        case INT: ints[sy.idx] = atoi(v); break;
+        case REAL: reals[sy.idx] = atof(v); break;
+        case REAL_NS: reals[sy.idx] = atof(v) * (1e-6); break;
+        case REAL_SR: reals[SRS_R_C_0_CA_JSR] = atof(v); reals[SRS_R_DEFAULT_CA_NSR] = atof(v); break;
+        case PROTOCOL:
+          if (strcmp(v, "gain") == 0) { ints[SRS_I_FLAG_V_CLAMP] = 1; ints[SRS_I_FLAG_FIDELITY_SIM] = 0; }
+          else if (strcmp(v, "fidelity") == 0) { ints[SRS_I_FLAG_V_CLAMP] = 0; ints[SRS_I_FLAG_FIDELITY_SIM] = 1; }
+          else { ints[SRS_I_FLAG_V_CLAMP] = 0; ints[SRS_I_FLAG_FIDELITY_SIM] = 0; }
+          break;
+        case INIT_RYR: if (strcmp(v, "") != 0) ints[SRS_I_INIT_RYR] = atoi(v); break;
+        case TITLE: if (title) snprintf(title, SRS_TITLE_LEN, "%s", v); break;
+      }
+      break;
+    }
+  }
+  return SRS_OK;
+}

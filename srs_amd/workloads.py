@@ -1,0 +1,224 @@
+"""Synthetic spark workloads on Kuhn-tet box meshes.
+
+* ``spark_box``: cytosol box 1.0 x 1.0 x 0.6 um, jSR slab 0.4 x 0.4 x 0.04 um
+  15 nm above the sarcolemmal z = 0 face, 7x7 RyR lattice at 31 nm, 4 LCCs,
+  cubes of h = 40 nm (56,250 tets).  The plain CPU-runnable spark.
+* ``dyad_box``: cytosol box 2 x 2 x 1 um with a 200 nm T-tubule cylinder
+  along y and a 1.0 x 1.0 x 0.04 um jSR slab 15 nm above it, h = 20 nm
+  (about 3M tets), 10x10 RyR lattice at 70% occupancy, 8 LCCs.  The plain
+  GPU workload.
+* ``ensemble``: many spark_box realizations on meshes of h = 40/30/25 nm
+  with irregular RyR clusters (9-100 RyRs, 50-90% occupancy) and SR loads
+  of 0.5-1.25 mM.
+
+Species and kinetics: Ca D = 0.22 um^2/ms, ATP 455 uM (D 0.14), CaM 24 uM
+(D 0.025), Fluo-4 50 uM (D 0.042), TnC 70 uM, CSQ 30 mM in the jSR,
+Ca 0.1 uM cytosol / 1 mM jSR at equilibrium, 2-state RyR
+k_open = 1e-4 [Ca]^2.1 /ms (phi = 1, A* = 0), k_close = 0.5 /ms, 0.15 pA
+at 1 mM gradient, LCCs clamped at 0 mV, explicit step 0.9 x the mesh
+stability bound.
+
+Two workload features have no counterpart in the reference's equations and
+are left out of the parameter vectors so that the reference reproduces every
+run: a jSR Ca diffusion coefficient different from the cytosolic one (the
+reference assembles the operator with D = 1 and scales Ca by D_CA in both
+domains, cpu/InitFuncs.cpp:1081, cpu/RunSimulation.cpp:62) and SR binding
+sites (the reference loads SR surface areas but uses them in no rate).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import BoxSpec, Grid, Params, assemble_operator, box_grid, equilibrium_state, read_parameters
+
+V_CELL = 25.84e3
+
+
+def spark_params(t_final=20.0, states_interval=0.1, dt=None) -> Params:
+    p = read_parameters(None)
+    p.title = "spark_box"
+    p.setr(D_CA=0.22, D_CA_JSR=0.22, D_ATP=0.14, D_CMDN=0.025, D_DYE=0.042,
+           B_TOT_ATP=455.0, B_TOT_CMDN=24.0, B_TOT_DYE=50.0, B_TOT_TRPN=70.0, B_T_JSR=30000.0,
+           C_0_CA=0.1, C_0_CA_JSR=1000.0, DEFAULT_CA_NSR=1000.0,
+           RYR_K_PLUS=1e-4, RYR_ETA=2.1, RYR_K_MINUS=0.5, RYR_PHI_M=0.0, RYR_A_STAR=0.0,
+           V_RYR=0.15 / (1000.0 * V_CELL * 96485 * 2 * 1e-6),
+           V_CLAMP=0.0, T_CLAMP=1e9, T_FINAL=t_final, OUTPUT_STATES_INTERVAL=states_interval)
+    p.seti(FLAG_V_CLAMP=1, FLAG_FIDELITY_SIM=0, FLAG_CLOCK_SEED=0, INIT_RYR=24)
+    if dt is not None:
+        p.setr(T_STEP=dt, INIT_RYR_HOLD=dt)
+    return p
+
+
+def stable_dt(aij: np.ndarray, D: float, safety: float = 0.9) -> float:
+    """0.9 x the positivity-preserving explicit bound 1/(D max|a_ii|)."""
+    return safety / (D * float(np.max(np.abs(aij[0::5]))))
+
+
+def _planes(lo, hi, n):
+    return np.linspace(lo, hi, n + 1)
+
+
+def spark_box_spec(h=0.04, n_xy=None, n_z_rest=None, jsr_layers=1, ryr_rows=7, ryr_cols=7, ryr_mask=None,
+                   n_lcc=4) -> BoxSpec:
+    """Cytosol 1.0 x 1.0 x 0.6 um: cleft 0 - 0.015 um, jSR 0.015 - 0.055 um."""
+    n_xy = n_xy or int(round(1.0 / h))
+    xs = _planes(0.0, 1.0, n_xy)
+    zr = n_z_rest or int(round(0.545 / h)) - 1
+    zs = np.concatenate([[0.0, 0.015], 0.015 + np.arange(1, jsr_layers + 1) * (0.04 / jsr_layers),
+                         np.linspace(0.055, 0.6, zr + 1)[1:]])
+    c = 0.52 if n_xy == 25 else 0.5
+    half = 0.2
+    sp = 0.031
+    lcc = []
+    for dx, dy in ((-1.5, -1.5), (1.5, -1.5), (-1.5, 1.5), (1.5, 1.5))[:n_lcc]:
+        lcc.append((c + dx * sp, c + dy * sp, 0.0075))
+    return BoxSpec(xs=xs, ys=xs.copy(), zs=zs, jsr_lo=(c - half, c - half, 0.015), jsr_hi=(c + half, c + half, 0.055),
+                   sl_face_z0=True, trpn_radius=-1.0, ryr_rows=ryr_rows, ryr_cols=ryr_cols, ryr_spacing=sp,
+                   ryr_center=(c, c), ryr_z_cyt=0.0075, ryr_z_jsr=0.015 + 0.02 / jsr_layers, ryr_mask=ryr_mask,
+                   lcc_xyz=np.array(lcc))
+
+
+def spark_box(h=0.04) -> BoxSpec:
+    """h = 40 nm: 25 x 25 x 15 cubes, 56,250 tets."""
+    return spark_box_spec(h=h)
+
+
+def dyad_box_spec(h=0.02, seed=38376427, occupancy=0.7) -> BoxSpec:
+    """2 x 2 x 1 um, T-tubule r = 100 nm along y at x = 1.0, z = 0.1; cleft
+    0.200 - 0.215 um above its top; jSR 0.215 - 0.255 um over 1.0 x 1.0 um."""
+    n_xy = int(round(2.0 / h))
+    xs = _planes(0.0, 2.0, n_xy)
+    n_low = int(round(0.2 / h))
+    n_top = 50 - n_low - 3 if abs(h - 0.02) < 1e-12 else int(round(0.745 / h))
+    zs = np.concatenate([np.linspace(0.0, 0.2, n_low + 1), [0.215, 0.235, 0.255], np.linspace(0.255, 1.0, n_top + 1)[1:]])
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    n = 100
+    mask = np.zeros(n, np.uint8)
+    mask[rng.permutation(n)[: int(round(occupancy * n))]] = 1
+    sp = 0.031
+    lcc = [(1.0 + dx * sp, 1.0 + dy * sp, 0.2075) for dx in (-0.5, 0.5) for dy in (-1.5, -0.5, 0.5, 1.5)]
+    return BoxSpec(xs=xs, ys=xs.copy(), zs=zs, jsr_lo=(0.5, 0.5, 0.215), jsr_hi=(1.5, 1.5, 0.255), sl_face_z0=False,
+                   tt_axis_x=1.0, tt_axis_z=0.1, tt_radius=0.1, trpn_radius=-1.0, ryr_rows=10, ryr_cols=10,
+                   ryr_spacing=sp, ryr_center=(1.0, 1.0), ryr_z_cyt=0.2075, ryr_z_jsr=0.225,
+                   ryr_mask=mask.reshape(10, 10), lcc_xyz=np.array(lcc))
+
+
+def tiny_box_spec(nx=4, ny=4, nz=3, tt=False, jsr=True, ryr=2, lcc=1, sl=True) -> BoxSpec:
+    """A few hundred tets with every feature the step has: jSR, SL face, T-tubule
+    hole, RyRs on the jSR face, LCCs."""
+    xs = np.linspace(0.0, 0.04 * nx, nx + 1)
+    ys = np.linspace(0.0, 0.04 * ny, ny + 1)
+    zs = np.concatenate([[0.0, 0.015], 0.015 + 0.04 * np.arange(1, nz)])
+    cx, cy = 0.02 * nx, 0.02 * ny
+    spec = BoxSpec(xs=xs, ys=ys, zs=zs, sl_face_z0=sl, trpn_radius=-1.0, ryr_rows=ryr, ryr_cols=ryr,
+                   ryr_spacing=0.031, ryr_center=(cx, cy), ryr_z_cyt=0.0075, ryr_z_jsr=0.035,
+                   lcc_xyz=np.array([(cx + 0.01, cy - 0.03, 0.0075)] * lcc).reshape(-1, 3))
+    if jsr:
+        spec.jsr_lo = (cx - 0.06, cy - 0.06, 0.015)
+        spec.jsr_hi = (cx + 0.06, cy + 0.06, 0.055)
+    if tt:
+        spec.tt_axis_x = 0.0
+        spec.tt_axis_z = 0.12
+        spec.tt_radius = 0.05
+        spec.trpn_radius = 0.08
+    return spec
+
+
+@dataclass
+class Prepared:
+    grid: Grid
+    aij: np.ndarray
+    nij: np.ndarray
+    X0: np.ndarray
+    bnd: np.ndarray
+    params: Params
+
+
+def prepare(spec: BoxSpec, params: Params | None = None, dt=None, t_final=None) -> Prepared:
+    g = box_grid(spec)
+    aij, nij = assemble_operator(g)
+    p = (params or spark_params()).copy()
+    if dt is None and not p.reals[1] > 0:
+        dt = stable_dt(aij, p.reals[5])
+    if dt is not None:
+        p.setr(T_STEP=dt, INIT_RYR_HOLD=dt)
+    if t_final is not None:
+        p.setr(T_FINAL=t_final)
+    X0, bnd = equilibrium_state(g, p.reals)
+    return Prepared(g, aij, nij, X0, bnd, p)
+
+
+def prepare_spark(h=0.04, **kw) -> Prepared:
+    spec = spark_box(h)
+    g = box_grid(spec)
+    aij, nij = assemble_operator(g)
+    p = spark_params(**{k: v for k, v in kw.items() if k in ("t_final", "states_interval")})
+    p.setr(T_STEP=stable_dt(aij, p.reals[5]))
+    p.setr(INIT_RYR_HOLD=p.reals[1])
+    X0, bnd = equilibrium_state(g, p.reals)
+    return Prepared(g, aij, nij, X0, bnd, p)
+
+
+def prepare_dyad(h=0.02, t_final=100.0) -> Prepared:
+    g = box_grid(dyad_box_spec(h))
+    aij, nij = assemble_operator(g)
+    p = spark_params(t_final=t_final)
+    p.title = "dyad_box"
+    p.setr(T_STEP=stable_dt(aij, p.reals[5]))
+    p.setr(INIT_RYR_HOLD=p.reals[1])
+    p.seti(INIT_RYR=0)
+    X0, bnd = equilibrium_state(g, p.reals)
+    return Prepared(g, aij, nij, X0, bnd, p)
+
+
+def ensemble_members(n_real: int, seed: int = 38376427):
+    """Per-realization draws: mesh index (round robin over h = 40/30/25 nm),
+    RyR count in 9..100, occupancy in 0.5-0.9, SR load in 500-1250 uM, and
+    the lattice mask (rows x cols, up to 12 x 12 so it fits the jSR)."""
+    rng = np.random.Generator(np.random.Philox(key=seed + 1))
+    out = []
+    for r in range(n_real):
+        n = int(rng.integers(9, 101))
+        occ = float(rng.uniform(0.5, 0.9))
+        L = int(min(12, np.ceil(np.sqrt(n / occ))))
+        mask = np.zeros(L * L, np.uint8)
+        mask[rng.permutation(L * L)[:n]] = 1
+        sr = float(rng.uniform(500.0, 1250.0))
+        out.append(dict(mesh=r % 3, n_ryr=n, L=L, mask=mask.reshape(L, L), ca_sr=sr))
+    return out
+
+
+ENSEMBLE_H = (0.04, 0.0303030303030303, 0.025)
+ENSEMBLE_SHAPES = ((25, 13, 1), (33, 18, 1), (40, 21, 2))  # (n_xy, z layers above the jSR, jSR layers)
+
+
+def ensemble_mesh_spec(m: int) -> BoxSpec:
+    """Mesh m of the ensemble with a full 12 x 12 RyR lattice; realizations
+    pick subsets of its positions."""
+    n_xy, zr, jl = ENSEMBLE_SHAPES[m]
+    return spark_box_spec(h=1.0 / n_xy, n_xy=n_xy, n_z_rest=zr, jsr_layers=jl, ryr_rows=12, ryr_cols=12)
+
+
+def lattice_subset(mask: np.ndarray, full_rows=12, full_cols=12):
+    """Indices into the full 12x12 lattice (centred) of an L x L mask, and the
+    up/down/left/right neighbour table of the occupied sites."""
+    L = mask.shape[0]
+    off = (full_rows - L) // 2
+    lab = -np.ones((L, L), np.int64)
+    idx = []
+    k = 0
+    for i in range(L):
+        for j in range(L):
+            if mask[i, j]:
+                lab[i, j] = k
+                idx.append((i + off) * full_cols + (j + off))
+                k += 1
+    nb = []
+    for i in range(L):
+        for j in range(L):
+            if mask[i, j]:
+                nb += [lab[i - 1, j] if i > 0 else -1, lab[i + 1, j] if i < L - 1 else -1,
+                       lab[i, j - 1] if j > 0 else -1, lab[i, j + 1] if j < L - 1 else -1]
+    return np.array(idx, np.int64), np.array(nb, np.int32)

@@ -1,0 +1,110 @@
+"""Shared helpers for the parity tests: building small workloads, running the
+HIP path through the C ABI, and an oracle-driven replica of the reference's
+RunSimulation loop (cpu/RunSimulation.cpp:402-589)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+import srs_amd as S
+from oracle.oracle import RNG_PHILOX, Fields, Oracle, grid_dict
+from srs_amd import workloads as W
+
+
+def fast_gating(p: S.Params) -> S.Params:
+    """Rates high enough that short runs see many RyR and LCC transitions."""
+    q = p.copy()
+    q.setr(RYR_K_MINUS=40.0, RYR_K_PLUS=0.5)
+    return q
+
+
+def initial_fields(pr, vm=0.0, ryr_open=(0,), lcc_state=6, lccv_state=2, seed=0):
+    g = pr.grid
+    nr, nl = g.n_ryr, g.n_lcc
+    rs = np.zeros(nr, np.int32)
+    for i in ryr_open:
+        if i < nr:
+            rs[i] = 1
+    rng = np.random.default_rng(seed)
+    R0 = np.log(rng.random(nr + nl))
+    return Fields(pr.X0, rs, np.full(nl, lcc_state), np.full(nl, lccv_state), R0,
+                  [vm, pr.params.reals[S.R["DEFAULT_CA_NSR"]]])
+
+
+def gpu_context(preps, fields, params=None, seeds=None, order=S.ORDER_RCM, record_capacity=0, share_mesh=False):
+    """A context with one realization per prepared workload."""
+    ctx = S.Context(0)
+    mids = []
+    for k, pr in enumerate(preps):
+        if share_mesh and k > 0:
+            mids.append(mids[0])
+        else:
+            mids.append(ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, order=order))
+    for k, pr in enumerate(preps):
+        g = pr.grid
+        p = params[k] if params is not None else pr.params
+        seed = seeds[k] if seeds is not None else 1000 + k
+        ctx.add_realization(mids[k], g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=seed, stream=k)
+    ctx.finalize(record_capacity)
+    for k, pr in enumerate(preps):
+        F = fields[k]
+        ctx.state_set(k, F.X)
+        ctx.channels_set(k, F.ryr_state, F.lcc_state, F.lccv_state, F.R)
+        ctx.globals_set(k, F.gv[0], F.gv[1])
+    return ctx
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.maximum(np.abs(b), 1e-300)
+    d = np.abs(a - b) / den
+    d[(a == b)] = 0.0
+    return float(d.max()) if d.size else 0.0
+
+
+def output_due(t, dt_last, iv):
+    return math.floor(t / iv) > math.floor((t - dt_last) / iv)
+
+
+def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, max_steps=10 ** 7):
+    """RunSimulation's main loop with the oracle's step (Philox gating).
+    Returns the records written inside and after the loop (the t = 0 record
+    before the loop is the caller's)."""
+    G = grid_dict(pr.grid, pr.aij, pr.nij)
+    P = p.reals
+    Iv = p.ints
+    dt = float(P[S.R["T_STEP"]])
+    t, dt_last = 0.0, 0.0
+    recs = []
+    n_open = int(F.ryr_state.sum())
+    ca_max = 0.0
+    lcc_open = int(Iv[S.I["FLAG_LCC"]])
+    fid = bool(Iv[S.I["FLAG_FIDELITY_SIM"]])
+    step = 0
+    ir = int(Iv[S.I["INIT_RYR"]])
+    ryr_out = bool(Iv[S.I["FLAG_RYR_OUT"]])
+    while t <= P[S.R["T_FINAL"]] and (not fid or ((lcc_open or n_open > 0 or ca_max > P[S.R["C_MIN"]])
+                                                  and n_open < Iv[S.I["N_RYR_FIDELITY"]])):
+        if Iv[S.I["FLAG_V_CLAMP"]]:
+            F.gv[0] = P[S.R["V_CLAMP"]] if t <= P[S.R["T_CLAMP"]] else P[S.R["DEFAULT_VM"]]
+        if ir != -1 and t < P[S.R["INIT_RYR_HOLD"]] and not F.ryr_state[ir]:
+            F.ryr_state[ir] = 1
+        if output_due(t, dt_last, P[S.R["OUTPUT_STATES_INTERVAL"]]):
+            n_open = int(F.ryr_state.sum())
+            Ca = F.X[: G["n_ele"]]
+            ca_max = 0.0
+            for e in G["ryr_ele"]:
+                if Ca[e] > ca_max:
+                    ca_max = Ca[e]
+            recs.append(O.record(t, G, F, P, ryr_out=ryr_out))
+        O.steps(G, P, F, 1, dt, gating=not Iv[S.I["FLAG_NO_GATING"]], rng_kind=RNG_PHILOX, key=key, stream=stream,
+                step0=step)
+        step += 1
+        t += dt
+        dt_last = dt
+        if step > max_steps:
+            raise RuntimeError("oracle loop did not terminate")
+    recs.append(O.record(t, G, F, P, ryr_out=ryr_out))
+    return recs, t, step

@@ -1,0 +1,165 @@
+"""HIP path vs the oracle, through the C ABI (needs an MI355X).
+
+Tolerance: the north-star bound rel <= 1e-10 for FP64 fields.  The sweep is
+built without FP contraction and in the reference's operation order, so most
+runs are bit-identical; the bound leaves room for the one-ulp differences of
+device exp/pow/log against glibc in the LCC flux and the gating rates.
+Channel trajectories (integer states) must match exactly.
+"""
+import numpy as np
+import pytest
+
+import srs_amd as S
+from oracle.oracle import RNG_PHILOX, Fields, Oracle, grid_dict
+from srs_amd import workloads as W
+from tests.helpers import fast_gating, gpu_context, initial_fields, oracle_loop, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _tiny(**kw):
+    return W.prepare(W.tiny_box_spec(**kw), W.spark_params())
+
+
+@pytest.mark.parametrize("tt", [False, True])
+def test_replay_schedule_matches_oracle(tt):
+    """Deterministic mode: RyR states replayed from a recorded schedule, gating off."""
+    pr = _tiny(tt=tt)
+    p = pr.params.copy().seti(FLAG_NO_GATING=1)
+    F = initial_fields(pr)
+    nsteps = 300
+    sch = (np.random.default_rng(3).random((nsteps, pr.grid.n_ryr)) < 0.5).astype(np.int32)
+    print(f"replay tt={tt} n_ele={pr.grid.n_ele}", flush=True)
+    with gpu_context([pr], [F], params=[p]) as ctx:
+        ctx.replay(0, sch)
+        Xg = ctx.state_get(0)
+    Fo = Oracle().steps(grid_dict(pr.grid, pr.aij, pr.nij), p.reals, F.copy(), nsteps, p.reals[1], gating=False,
+                        schedule=sch)
+    assert rel_err(Xg, Fo.X) <= TOL
+
+
+@pytest.mark.parametrize("order", [S.ORDER_AS_GIVEN, S.ORDER_RCM])
+def test_gating_trajectory_matches_oracle(order):
+    """Stochastic mode with the same Philox stream: identical channel trajectories."""
+    pr = _tiny(tt=True)
+    p = fast_gating(pr.params)
+    F = initial_fields(pr, seed=5)
+    n = 600
+    print(f"gating order={order}", flush=True)
+    with gpu_context([pr], [F], params=[p], seeds=[77], order=order) as ctx:
+        ctx.explicit_step(n)
+        Xg = ctx.state_get(0)
+        rs, ls, lvs, Rg = ctx.channels_get(0)
+    Fo = Oracle().steps(grid_dict(pr.grid, pr.aij, pr.nij), p.reals, F.copy(), n, p.reals[1], gating=True,
+                        rng_kind=RNG_PHILOX, key=77, stream=0)
+    assert np.array_equal(rs, Fo.ryr_state)
+    assert np.array_equal(ls, Fo.lcc_state) and np.array_equal(lvs, Fo.lccv_state)
+    assert rel_err(Rg, Fo.R) <= 1e-9
+    assert rel_err(Xg, Fo.X) <= TOL
+
+
+def test_coupled_ryr_gating_is_sequential():
+    """A* != 0: later RyRs see earlier flips (cpu/RunSimulation.cpp:661-666)."""
+    pr = W.prepare(W.tiny_box_spec(nx=6, ny=6, ryr=3), W.spark_params())
+    p = fast_gating(pr.params).setr(RYR_A_STAR=0.8, RYR_K_MINUS=60.0, RYR_K_PLUS=2.0)
+    F = initial_fields(pr, ryr_open=(0, 4, 8), seed=9)
+    n = 500
+    with gpu_context([pr], [F], params=[p], seeds=[5]) as ctx:
+        ctx.explicit_step(n)
+        Xg = ctx.state_get(0)
+        rs, _, _, Rg = ctx.channels_get(0)
+    Fo = Oracle().steps(grid_dict(pr.grid, pr.aij, pr.nij), p.reals, F.copy(), n, p.reals[1], gating=True,
+                        rng_kind=RNG_PHILOX, key=5, stream=0)
+    assert np.array_equal(rs, Fo.ryr_state)
+    assert rel_err(Xg, Fo.X) <= TOL
+
+
+def test_graph_replay_equals_launches():
+    pr = _tiny(tt=True)
+    p = fast_gating(pr.params)
+    F = initial_fields(pr, seed=2)
+    out = []
+    for flags in (S.STEP_PLAIN, S.STEP_GRAPH):
+        with gpu_context([pr], [F], params=[p], seeds=[3]) as ctx:
+            ctx.explicit_step(300, flags)
+            out.append((ctx.state_get(0), ctx.channels_get(0)[0]))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_batch_equals_single_runs():
+    """Realizations of different meshes and parameters in one launch reproduce
+    their single-realization runs bit for bit."""
+    preps = [_tiny(tt=False), _tiny(tt=True), W.prepare(W.tiny_box_spec(nx=5, ny=3), W.spark_params())]
+    params = [fast_gating(pr.params) for pr in preps]
+    params[2].setr(C_0_CA_JSR=700.0, DEFAULT_CA_NSR=700.0)
+    fields = [initial_fields(pr, seed=k) for k, pr in enumerate(preps)]
+    seeds = [11, 12, 13]
+    with gpu_context(preps, fields, params=params, seeds=seeds) as ctx:
+        ctx.explicit_step(250, S.STEP_GRAPH)
+        batch = [(ctx.state_get(k), ctx.channels_get(k)[0]) for k in range(3)]
+    for k, pr in enumerate(preps):
+        Fo = Oracle().steps(grid_dict(pr.grid, pr.aij, pr.nij), params[k].reals, fields[k].copy(), 250,
+                            params[k].reals[1], gating=True, rng_kind=RNG_PHILOX, key=seeds[k], stream=k)
+        assert np.array_equal(batch[k][1], Fo.ryr_state)
+        assert rel_err(batch[k][0], Fo.X) <= TOL
+
+
+def test_protocol_loop_records_match_oracle_loop():
+    """The device-side RunSimulation loop body (clamp, RyR hold, states output,
+    while-condition, final record) against an oracle-driven loop."""
+    pr = _tiny(tt=True)
+    p = fast_gating(pr.params)
+    dt = p.reals[1]
+    p.setr(T_FINAL=900 * dt, OUTPUT_STATES_INTERVAL=37 * dt, INIT_RYR_HOLD=100 * dt, T_CLAMP=450 * dt,
+           V_CLAMP=-10.0, DEFAULT_VM=-80.0)
+    p.seti(INIT_RYR=1, FLAG_V_CLAMP=1, FLAG_RYR_OUT=1)
+    F = initial_fields(pr, ryr_open=(), vm=-80.0, seed=4)
+    O = Oracle()
+    recs_o, t_end, nsteps = oracle_loop(O, pr, p, F.copy(), key=21, stream=0)
+    # device: apply the t = 0 iteration on the host, as srs_run_cru3d does
+    Fd = F.copy()
+    Fd.gv[0] = p.reals[S.R["V_CLAMP"]]
+    Fd.ryr_state[1] = 1
+    with gpu_context([pr], [Fd], params=[p], seeds=[21], record_capacity=256) as ctx:
+        ctx.loop_set(0, n_open=int(F.ryr_state.sum()))
+        ctx.explicit_step(nsteps + 5, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        recs_d = ctx.records(0)
+        ls = ctx.loop_get(0)
+    assert ls["done"] == 1 and ls["step"] == nsteps and ls["t"] == t_end
+    assert recs_d.shape[0] == len(recs_o)
+    np.testing.assert_array_equal(recs_d, np.array(recs_o))
+
+
+def test_spark_box_replay_full_mesh():
+    """The 56,250-tet spark box, replayed schedule, 200 steps."""
+    pr = W.prepare_spark()
+    p = pr.params.copy().seti(FLAG_NO_GATING=1)
+    F = initial_fields(pr, ryr_open=(24,))
+    sch = np.zeros((200, pr.grid.n_ryr), np.int32)
+    sch[:, 24] = 1
+    sch[100:, 17] = 1
+    print(f"spark box n_ele={pr.grid.n_ele}", flush=True)
+    with gpu_context([pr], [F], params=[p]) as ctx:
+        ctx.replay(0, sch)
+        Xg = ctx.state_get(0)
+    Fo = Oracle().steps(grid_dict(pr.grid, pr.aij, pr.nij), p.reals, F.copy(), 200, p.reals[1], gating=False,
+                        schedule=sch)
+    assert rel_err(Xg, Fo.X) <= TOL
+
+
+def test_no_channels_and_done_realizations():
+    """Edge cases: a realization without channels; a realization marked done is not stepped."""
+    pr = _tiny(ryr=0, lcc=0)
+    F = initial_fields(pr)
+    pr2 = _tiny()
+    F2 = initial_fields(pr2)
+    with gpu_context([pr, pr2], [F, F2]) as ctx:
+        ctx.loop_set(1, done=1)
+        ctx.explicit_step(50)
+        X1 = ctx.state_get(1)
+        X0 = ctx.state_get(0)
+    assert np.array_equal(X1, F2.X)
+    Fo = Oracle().steps(grid_dict(pr.grid, pr.aij, pr.nij), pr.params.reals, F.copy(), 50, pr.params.reals[1],
+                        gating=True, rng_kind=RNG_PHILOX, key=1000, stream=0)
+    assert rel_err(X0, Fo.X) <= TOL
