@@ -146,6 +146,8 @@ def prepare(spec: BoxSpec, params: Params | None = None, dt=None, t_final=None) 
         p.setr(T_STEP=dt, INIT_RYR_HOLD=dt)
     if t_final is not None:
         p.setr(T_FINAL=t_final)
+    if p.ints[18] >= g.n_ryr:  # the initial RyR must exist on this mesh
+        p.ints[18] = 0 if g.n_ryr > 0 else -1
     X0, bnd = equilibrium_state(g, p.reals)
     return Prepared(g, aij, nij, X0, bnd, p)
 
