@@ -1,0 +1,218 @@
+"""Spark time-step benchmark on one MI355X.
+
+Default workload (the plain GPU drop-in): the dyad box -- 2 x 2 x 1 um
+cytosol, 200 nm T-tubule, 1.0 x 1.0 x 0.04 um jSR, h = 20 nm Kuhn tets
+(2,952,000 elements), 10x10 RyR lattice at 70% occupancy (70 RyRs), 8 LCCs,
+FP64, one realization.  A step is one ExplicitStep (fused sweep + gating) of
+that realization with its inputs resident in HBM.
+
+Printed as one JSON line:
+  value        simulated spark time per wall second (spark-ms/s), plain path
+  roofline     the fused sweep kernel against MI355X HBM bandwidth: algorithmic
+               bytes per launch (153 B per element) / mean kernel time from HIP
+               events on the context's stream
+  cpu_baseline the reference's own cpu/RunSimulation.cpp ExplicitStep
+               (oracle/_ref) on the same mesh, 1 host core, bounded sample
+  batched      the ensemble workload: many independent spark realizations on
+               h = 40/30/25 nm meshes in one launch per step, against the
+               plain path run one realization at a time on the h = 40 nm mesh
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "spark-ms simulated/s & HBM GB/s % peak; batched vs plain, 1 MI355X"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(domain, tropc):
+    """HBM bytes one sweep must move per step: each live FP64 species read and
+    written once (Ca, ATP, CaM, Fluo everywhere; TnC in cytosol elements with
+    troponin; CSQ in the jSR), the ELL row (4 int32 + 5 FP64) and a 1-byte
+    flag word.  Neighbour gathers are counted as cache hits."""
+    jsr = np.asarray(domain) != 0
+    live = 4 + ((~jsr) & (np.asarray(tropc) != 0)) + jsr
+    return int(np.sum(16 * live + 57))
+
+
+def setup_plain(pr, order="block"):
+    import srs_amd as S
+    from srs_amd.workloads import initial_channels, kuhn_block_order
+
+    ctx = S.Context(0)
+    if order == "block":
+        mid = ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, perm=kuhn_block_order(pr.grid))
+    else:
+        mid = ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, order=S.ORDER_RCM if order == "rcm" else S.ORDER_AS_GIVEN)
+    g = pr.grid
+    ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, pr.params, seed=38376427, stream=0)
+    ctx.finalize()
+    rs, ls, lvs, R = initial_channels(pr)
+    ctx.state_set(0, pr.X0)
+    ctx.channels_set(0, rs, ls, lvs, R)
+    ctx.globals_set(0, 0.0, pr.params.reals[16])
+    return ctx
+
+
+def bench_plain(pr, steps, warmup, order):
+    import srs_amd as S
+
+    ctx = setup_plain(pr, order)
+    ctx.explicit_step(warmup, S.STEP_GRAPH)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.explicit_step(steps, S.STEP_GRAPH)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    total_ms, _ = ctx.last_timing()
+    sweep_ms, gate_ms = ctx.time_sweep(50)
+    ctx.close()
+    return wall, total_ms, sweep_ms, gate_ms
+
+
+def bench_cpu(pr, budget_s=15.0):
+    from oracle.oracle import Fields, Oracle, RefStep, grid_dict
+    from srs_amd.workloads import initial_channels
+
+    G = grid_dict(pr.grid, pr.aij, pr.nij)
+    rs, ls, lvs, R = initial_channels(pr)
+    F = Fields(pr.X0, rs, ls, lvs, R, [0.0, pr.params.reals[16]])
+    p = pr.params
+    if RefStep.available():
+        runner, kind = RefStep(), "reference"
+        run = lambda n: runner.steps(G, p.ints, p.reals, F, n, p.reals[1])  # noqa: E731
+    else:
+        runner, kind = Oracle(), "port"
+        run = lambda n: runner.steps(G, p.reals, F, n, p.reals[1], gating=True)  # noqa: E731
+    t0 = time.perf_counter()
+    run(2)
+    per = (time.perf_counter() - t0) / 2
+    n = max(3, int(budget_s / per))
+    t0 = time.perf_counter()
+    run(n)
+    wall = time.perf_counter() - t0
+    return n, wall, kind
+
+
+def bench_batched(n_real, steps, warmup, order):
+    import srs_amd as S
+    from srs_amd import workloads as W
+
+    ctx, dts, n_ele, nbytes = W.ensemble_context(n_real, order=order)
+    ctx.explicit_step(warmup, S.STEP_GRAPH)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.explicit_step(steps, S.STEP_GRAPH)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    sweep_ms, gate_ms = ctx.time_sweep(5)
+    ctx.close()
+    # plain path, one realization at a time, on the h = 40 nm member mesh
+    pr = W.prepare_spark()
+    pctx = setup_plain(pr, order)
+    pctx.explicit_step(200, S.STEP_GRAPH)
+    pctx.sync()
+    t0 = time.perf_counter()
+    pctx.explicit_step(2000, S.STEP_GRAPH)
+    pctx.sync()
+    pwall = time.perf_counter() - t0
+    pctx.close()
+    return dict(wall=wall, sim_ms=float(np.sum(dts)) * steps, sweep_ms=sweep_ms, gate_ms=gate_ms, n_ele=n_ele, nbytes=nbytes,
+                plain_rate=2000 * pr.params.reals[1] / pwall, plain_dt=float(pr.params.reals[1]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--batch", type=int, default=2048, help="realizations in the batched leg (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work (0: skip)")
+    ap.add_argument("--order", choices=("block", "rcm", "given"), default="block",
+                    help="device element order: Kuhn cube blocks, reverse Cuthill-McKee, or the reference BFS order")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one device copy of 512 MiB (HBM byte-counter calibration under rocprofv3 --pmc)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the spark step does not shard across GPUs "
+              "in this build", file=sys.stderr)
+        sys.exit(2)
+
+    from srs_amd import workloads as W
+
+    if args.calib_copy:
+        import torch
+
+        a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 512 MiB read + 512 MiB written")
+        del a, b
+    t0 = time.perf_counter()
+    pr = W.prepare_dyad()
+    log(f"dyad box: {pr.grid.n_ele} elements, {pr.grid.n_ryr} RyR, {pr.grid.n_lcc} LCC, dt {pr.params.reals[1]:.4e} ms "
+        f"(setup {time.perf_counter() - t0:.1f} s)")
+    dt = float(pr.params.reals[1])
+    N = pr.grid.n_ele
+    wall, total_ms, sweep_ms, gate_ms = bench_plain(pr, args.steps, args.warmup, args.order)
+    log(f"plain: {args.steps} steps in {wall:.3f} s ({1e3 * wall / args.steps:.4f} ms/step), sweep {sweep_ms * 1e3:.2f} us,"
+        f" gating {gate_ms * 1e3:.2f} us")
+    value = args.steps * dt / wall
+    nbytes = algorithmic_bytes(pr.grid.domain, pr.grid.tropc)
+    achieved = nbytes / (sweep_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "sweep_dyad_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
+    out = {
+        "metric": METRIC, "value": value, "unit": "spark-ms simulated/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"dyad_box 2x2x1 um, T-tubule r=100 nm, jSR 1.0x1.0x0.04 um, h=20 nm Kuhn tets "
+                               f"({N} elements), {pr.grid.n_ryr} RyR + {pr.grid.n_lcc} LCC, 1 realization, FP64",
+                   "n_elements": N, "dt_ms": dt, "element_order": args.order,
+                   "parallelism": "single realization, one GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_sweep", "kernel_ms": sweep_ms, "bytes_per_launch": nbytes,
+                     "bytes_per_element": nbytes / N},
+    }
+    if args.cpu_budget > 0:
+        n, cwall, kind = bench_cpu(pr, args.cpu_budget)
+        out["cpu_baseline"] = {"value": n * dt / cwall, "unit": "spark-ms simulated/s", "cores": 1, "kind": kind,
+                               "sample": f"{n} ExplicitStep calls with gating on the same {N}-element mesh"}
+        log(f"cpu {kind}: {n} steps in {cwall:.2f} s")
+    if args.batch > 0:
+        b = bench_batched(args.batch, args.batch_steps, 2, args.order)
+        brate = b["sim_ms"] / b["wall"]
+        bach = b["nbytes"] / (b["sweep_ms"] * 1e-3) / 1e9
+        out["batched"] = {
+            "workload": f"{args.batch} realizations on h=40/30/25 nm spark boxes (56,250/130,680/230,400 elements), "
+                        "9-100 RyR each, SR load 0.5-1.25 mM, FP64",
+            "value": brate, "unit": "spark-ms simulated/s", "sparks_per_s": brate / 50.0,
+            "steps": args.batch_steps, "ms_per_step": 1e3 * b["wall"] / args.batch_steps,
+            "sweep_ms": b["sweep_ms"], "sweep_gbs": bach, "sweep_frac": bach / HBM_PEAK_GBS,
+            "plain_sequential_value": b["plain_rate"], "speedup_vs_plain_sequential": brate / b["plain_rate"],
+        }
+        log(f"batched: {brate:.3f} spark-ms/s vs plain {b['plain_rate']:.4f} ({brate / b['plain_rate']:.1f}x)")
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

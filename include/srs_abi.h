@@ -240,6 +240,7 @@ typedef struct srs_ctx srs_ctx;
 /* Element order inside a context (never visible through the ABI). */
 #define SRS_ORDER_AS_GIVEN 0
 #define SRS_ORDER_RCM 1
+#define SRS_ORDER_CUSTOM 2   /* srs_mesh_desc.perm gives the order */
 
 /* Replaces InitializeDeviceConstants / cudaSetDevice (gpu/RunSimulation.cu:503-506, 869-872). */
 int srs_ctx_create(int device, srs_ctx** out);
@@ -260,6 +261,7 @@ typedef struct srs_mesh_desc {
   const double* V0;        /* [n_ele] */
   const double* tt_area;   /* [n_ele] */
   int32_t order;
+  const int32_t* perm;     /* SRS_ORDER_CUSTOM: perm[k] = element stored at position k */
 } srs_mesh_desc;
 int srs_ctx_add_mesh(srs_ctx* ctx, const srs_mesh_desc* mesh, int32_t* mesh_id);
 

@@ -54,6 +54,7 @@ STEP_PROTOCOL = 1
 STEP_GRAPH = 2
 ORDER_AS_GIVEN = 0
 ORDER_RCM = 1
+ORDER_CUSTOM = 2
 
 
 def _p(a, t):
@@ -303,7 +304,11 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
-    def add_mesh(self, aij, nij, domain, tropc, V0, tt_area, br=None, order=ORDER_RCM) -> int:
+    def add_mesh(self, aij, nij, domain, tropc, V0, tt_area, br=None, order=ORDER_RCM, perm=None) -> int:
+        """order: ORDER_AS_GIVEN, ORDER_RCM, or a permutation array (perm[k] =
+        element stored at device position k)."""
+        if perm is not None:
+            order = ORDER_CUSTOM
         arrs = [np.ascontiguousarray(aij, np.float64), np.ascontiguousarray(nij, np.int32),
                 np.ascontiguousarray(domain, np.int32), np.ascontiguousarray(tropc, np.int32),
                 np.ascontiguousarray(V0, np.float64), np.ascontiguousarray(tt_area, np.float64)]
@@ -314,14 +319,18 @@ class Context:
         d.br = _f64(brr) if brr is not None else None
         d.domain, d.tropc, d.V0, d.tt_area = _i32(arrs[2]), _i32(arrs[3]), _f64(arrs[4]), _f64(arrs[5])
         d.order = order
+        if perm is not None:
+            pa = np.ascontiguousarray(perm, np.int32)
+            self._keep.append(pa)
+            d.perm = _i32(pa)
         mid = C.c_int32()
         check(lib().srs_ctx_add_mesh(self._h, C.byref(d), C.byref(mid)))
         return mid.value
 
-    def add_mesh_grid(self, g: Grid, aij=None, nij=None, br=None, order=ORDER_RCM) -> int:
+    def add_mesh_grid(self, g: Grid, aij=None, nij=None, br=None, order=ORDER_RCM, perm=None) -> int:
         if aij is None:
             aij, nij = assemble_operator(g)
-        return self.add_mesh(aij, nij, g.domain, g.tropc, g.V0, g.tt_area, br=br, order=order)
+        return self.add_mesh(aij, nij, g.domain, g.tropc, g.V0, g.tt_area, br=br, order=order, perm=perm)
 
     def add_realization(self, mesh_id, ryr_ele, ryr_jsr, ryr_nb, lcc_ele, params: Params, seed: int,
                         stream: int) -> int:

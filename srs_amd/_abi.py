@@ -55,6 +55,7 @@ class SrsMeshDesc(C.Structure):
     _fields_ = [
         ("n_ele", C.c_int32), ("aij", f64p), ("nij", i32p), ("br", f64p),
         ("domain", i32p), ("tropc", i32p), ("V0", f64p), ("tt_area", f64p), ("order", C.c_int32),
+        ("perm", i32p),
     ]
 
 

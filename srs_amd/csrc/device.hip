@@ -124,7 +124,14 @@ __global__ __launch_bounds__(TILE) void k_sweep(const RealDesc* __restrict__ D, 
   const double* Sat = Xo + srs::ATP * np;
   const double* Scm = Xo + srs::CMDN * np;
   const double c_ca = Sca[j], c_dy = Sdy[j], c_at = Sat[j], c_cm = Scm[j];
-  const double c_tr = Xo[srs::TRPN * np + j], c_cs = Xo[srs::CSQN * np + j];
+  // TnC has a nonzero rate only in cytosol elements with troponin, CSQ only
+  // in the jSR (cpu/RunSimulation.cpp:45-46 reset both rates to zero).  A
+  // zero rate leaves the value unchanged, and both state buffers hold the same
+  // value, so those planes are neither read nor written elsewhere.
+  const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
+  const bool live_cs = (f & F_JSR) != 0;
+  const double c_tr = live_tr ? Xo[srs::TRPN * np + j] : 0.0;
+  const double c_cs = live_cs ? Xo[srs::CSQN * np + j] : 0.0;
 
   double br_ca = 0.0, br_dy = 0.0, br_at = 0.0, br_cm = 0.0;
   if (f & F_BR) {
@@ -209,8 +216,8 @@ __global__ __launch_bounds__(TILE) void k_sweep(const RealDesc* __restrict__ D, 
   Xn[srs::CA * np + j] = c_ca + rca * dt;
   Xn[srs::ATP * np + j] = c_at + rat * dt;
   Xn[srs::CMDN * np + j] = c_cm + rcm * dt;
-  Xn[srs::TRPN * np + j] = c_tr + rtr * dt;
-  Xn[srs::CSQN * np + j] = c_cs + rcs * dt;
+  if (live_tr) Xn[srs::TRPN * np + j] = c_tr + rtr * dt;
+  if (live_cs) Xn[srs::CSQN * np + j] = c_cs + rcs * dt;
   Xn[srs::DYE * np + j] = c_dy + rdy * dt;
 }
 
@@ -553,6 +560,14 @@ int srs_ctx_add_mesh(srs_ctx* c, const srs_mesh_desc* m, int32_t* mesh_id) {
   // element order inside the context
   if (m->order == SRS_ORDER_RCM) {
     srs::rcm_order(n, m->nij, hm->perm);
+  } else if (m->order == SRS_ORDER_CUSTOM) {
+    if (!m->perm) return srs::fail(SRS_E_ARG, "SRS_ORDER_CUSTOM needs a permutation");
+    hm->perm.assign(m->perm, m->perm + n);
+    std::vector<uint8_t> seen(n, 0);
+    for (int i = 0; i < n; ++i) {
+      if (hm->perm[i] < 0 || hm->perm[i] >= n || seen[hm->perm[i]]) return srs::fail(SRS_E_ARG, "perm is not a permutation");
+      seen[hm->perm[i]] = 1;
+    }
   } else {
     hm->perm.resize(n);
     std::iota(hm->perm.begin(), hm->perm.end(), 0);

@@ -155,7 +155,7 @@ extern "C" int srs_run_cru3d(const char* param_path, const char* mesh_base, cons
   const int n_sims = I[SRS_I_N_SIMS] - start_index;
   if (n_sims <= 0) return SRS_OK;
   if ((rc = srs_ctx_create(I[SRS_I_CUDA_DEVICE], &G.ctx))) return rc;
-  srs_mesh_desc md{N, aij.data(), nij.data(), br.data(), g->domain, g->tropc, g->V0, g->tt_area, SRS_ORDER_RCM};
+  srs_mesh_desc md{N, aij.data(), nij.data(), br.data(), g->domain, g->tropc, g->V0, g->tt_area, SRS_ORDER_RCM, nullptr};
   int32_t mesh_id;
   if ((rc = srs_ctx_add_mesh(G.ctx, &md, &mesh_id))) return rc;
   srs_chan_desc cd{g->n_ryr, g->ryr_ele, g->ryr_jsr, g->ryr_nb, g->n_lcc, g->lcc_ele};

@@ -224,3 +224,97 @@ def lattice_subset(mask: np.ndarray, full_rows=12, full_cols=12):
                 nb += [lab[i - 1, j] if i > 0 else -1, lab[i + 1, j] if i < L - 1 else -1,
                        lab[i, j - 1] if j > 0 else -1, lab[i, j + 1] if j < L - 1 else -1]
     return np.array(idx, np.int64), np.array(nb, np.int32)
+
+
+def initial_channels(pr: Prepared, open_ryr=None, seed=0):
+    """Channel states at the start of a run: the initial RyR open (the
+    parameter vector's ryr_open_init unless given), LCCs closed-resting
+    (state 1, gate open 2, cpu/InitFuncs.cpp:971-982) and residuals ln U."""
+    g = pr.grid
+    rs = np.zeros(g.n_ryr, np.int32)
+    i0 = int(pr.params.ints[18]) if open_ryr is None else open_ryr
+    if 0 <= i0 < g.n_ryr:
+        rs[i0] = 1
+    rng = np.random.default_rng(seed)
+    R = np.log(rng.random(g.n_ryr + g.n_lcc) * (1 - 2**-53) + 2**-54)
+    return rs, np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R
+
+
+def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0, t_final: float = 50.0,
+                     order: str = "block"):
+    """The batched Monte Carlo workload in one context: returns (ctx, dt per
+    realization, total elements over realizations, algorithmic sweep bytes
+    per step: live species of every realization plus each distinct mesh's
+    operator rows once)."""
+    from . import ORDER_AS_GIVEN, ORDER_RCM, Context
+
+    members = ensemble_members(n_real, seed)
+    ctx = Context(0)
+    meshes = []
+    for m in range(3):
+        g = box_grid(ensemble_mesh_spec(m))
+        aij, nij = assemble_operator(g)
+        if order == "block":
+            mid = ctx.add_mesh(aij, nij, g.domain, g.tropc, g.V0, g.tt_area, perm=kuhn_block_order(g))
+        else:
+            mid = ctx.add_mesh(aij, nij, g.domain, g.tropc, g.V0, g.tt_area,
+                               order=ORDER_RCM if order == "rcm" else ORDER_AS_GIVEN)
+        meshes.append((g, aij, nij, mid, stable_dt(aij, 0.22)))
+    dts, n_ele, nbytes = [], 0, 0
+    plist = []
+    for g, *_ in meshes:
+        nbytes += 57 * g.n_ele
+    for r, mem in enumerate(members):
+        g, aij, nij, mid, dt = meshes[mem["mesh"]]
+        idx, nb = lattice_subset(mem["mask"])
+        p = spark_params(t_final=t_final)
+        p.setr(T_STEP=dt, INIT_RYR_HOLD=dt, C_0_CA_JSR=mem["ca_sr"], DEFAULT_CA_NSR=mem["ca_sr"])
+        p.seti(INIT_RYR=int(len(idx) // 2))
+        ctx.add_realization(mid, g.ryr_ele[idx], g.ryr_jsr[idx], nb, g.lcc_ele, p, seed=seed + r, stream=r)
+        dts.append(dt)
+        n_ele += g.n_ele
+        jsr = g.domain != 0
+        nbytes += int(np.sum(16 * (4 + ((~jsr) & (g.tropc != 0)) + jsr)))
+        plist.append((g, p, len(idx)))
+    ctx.finalize(record_capacity)
+    for r, (g, p, nr) in enumerate(plist):
+        X0, _ = equilibrium_state(g, p.reals)
+        ctx.state_set(r, X0)
+        rs = np.zeros(nr, np.int32)
+        rs[nr // 2] = 1
+        rng = np.random.default_rng(seed + r)
+        R = np.log(rng.random(nr + g.n_lcc) * (1 - 2**-53) + 2**-54)
+        ctx.channels_set(r, rs, np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R)
+        ctx.globals_set(r, 0.0, p.reals[16])
+    return ctx, np.array(dts), n_ele, nbytes
+
+
+def kuhn_block_order(g: Grid, block: int = 64) -> np.ndarray:
+    """Device element order for a Kuhn box mesh: cubes ordered with z fastest,
+    then x, then y (the longest stride is nz*nx cubes), grouped in blocks of
+    `block` cubes, and inside a block the six tets of one Kuhn type stored
+    together.  A wavefront then holds one tet type over consecutive cubes, so
+    its neighbour gathers fall at (nearly) uniform offsets.  Returns perm with
+    perm[k] = reference-numbered element stored at position k."""
+    n = g.n_ele
+    ftype = g.perm % 6  # file order is cube-by-cube, 6 tets per cube, one per axis permutation
+    nodes = g.ele_nodes.reshape(-1, 4)
+    xyz = g.node_xyz.reshape(-1, 3)
+    cen = xyz[nodes].mean(axis=1)
+    planes = []
+    for ax in range(3):
+        planes.append(np.unique(xyz[:, ax]))
+    idx = [np.clip(np.searchsorted(planes[ax], cen[:, ax]) - 1, 0, len(planes[ax]) - 2) for ax in range(3)]
+    nx, nz = len(planes[0]) - 1, len(planes[2]) - 1
+    key = (idx[1].astype(np.int64) * nx + idx[0]) * nz + idx[2]
+    ucubes, cube_rank = np.unique(key, return_inverse=True)
+    if len(ucubes) * 6 != n:
+        raise ValueError("not a Kuhn box mesh (6 tets per cube expected)")
+    nc = len(ucubes)
+    blk = cube_rank // block
+    start = blk * block
+    width = np.minimum(block, nc - start)
+    pos = start * 6 + ftype * width + (cube_rank - start)
+    perm = np.empty(n, np.int32)
+    perm[pos] = np.arange(n, dtype=np.int32)
+    return perm

@@ -39,6 +39,8 @@ def gpu_context(preps, fields, params=None, seeds=None, order=S.ORDER_RCM, recor
     for k, pr in enumerate(preps):
         if share_mesh and k > 0:
             mids.append(mids[0])
+        elif order == "block":
+            mids.append(ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, perm=W.kuhn_block_order(pr.grid)))
         else:
             mids.append(ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, order=order))
     for k, pr in enumerate(preps):

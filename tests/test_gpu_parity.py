@@ -39,7 +39,7 @@ def test_replay_schedule_matches_oracle(tt):
     assert rel_err(Xg, Fo.X) <= TOL
 
 
-@pytest.mark.parametrize("order", [S.ORDER_AS_GIVEN, S.ORDER_RCM])
+@pytest.mark.parametrize("order", [S.ORDER_AS_GIVEN, S.ORDER_RCM, "block"])
 def test_gating_trajectory_matches_oracle(order):
     """Stochastic mode with the same Philox stream: identical channel trajectories."""
     pr = _tiny(tt=True)
@@ -140,12 +140,13 @@ def test_spark_box_replay_full_mesh():
     sch[:, 24] = 1
     sch[100:, 17] = 1
     print(f"spark box n_ele={pr.grid.n_ele}", flush=True)
-    with gpu_context([pr], [F], params=[p]) as ctx:
-        ctx.replay(0, sch)
-        Xg = ctx.state_get(0)
-    Fo = Oracle().steps(grid_dict(pr.grid, pr.aij, pr.nij), p.reals, F.copy(), 200, p.reals[1], gating=False,
-                        schedule=sch)
-    assert rel_err(Xg, Fo.X) <= TOL
+    for order in (S.ORDER_RCM, "block"):
+        with gpu_context([pr], [F], params=[p], order=order) as ctx:
+            ctx.replay(0, sch)
+            Xg = ctx.state_get(0)
+        Fo = Oracle().steps(grid_dict(pr.grid, pr.aij, pr.nij), p.reals, F.copy(), 200, p.reals[1], gating=False,
+                            schedule=sch)
+        assert rel_err(Xg, Fo.X) <= TOL
 
 
 def test_no_channels_and_done_realizations():
