@@ -48,11 +48,13 @@ def algorithmic_bytes(domain, tropc):
     return int(np.sum(16 * live + 57))
 
 
-def setup_plain(pr, order="block"):
+def setup_plain(pr, order="block", tune=None):
     import srs_amd as S
     from srs_amd.workloads import initial_channels, kuhn_block_order
 
     ctx = S.Context(0)
+    tune = dict(tune or {})
+    ctx.tune(**{k: tune[k] for k in ("tile", "plane_pad") if k in tune})
     if order == "block":
         mid = ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, perm=kuhn_block_order(pr.grid))
     else:
@@ -64,13 +66,14 @@ def setup_plain(pr, order="block"):
     ctx.state_set(0, pr.X0)
     ctx.channels_set(0, rs, ls, lvs, R)
     ctx.globals_set(0, 0.0, pr.params.reals[16])
+    ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state") if k in tune})
     return ctx
 
 
-def bench_plain(pr, steps, warmup, order):
+def bench_plain(pr, steps, warmup, order, tune):
     import srs_amd as S
 
-    ctx = setup_plain(pr, order)
+    ctx = setup_plain(pr, order, tune)
     ctx.explicit_step(warmup, S.STEP_GRAPH)
     ctx.sync()
     t0 = time.perf_counter()
@@ -107,11 +110,11 @@ def bench_cpu(pr, budget_s=15.0):
     return n, wall, kind
 
 
-def bench_batched(n_real, steps, warmup, order):
+def bench_batched(n_real, steps, warmup, order, tune):
     import srs_amd as S
     from srs_amd import workloads as W
 
-    ctx, dts, n_ele, nbytes = W.ensemble_context(n_real, order=order)
+    ctx, dts, n_ele, nbytes = W.ensemble_context(n_real, order=order, tune=tune)
     ctx.explicit_step(warmup, S.STEP_GRAPH)
     ctx.sync()
     t0 = time.perf_counter()
@@ -122,7 +125,7 @@ def bench_batched(n_real, steps, warmup, order):
     ctx.close()
     # plain path, one realization at a time, on the h = 40 nm member mesh
     pr = W.prepare_spark()
-    pctx = setup_plain(pr, order)
+    pctx = setup_plain(pr, order, tune)
     pctx.explicit_step(200, S.STEP_GRAPH)
     pctx.sync()
     t0 = time.perf_counter()
@@ -144,6 +147,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work (0: skip)")
     ap.add_argument("--order", choices=("block", "rcm", "given"), default="block",
                     help="device element order: Kuhn cube blocks, reverse Cuthill-McKee, or the reference BFS order")
+    ap.add_argument("--tune", default="auto",
+                    help="'auto' (autotune the sweep on scratch contexts), 'default', or a JSON object of "
+                         "tile/plane_pad/nt_geometry/nt_state")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device copy of 512 MiB (HBM byte-counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -169,7 +175,17 @@ def main():
         f"(setup {time.perf_counter() - t0:.1f} s)")
     dt = float(pr.params.reals[1])
     N = pr.grid.n_ele
-    wall, total_ms, sweep_ms, gate_ms = bench_plain(pr, args.steps, args.warmup, args.order)
+    trials = None
+    if args.tune == "auto":
+        from srs_amd.autotune import autotune
+
+        tune, trials = autotune(lambda cfg: setup_plain(pr, args.order, cfg), log=log)
+        log(f"autotuned: {tune}")
+    elif args.tune == "default":
+        tune = {}
+    else:
+        tune = json.loads(args.tune)
+    wall, total_ms, sweep_ms, gate_ms = bench_plain(pr, args.steps, args.warmup, args.order, tune)
     log(f"plain: {args.steps} steps in {wall:.3f} s ({1e3 * wall / args.steps:.4f} ms/step), sweep {sweep_ms * 1e3:.2f} us,"
         f" gating {gate_ms * 1e3:.2f} us")
     value = args.steps * dt / wall
@@ -186,7 +202,7 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"dyad_box 2x2x1 um, T-tubule r=100 nm, jSR 1.0x1.0x0.04 um, h=20 nm Kuhn tets "
                                f"({N} elements), {pr.grid.n_ryr} RyR + {pr.grid.n_lcc} LCC, 1 realization, FP64",
-                   "n_elements": N, "dt_ms": dt, "element_order": args.order,
+                   "n_elements": N, "dt_ms": dt, "element_order": args.order, "tuning": tune,
                    "parallelism": "single realization, one GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -199,7 +215,7 @@ def main():
                                "sample": f"{n} ExplicitStep calls with gating on the same {N}-element mesh"}
         log(f"cpu {kind}: {n} steps in {cwall:.2f} s")
     if args.batch > 0:
-        b = bench_batched(args.batch, args.batch_steps, 2, args.order)
+        b = bench_batched(args.batch, args.batch_steps, 2, args.order, tune)
         brate = b["sim_ms"] / b["wall"]
         bach = b["nbytes"] / (b["sweep_ms"] * 1e-3) / 1e9
         out["batched"] = {
@@ -211,6 +227,8 @@ def main():
             "plain_sequential_value": b["plain_rate"], "speedup_vs_plain_sequential": brate / b["plain_rate"],
         }
         log(f"batched: {brate:.3f} spark-ms/s vs plain {b['plain_rate']:.4f} ({brate / b['plain_rate']:.1f}x)")
+    if trials:
+        out["autotune"] = [{**cfg, "sweep_us": round(ms * 1e3, 2)} for cfg, ms in trials]
     print(json.dumps(out))
 
 

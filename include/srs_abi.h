@@ -244,6 +244,16 @@ typedef struct srs_ctx srs_ctx;
 
 /* Replaces InitializeDeviceConstants / cudaSetDevice (gpu/RunSimulation.cu:503-506, 869-872). */
 int srs_ctx_create(int device, srs_ctx** out);
+/* Kernel and layout tunables (no reference counterpart; the reference fixes
+ * 256-thread blocks, gpu/RunSimulation.cu:784).  They change speed only,
+ * never results.  Layout keys must be set before the first mesh is added,
+ * the tile before srs_ctx_finalize; the rest at any time. */
+#define SRS_TUNE_TILE 1          /* elements per sweep workgroup: 64, 128, 256 (default), 512 */
+#define SRS_TUNE_NT_GEOMETRY 2   /* 1: non-temporal loads of operator rows and flags */
+#define SRS_TUNE_NT_STATE 3      /* 1: non-temporal stores of the new state */
+#define SRS_TUNE_PLANE_PAD 4     /* extra elements (multiple of 64) between SoA planes */
+int srs_ctx_tune(srs_ctx* ctx, int32_t key, int32_t value);
+
 /* Replaces FreeDomainStruct / FreeGlobalStruct (gpu/RunSimulation.cu:874-901). */
 int srs_ctx_destroy(srs_ctx* ctx);
 

@@ -55,6 +55,10 @@ STEP_GRAPH = 2
 ORDER_AS_GIVEN = 0
 ORDER_RCM = 1
 ORDER_CUSTOM = 2
+TUNE_TILE = 1
+TUNE_NT_GEOMETRY = 2
+TUNE_NT_STATE = 3
+TUNE_PLANE_PAD = 4
 
 
 def _p(a, t):
@@ -286,6 +290,13 @@ class Context:
         self._keep = []
         self.n_real = 0
         self._sizes = []
+
+    def tune(self, **kw):
+        """tile=, nt_geometry=, nt_state=, plane_pad= (see srs_ctx_tune)."""
+        keys = dict(tile=TUNE_TILE, nt_geometry=TUNE_NT_GEOMETRY, nt_state=TUNE_NT_STATE, plane_pad=TUNE_PLANE_PAD)
+        for k, v in kw.items():
+            check(lib().srs_ctx_tune(self._h, keys[k], int(v)))
+        return self
 
     def close(self):
         if self._h:

@@ -92,6 +92,7 @@ _PROTOS = {
                                 C.POINTER(C.c_double)]),
     "srs_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "srs_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "srs_ctx_tune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "srs_ctx_add_mesh": (C.c_int, [C.c_void_p, C.POINTER(SrsMeshDesc), i32p]),
     "srs_ctx_add_realization": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(SrsChanDesc), i32p, f64p,
                                           C.c_uint64, C.c_uint64, i32p]),

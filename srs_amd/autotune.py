@@ -1,0 +1,48 @@
+"""Autotuner for the sweep kernel's launch shape and cache hints.
+
+Candidates: elements per workgroup (tile), non-temporal loads of the
+read-once operator rows (nt_geometry), non-temporal stores of the new state
+(nt_state), and SoA plane padding (plane_pad).  Every candidate computes the
+same bits; only speed differs.  Each trial runs on a scratch context built
+by the caller's factory, so the context that is finally used has taken no
+steps.  Timing is the mean sweep-kernel duration from HIP events.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+from typing import Callable
+
+from . import Context
+
+LAYOUT_KEYS = ("tile", "plane_pad")
+KERNEL_KEYS = ("nt_geometry", "nt_state")
+
+
+def autotune(make: Callable[[dict], Context], tiles=(128, 256, 512), pads=(0, 448), nt_geometry=(0, 1),
+             nt_state=(0, 1), reps: int = 20, warm: int = 3, log=None) -> tuple[dict, list]:
+    """Returns (best configuration, all trials as (config, sweep_ms))."""
+    trials = []
+    for tile, pad in itertools.product(tiles, pads):
+        ctx = make({"tile": tile, "plane_pad": pad})
+        try:
+            for g, s in itertools.product(nt_geometry, nt_state):
+                ctx.tune(nt_geometry=g, nt_state=s)
+                ctx.explicit_step(warm)
+                t0 = time.perf_counter()
+                ms, _ = ctx.time_sweep(reps)
+                cfg = {"tile": tile, "plane_pad": pad, "nt_geometry": g, "nt_state": s}
+                trials.append((cfg, ms))
+                if log:
+                    log(f"autotune {cfg}: sweep {ms * 1e3:.2f} us ({time.perf_counter() - t0:.2f} s)")
+        finally:
+            ctx.close()
+    best = min(trials, key=lambda x: x[1])[0]
+    return best, trials
+
+
+def apply(ctx: Context, cfg: dict, layout: bool) -> Context:
+    """Applies the layout keys (before any mesh is added) or the kernel keys."""
+    keys = LAYOUT_KEYS if layout else KERNEL_KEYS
+    ctx.tune(**{k: v for k, v in cfg.items() if k in keys})
+    return ctx

@@ -35,7 +35,7 @@
 
 namespace {
 
-constexpr int TILE = 256;      // elements per workgroup in k_sweep
+constexpr int MAX_TILE = 512;  // largest k_sweep workgroup (elements per workgroup)
 constexpr int GATE_THREADS = 256;
 constexpr int MAX_RYR = 1024;  // the reference GPU path launches <<<1, N_RyR>>>
 constexpr int MAX_LCC = 1024;
@@ -96,15 +96,29 @@ __device__ __forceinline__ int xcd_tile(int bid, int grid) {
   return (bid % NXCD) * (grid / NXCD) + bid / NXCD;
 }
 
-__global__ __launch_bounds__(TILE) void k_sweep(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
-                                                const int2* __restrict__ tiles, int n_tiles) {
+// Loads of data read once per step (operator rows, flags) may bypass cache
+// allocation so that the state planes stay resident between steps.
+template <bool NT, class T>
+__device__ __forceinline__ T ldg(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, class T>
+__device__ __forceinline__ void stg(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <int BS, bool NTG, bool NTS>
+__global__ __launch_bounds__(BS) void k_sweep(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                                              const int2* __restrict__ tiles, int n_tiles) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   if (b >= n_tiles) return;
   const int2 tl = tiles[b];
   const RealDesc& d = D[tl.x];
   const RealLive& l = L[tl.x];
   if (l.done) return;
-  const int j = tl.y * TILE + threadIdx.x;
+  const int j = tl.y * BS + threadIdx.x;
   if (j >= d.n) return;
 
   const int np = d.np;
@@ -113,11 +127,12 @@ __global__ __launch_bounds__(TILE) void k_sweep(const RealDesc* __restrict__ D, 
   double* __restrict__ Xn = d.X[cur ^ 1];
   const double* P = d.P;
   const double dt = P[SRS_R_T_STEP];
-  const uint8_t f = d.flags[j];
+  const uint8_t f = ldg<NTG>(d.flags + j);
 
-  const double a0 = d.coef[j], a1 = d.coef[np + j], a2 = d.coef[2 * np + j], a3 = d.coef[3 * np + j],
-               a4 = d.coef[4 * np + j];
-  const int n1 = d.nbr[j], n2 = d.nbr[np + j], n3 = d.nbr[2 * np + j], n4 = d.nbr[3 * np + j];
+  const double a0 = ldg<NTG>(d.coef + j), a1 = ldg<NTG>(d.coef + np + j), a2 = ldg<NTG>(d.coef + 2 * np + j),
+               a3 = ldg<NTG>(d.coef + 3 * np + j), a4 = ldg<NTG>(d.coef + 4 * np + j);
+  const int n1 = ldg<NTG>(d.nbr + j), n2 = ldg<NTG>(d.nbr + np + j), n3 = ldg<NTG>(d.nbr + 2 * np + j),
+            n4 = ldg<NTG>(d.nbr + 3 * np + j);
 
   const double* Sca = Xo + srs::CA * np;
   const double* Sdy = Xo + srs::DYE * np;
@@ -213,12 +228,12 @@ __global__ __launch_bounds__(TILE) void k_sweep(const RealDesc* __restrict__ D, 
   }
 
   // :640-644 forward Euler
-  Xn[srs::CA * np + j] = c_ca + rca * dt;
-  Xn[srs::ATP * np + j] = c_at + rat * dt;
-  Xn[srs::CMDN * np + j] = c_cm + rcm * dt;
-  if (live_tr) Xn[srs::TRPN * np + j] = c_tr + rtr * dt;
-  if (live_cs) Xn[srs::CSQN * np + j] = c_cs + rcs * dt;
-  Xn[srs::DYE * np + j] = c_dy + rdy * dt;
+  stg<NTS>(Xn + srs::CA * np + j, c_ca + rca * dt);
+  stg<NTS>(Xn + srs::ATP * np + j, c_at + rat * dt);
+  stg<NTS>(Xn + srs::CMDN * np + j, c_cm + rcm * dt);
+  if (live_tr) stg<NTS>(Xn + srs::TRPN * np + j, c_tr + rtr * dt);
+  if (live_cs) stg<NTS>(Xn + srs::CSQN * np + j, c_cs + rcs * dt);
+  stg<NTS>(Xn + srs::DYE * np + j, c_dy + rdy * dt);
 }
 
 __device__ void write_record(const RealDesc& d, RealLive& l, const double* Ca, double t) {
@@ -454,6 +469,11 @@ struct HostReal {
 
 struct srs_ctx {
   int device = 0;
+  int tile = 256;        // elements per k_sweep workgroup
+  int nt_geometry = 0;   // non-temporal loads of operator rows and flags
+  int nt_state = 0;      // non-temporal stores of the new state
+  int plane_pad = 0;     // extra elements per SoA plane
+  bool layout_fixed = false;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool finalized = false;
@@ -496,9 +516,30 @@ int check_real(srs_ctx* c, int32_t r) {
   return SRS_OK;
 }
 
+using SweepFn = void (*)(const RealDesc*, const RealLive*, const int2*, int);
+
+template <int BS>
+SweepFn pick3(int ntg, int nts) {
+  if (ntg) return nts ? k_sweep<BS, true, true> : k_sweep<BS, true, false>;
+  return nts ? k_sweep<BS, false, true> : k_sweep<BS, false, false>;
+}
+
+SweepFn pick_sweep(const srs_ctx* c) {
+  switch (c->tile) {
+    case 64: return pick3<64>(c->nt_geometry, c->nt_state);
+    case 128: return pick3<128>(c->nt_geometry, c->nt_state);
+    case 512: return pick3<512>(c->nt_geometry, c->nt_state);
+    default: return pick3<256>(c->nt_geometry, c->nt_state);
+  }
+}
+
+void launch_sweep(srs_ctx* c) {
+  hipLaunchKernelGGL(pick_sweep(c), dim3(c->grid_sweep), dim3(c->tile), 0, c->stream, c->d_desc, c->d_live,
+                     c->d_tiles, c->n_tiles);
+}
+
 int launch_step(srs_ctx* c, uint32_t gmode) {
-  hipLaunchKernelGGL(k_sweep, dim3(c->grid_sweep), dim3(TILE), 0, c->stream, c->d_desc, c->d_live, c->d_tiles,
-                     c->n_tiles);
+  launch_sweep(c);
   hipLaunchKernelGGL(k_gate, dim3((unsigned)c->reals.size()), dim3(GATE_THREADS), 0, c->stream, c->d_desc,
                      c->d_live, gmode);
   return SRS_OK;
@@ -531,6 +572,27 @@ int srs_ctx_create(int device, srs_ctx** out) {
   return SRS_OK;
 }
 
+int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
+  srs::clear_error();
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  switch (key) {
+    case SRS_TUNE_TILE:
+      if (value != 64 && value != 128 && value != 256 && value != 512)
+        return srs::fail(SRS_E_ARG, "tile must be 64, 128, 256 or 512");
+      if (c->finalized) return srs::fail(SRS_E_STATE, "tile must be set before srs_ctx_finalize");
+      c->tile = value;
+      return SRS_OK;
+    case SRS_TUNE_PLANE_PAD:
+      if (value < 0 || value % 64 || value > 1 << 20) return srs::fail(SRS_E_ARG, "plane pad must be a multiple of 64");
+      if (c->layout_fixed) return srs::fail(SRS_E_STATE, "plane pad must be set before the first mesh");
+      c->plane_pad = value;
+      return SRS_OK;
+    case SRS_TUNE_NT_GEOMETRY: c->nt_geometry = value != 0; return SRS_OK;
+    case SRS_TUNE_NT_STATE: c->nt_state = value != 0; return SRS_OK;
+    default: return srs::fail(SRS_E_ARG, "unknown tuning key %d", key);
+  }
+}
+
 int srs_ctx_destroy(srs_ctx* c) {
   if (!c) return SRS_OK;
   (void)hipSetDevice(c->device);
@@ -555,7 +617,8 @@ int srs_ctx_add_mesh(srs_ctx* c, const srs_mesh_desc* m, int32_t* mesh_id) {
     if (m->nij[k] < 0 || m->nij[k] >= n) return srs::fail(SRS_E_ARG, "nij[%lld] = %d out of range", (long long)k, m->nij[k]);
   auto hm = std::make_unique<HostMesh>();
   hm->n = n;
-  hm->np = (n + TILE - 1) / TILE * TILE;
+  c->layout_fixed = true;
+  hm->np = (n + MAX_TILE - 1) / MAX_TILE * MAX_TILE + c->plane_pad;
   const int np = hm->np;
   // element order inside the context
   if (m->order == SRS_ORDER_RCM) {
@@ -738,7 +801,7 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
     lv.vm = hr.reals[SRS_R_DEFAULT_VM];
     lv.ca_nsr = hr.reals[SRS_R_DEFAULT_CA_NSR];
     lives.push_back(lv);
-    const int nt = (hm.n + TILE - 1) / TILE;
+    const int nt = (hm.n + c->tile - 1) / c->tile;
     for (int t = 0; t < nt; ++t) tiles.push_back(make_int2((int)r, t));
   }
   c->n_tiles = (int)tiles.size();
@@ -1027,8 +1090,7 @@ int srs_time_sweep(srs_ctx* c, int32_t reps, double* sweep_ms, double* gating_ms
   const uint32_t gmode = G_AFTER_SWEEP;
   for (int k = 0; k < reps; ++k) {
     HIPCHK(hipEventRecord(ev[3 * k], c->stream));
-    hipLaunchKernelGGL(k_sweep, dim3(c->grid_sweep), dim3(TILE), 0, c->stream, c->d_desc, c->d_live, c->d_tiles,
-                       c->n_tiles);
+    launch_sweep(c);
     HIPCHK(hipEventRecord(ev[3 * k + 1], c->stream));
     hipLaunchKernelGGL(k_gate, dim3((unsigned)c->reals.size()), dim3(GATE_THREADS), 0, c->stream, c->d_desc,
                        c->d_live, gmode);

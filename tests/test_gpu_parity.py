@@ -164,3 +164,27 @@ def test_no_channels_and_done_realizations():
     Fo = Oracle().steps(grid_dict(pr.grid, pr.aij, pr.nij), pr.params.reals, F.copy(), 50, pr.params.reals[1],
                         gating=True, rng_kind=RNG_PHILOX, key=1000, stream=0)
     assert rel_err(X0, Fo.X) <= TOL
+
+
+@pytest.mark.parametrize("cfg", [dict(tile=64), dict(tile=128, nt_geometry=1), dict(tile=512, nt_state=1),
+                                 dict(tile=256, plane_pad=448, nt_geometry=1, nt_state=1)])
+def test_tunables_do_not_change_results(cfg):
+    pr = _tiny(tt=True)
+    p = fast_gating(pr.params)
+    F = initial_fields(pr, seed=8)
+    outs = []
+    for tune in ({}, cfg):
+        ctx = S.Context(0)
+        ctx.tune(**{k: v for k, v in tune.items() if k in ("tile", "plane_pad")})
+        mid = ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, perm=W.kuhn_block_order(pr.grid))
+        g = pr.grid
+        ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=4, stream=0)
+        ctx.finalize()
+        ctx.tune(**{k: v for k, v in tune.items() if k in ("nt_geometry", "nt_state")})
+        ctx.state_set(0, F.X)
+        ctx.channels_set(0, F.ryr_state, F.lcc_state, F.lccv_state, F.R)
+        ctx.globals_set(0, F.gv[0], F.gv[1])
+        ctx.explicit_step(200, S.STEP_GRAPH)
+        outs.append((ctx.state_get(0), ctx.channels_get(0)[0]))
+        ctx.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
