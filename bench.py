@@ -54,7 +54,7 @@ def setup_plain(pr, order="block", tune=None):
 
     ctx = S.Context(0)
     tune = dict(tune or {})
-    ctx.tune(**{k: tune[k] for k in ("tile", "plane_pad") if k in tune})
+    ctx.tune(**{k: tune[k] for k in ("tile", "plane_pad", "compact") if k in tune})
     if order == "block":
         mid = ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, perm=kuhn_block_order(pr.grid))
     else:
@@ -66,7 +66,7 @@ def setup_plain(pr, order="block", tune=None):
     ctx.state_set(0, pr.X0)
     ctx.channels_set(0, rs, ls, lvs, R)
     ctx.globals_set(0, 0.0, pr.params.reals[16])
-    ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state") if k in tune})
+    ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state", "occupancy") if k in tune})
     return ctx
 
 

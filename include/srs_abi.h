@@ -248,10 +248,13 @@ int srs_ctx_create(int device, srs_ctx** out);
  * 256-thread blocks, gpu/RunSimulation.cu:784).  They change speed only,
  * never results.  Layout keys must be set before the first mesh is added,
  * the tile before srs_ctx_finalize; the rest at any time. */
-#define SRS_TUNE_TILE 1          /* elements per sweep workgroup: 64, 128, 256 (default), 512 */
+#define SRS_TUNE_TILE 1          /* elements per sweep workgroup: 128, 256 (default), 512 */
 #define SRS_TUNE_NT_GEOMETRY 2   /* 1: non-temporal loads of operator rows and flags */
 #define SRS_TUNE_NT_STATE 3      /* 1: non-temporal stores of the new state */
 #define SRS_TUNE_PLANE_PAD 4     /* extra elements (multiple of 64) between SoA planes */
+#define SRS_TUNE_COMPACT 5       /* 1 (default): store operator rows as class + int16 ulp deltas and
+                                    neighbours as int16 offsets where that reproduces them exactly */
+#define SRS_TUNE_OCCUPANCY 6     /* 8: sweep built for eight wavefronts per SIMD (<= 64 VGPRs); 0: default */
 int srs_ctx_tune(srs_ctx* ctx, int32_t key, int32_t value);
 
 /* Replaces FreeDomainStruct / FreeGlobalStruct (gpu/RunSimulation.cu:874-901). */

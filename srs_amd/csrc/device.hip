@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -46,8 +47,14 @@ enum : uint8_t { F_JSR = 1, F_TROPC = 2, F_SL = 4, F_BR = 8, F_CHAN = 16 };
 // Immutable per-realization descriptor (read with scalar loads).
 struct RealDesc {
   const int32_t* nbr;     // [4][np]
-  const double* coef;     // [5][np]
+  const double* coef;     // [5][np] (plain layout)
+  const int16_t* off;     // compact layout: [4][np] neighbour offsets j' - j
+  const uint16_t* cls;    // compact layout: [np] operator-row class
+  const int16_t* dq;      // compact layout: [5][np] coefficient deltas in ulps of the class row
+  const double* tab;      // compact layout: [n_class][10] class row (5 values) and its ulps (5 values)
+  int32_t n_class;
   const uint8_t* flags;   // [np]
+  const uint16_t* wsum;   // [np/64] per-wavefront flag summary: OR | AND << 8
   const double* V0;       // [np]
   const double* tt;       // [np]
   const double* br;       // [4][np] Ca, Fluo, ATP, CaM boundary rates, or null
@@ -78,6 +85,10 @@ struct RealDesc {
 // Mutable per-realization loop state.
 struct RealLive {
   double t, dt_last, vm, ca_nsr, t_lcc_close, ca_max_last;
+  // CaNSR-dependent SERCA subexpressions of cpu/RunSimulation.cpp:85-88, the
+  // same for every element: 0.104217 + 17.923 K_sr, 1.75583e6 + K_sr 7.61673e6,
+  // 6.08463e11 + K_sr 4.50544e11, 9.17846e6 - 11478.2 K_sr, 0.329904 K_sr
+  double serca[5];
   int64_t step;
   int64_t sched_base;
   int32_t done, lcc_open, n_open_last, cur, rec_count, rec_overflow;
@@ -97,47 +108,69 @@ __device__ __forceinline__ int xcd_tile(int bid, int grid) {
 }
 
 // Loads of data read once per step (operator rows, flags) may bypass cache
-// allocation so that the state planes stay resident between steps.
+// allocation so that the state planes stay resident between steps.  Pointers
+// read from the descriptor are generic; casting them to the global address
+// space lets the compiler use global (not flat) memory instructions.
+#define SRS_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const SRS_GLOBAL T* gp(const T* p) { return (const SRS_GLOBAL T*)p; }
+template <class T>
+__device__ __forceinline__ SRS_GLOBAL T* gpw(T* p) { return (SRS_GLOBAL T*)p; }
+
 template <bool NT, class T>
-__device__ __forceinline__ T ldg(const T* p) {
+__device__ __forceinline__ T ldg(const SRS_GLOBAL T* p) {
   if constexpr (NT) return __builtin_nontemporal_load(p);
   else return *p;
 }
 template <bool NT, class T>
-__device__ __forceinline__ void stg(T* p, T v) {
+__device__ __forceinline__ void stg(SRS_GLOBAL T* p, T v) {
   if constexpr (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }
 
-template <int BS, bool NTG, bool NTS>
-__global__ __launch_bounds__(BS) void k_sweep(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
-                                              const int2* __restrict__ tiles, int n_tiles) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  if (b >= n_tiles) return;
-  const int2 tl = tiles[b];
-  const RealDesc& d = D[tl.x];
-  const RealLive& l = L[tl.x];
-  if (l.done) return;
-  const int j = tl.y * BS + threadIdx.x;
-  if (j >= d.n) return;
-
+// One element of the fused step.  FK >= 0: the caller knows the element's
+// flags (a wavefront whose lanes are all alike), so none are loaded.
+template <bool NTG, bool NTS, bool COMPACT, int FK>
+__device__ __forceinline__ void sweep_one(const RealDesc& d, const RealLive& l, const int j) {
   const int np = d.np;
   const int cur = l.cur;
-  const double* __restrict__ Xo = d.X[cur];
-  double* __restrict__ Xn = d.X[cur ^ 1];
+  const SRS_GLOBAL double* __restrict__ Xo = gp(d.X[cur]);
+  SRS_GLOBAL double* __restrict__ Xn = gpw(d.X[cur ^ 1]);
   const double* P = d.P;
   const double dt = P[SRS_R_T_STEP];
-  const uint8_t f = ldg<NTG>(d.flags + j);
+  const uint8_t f = FK >= 0 ? (uint8_t)FK : ldg<NTG>(gp(d.flags) + j);
 
-  const double a0 = ldg<NTG>(d.coef + j), a1 = ldg<NTG>(d.coef + np + j), a2 = ldg<NTG>(d.coef + 2 * np + j),
-               a3 = ldg<NTG>(d.coef + 3 * np + j), a4 = ldg<NTG>(d.coef + 4 * np + j);
-  const int n1 = ldg<NTG>(d.nbr + j), n2 = ldg<NTG>(d.nbr + np + j), n3 = ldg<NTG>(d.nbr + 2 * np + j),
-            n4 = ldg<NTG>(d.nbr + 3 * np + j);
+  double a0, a1, a2, a3, a4;
+  int n1, n2, n3, n4;
+  if constexpr (COMPACT) {
+    // Operator row = class row + small integer multiples of its ulps, which
+    // reproduces the assembled FP64 values exactly (every row was checked
+    // when the mesh was added).
+    const SRS_GLOBAL int16_t* off = gp(d.off);
+    n1 = j + ldg<NTG>(off + j);
+    n2 = j + ldg<NTG>(off + np + j);
+    n3 = j + ldg<NTG>(off + 2 * np + j);
+    n4 = j + ldg<NTG>(off + 3 * np + j);
+    const SRS_GLOBAL double* t = gp(d.tab) + 10 * (int)ldg<NTG>(gp(d.cls) + j);
+    const SRS_GLOBAL int16_t* dq = gp(d.dq);
+    a0 = t[0] + (double)ldg<NTG>(dq + j) * t[5];
+    a1 = t[1] + (double)ldg<NTG>(dq + np + j) * t[6];
+    a2 = t[2] + (double)ldg<NTG>(dq + 2 * np + j) * t[7];
+    a3 = t[3] + (double)ldg<NTG>(dq + 3 * np + j) * t[8];
+    a4 = t[4] + (double)ldg<NTG>(dq + 4 * np + j) * t[9];
+  } else {
+    const SRS_GLOBAL double* cf = gp(d.coef);
+    const SRS_GLOBAL int32_t* nb = gp(d.nbr);
+    a0 = ldg<NTG>(cf + j); a1 = ldg<NTG>(cf + np + j); a2 = ldg<NTG>(cf + 2 * np + j);
+    a3 = ldg<NTG>(cf + 3 * np + j); a4 = ldg<NTG>(cf + 4 * np + j);
+    n1 = ldg<NTG>(nb + j); n2 = ldg<NTG>(nb + np + j); n3 = ldg<NTG>(nb + 2 * np + j);
+    n4 = ldg<NTG>(nb + 3 * np + j);
+  }
 
-  const double* Sca = Xo + srs::CA * np;
-  const double* Sdy = Xo + srs::DYE * np;
-  const double* Sat = Xo + srs::ATP * np;
-  const double* Scm = Xo + srs::CMDN * np;
+  const SRS_GLOBAL double* Sca = Xo + srs::CA * np;
+  const SRS_GLOBAL double* Sdy = Xo + srs::DYE * np;
+  const SRS_GLOBAL double* Sat = Xo + srs::ATP * np;
+  const SRS_GLOBAL double* Scm = Xo + srs::CMDN * np;
   const double c_ca = Sca[j], c_dy = Sdy[j], c_at = Sat[j], c_cm = Scm[j];
   // TnC has a nonzero rate only in cytosol elements with troponin, CSQ only
   // in the jSR (cpu/RunSimulation.cpp:45-46 reset both rates to zero).  A
@@ -150,10 +183,11 @@ __global__ __launch_bounds__(BS) void k_sweep(const RealDesc* __restrict__ D, co
 
   double br_ca = 0.0, br_dy = 0.0, br_at = 0.0, br_cm = 0.0;
   if (f & F_BR) {
-    br_ca = d.br[j];
-    br_dy = d.br[np + j];
-    br_at = d.br[2 * np + j];
-    br_cm = d.br[3 * np + j];
+    const SRS_GLOBAL double* br = gp(d.br);
+    br_ca = br[j];
+    br_dy = br[np + j];
+    br_at = br[2 * np + j];
+    br_cm = br[3 * np + j];
   }
   // cpu/RunSimulation.cpp:61-71 (Ca uses D_CA in both domains)
   double rca = P[SRS_R_D_CA] * (c_ca * a0 + Sca[n1] * a1 + Sca[n2] * a2 + Sca[n3] * a3 + Sca[n4] * a4) + br_ca;
@@ -170,12 +204,8 @@ __global__ __launch_bounds__(BS) void k_sweep(const RealDesc* __restrict__ D, co
       rtr = r;
       double K_i = c_ca / P[SRS_R_K_D_I];
       K_i = K_i * K_i;
-      double K_sr = nsr / P[SRS_R_K_D_SR];
-      K_sr = K_sr * K_sr;
-      const double D_cycle = 0.104217 + 17.923 * K_sr + K_i * (1.75583e6 + K_sr * 7.61673e6) +
-                             K_i * K_i * (6.08463e11 + K_sr * 4.50544e11);
-      const double v_cycle =
-          (K_i * K_i * 3.24873e12 + K_i * (9.17846e6 - 11478.2 * K_sr) - 0.329904 * K_sr) / D_cycle;
+      const double D_cycle = l.serca[0] + K_i * l.serca[1] + K_i * K_i * l.serca[2];
+      const double v_cycle = (K_i * K_i * 3.24873e12 + K_i * l.serca[3] - l.serca[4]) / D_cycle;
       const double d_ca = (2e-3) * v_cycle * P[SRS_R_A_P];
       rca -= d_ca;
     }
@@ -220,8 +250,9 @@ __global__ __launch_bounds__(BS) void k_sweep(const RealDesc* __restrict__ D, co
   }
 
   if (f & F_SL) {  // :15-22; B == 1 exactly where the TT area is zero
-    const double btothi = d.tt[j] * P[SRS_R_B_TOT_SL_HI] / (d.V0[j] * (1e-15));
-    const double btotlo = d.tt[j] * P[SRS_R_B_TOT_SL_LO] / (d.V0[j] * (1e-15));
+    const double ttj = gp(d.tt)[j], v0j = gp(d.V0)[j];
+    const double btothi = ttj * P[SRS_R_B_TOT_SL_HI] / (v0j * (1e-15));
+    const double btotlo = ttj * P[SRS_R_B_TOT_SL_LO] / (v0j * (1e-15));
     const double kh = P[SRS_R_K_D_SL_HI], kl = P[SRS_R_K_D_SL_LO];
     const double B = 1 + (btothi * kh / ((kh + c_ca) * (kh + c_ca))) + (btotlo * kl / ((kl + c_ca) * (kl + c_ca)));
     rca /= B;
@@ -234,6 +265,41 @@ __global__ __launch_bounds__(BS) void k_sweep(const RealDesc* __restrict__ D, co
   if (live_tr) stg<NTS>(Xn + srs::TRPN * np + j, c_tr + rtr * dt);
   if (live_cs) stg<NTS>(Xn + srs::CSQN * np + j, c_cs + rcs * dt);
   stg<NTS>(Xn + srs::DYE * np + j, c_dy + rdy * dt);
+}
+
+// Flags shared by every lane of a wavefront-sized element group: the low byte
+// is the OR of the lanes' flags, the high byte the AND.  A wavefront of plain
+// cytosol elements with troponin takes the path with the flags known.
+constexpr uint8_t F_SPECIAL = F_JSR | F_SL | F_BR | F_CHAN;
+
+template <int BS, bool NTG, bool NTS, bool COMPACT>
+__device__ __forceinline__ void sweep_tile(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                                           const int2* __restrict__ tiles, int n_tiles) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  if (b >= n_tiles) return;
+  const int2 tl = tiles[b];
+  const RealDesc& d = D[tl.x];
+  const RealLive& l = L[tl.x];
+  if (l.done) return;
+  const int j = tl.y * BS + threadIdx.x;
+  if (j >= d.n) return;
+  const int wave = __builtin_amdgcn_readfirstlane(j >> 6);
+  const uint16_t ws = d.wsum[wave];
+  if ((ws & F_SPECIAL) == 0 && ((ws >> 8) & F_TROPC)) sweep_one<NTG, NTS, COMPACT, F_TROPC>(d, l, j);
+  else sweep_one<NTG, NTS, COMPACT, -1>(d, l, j);
+}
+
+// Two register budgets: the compiler's choice, or at most 64 VGPRs so that
+// eight wavefronts share each SIMD.
+template <int BS, bool NTG, bool NTS, bool COMPACT>
+__global__ __launch_bounds__(BS) void k_sweep(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                                              const int2* __restrict__ tiles, int n_tiles) {
+  sweep_tile<BS, NTG, NTS, COMPACT>(D, L, tiles, n_tiles);
+}
+template <int BS, bool NTG, bool NTS, bool COMPACT>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_sweep_o8(
+    const RealDesc* __restrict__ D, const RealLive* __restrict__ L, const int2* __restrict__ tiles, int n_tiles) {
+  sweep_tile<BS, NTG, NTS, COMPACT>(D, L, tiles, n_tiles);
 }
 
 __device__ void write_record(const RealDesc& d, RealLive& l, const double* Ca, double t) {
@@ -450,10 +516,100 @@ struct HostMesh {
   std::vector<double> coef;   // [5][np]
   std::vector<uint8_t> flags;
   std::vector<double> V0, tt, br;  // br [4][np] or empty
+  // compact operator layout
+  bool compact = false;
+  std::vector<int16_t> off, dq;
+  std::vector<uint16_t> cls;
+  std::vector<double> tab;
+  int32_t n_class = 0;
+  std::vector<uint16_t> wsum;
+  uint16_t* d_wsum = nullptr;
   int32_t *d_nbr = nullptr;
-  double *d_coef = nullptr, *d_V0 = nullptr, *d_tt = nullptr, *d_br = nullptr;
+  double *d_coef = nullptr, *d_V0 = nullptr, *d_tt = nullptr, *d_br = nullptr, *d_tab = nullptr;
   uint8_t* d_flags = nullptr;
+  int16_t *d_off = nullptr, *d_dq = nullptr;
+  uint16_t* d_cls = nullptr;
 };
+
+// Rounds the significand to `bits` bits (class key of a coefficient).
+double quantize(double a, int bits) {
+  if (a == 0.0 || !std::isfinite(a)) return a;
+  int e;
+  const double m = std::frexp(a, &e);
+  return std::ldexp(std::nearbyint(std::ldexp(m, bits)), e - bits);
+}
+
+// Builds the compact operator layout: neighbours as int16 offsets, and each
+// row as a class row plus int16 multiples of the class row's ulps.  Leaves
+// hm.compact false unless every row of the mesh reconstructs exactly.
+void build_compact(HostMesh& hm) {
+  const int n = hm.n, np = hm.np;
+  for (int q = 0; q < 4; ++q)
+    for (int k = 0; k < n; ++k) {
+      const long o = (long)hm.nbr[(size_t)q * np + k] - k;
+      if (o < -32767 || o > 32767) return;
+    }
+  std::map<std::array<double, 5>, int> classes;
+  std::vector<int> row_class(n);
+  for (int k = 0; k < n; ++k) {
+    std::array<double, 5> key;
+    for (int q = 0; q < 5; ++q) key[q] = quantize(hm.coef[(size_t)q * np + k], 40);
+    auto it = classes.find(key);
+    if (it == classes.end()) {
+      if (classes.size() >= 65535) return;
+      it = classes.emplace(key, (int)classes.size()).first;
+    }
+    row_class[k] = it->second;
+  }
+  const int nc = (int)classes.size();
+  std::vector<double> tab(10 * (size_t)nc);
+  for (auto& kv : classes)
+    for (int q = 0; q < 5; ++q) {
+      const double b = kv.first[q];
+      tab[10 * (size_t)kv.second + q] = b;
+      int e;
+      std::frexp(b, &e);
+      tab[10 * (size_t)kv.second + 5 + q] = (b == 0.0) ? 0.0 : std::ldexp(1.0, e - 53);  // ulp of b
+    }
+  std::vector<int16_t> off(4 * (size_t)np, 0), dq(5 * (size_t)np, 0);
+  std::vector<uint16_t> cls(np, 0);
+  for (int k = 0; k < n; ++k) {
+    for (int q = 0; q < 4; ++q) off[(size_t)q * np + k] = (int16_t)(hm.nbr[(size_t)q * np + k] - k);
+    const double* t = &tab[10 * (size_t)row_class[k]];
+    int16_t d[5];
+    for (int q = 0; q < 5; ++q) {
+      const double a = hm.coef[(size_t)q * np + k];
+      if (t[5 + q] == 0.0) {
+        if (!(a == t[q] && !std::signbit(a))) return;
+        d[q] = 0;
+        continue;
+      }
+      const double m = (a - t[q]) / t[5 + q];
+      if (!(std::fabs(m) <= 32767.0) || m != std::nearbyint(m)) return;
+      d[q] = (int16_t)m;
+      if ((t[q] + (double)d[q] * t[5 + q]) != a) return;  // exact, as the kernel computes it
+    }
+    cls[k] = (uint16_t)row_class[k];
+    for (int q = 0; q < 5; ++q) dq[(size_t)q * np + k] = d[q];
+  }
+  hm.off.swap(off);
+  hm.dq.swap(dq);
+  hm.cls.swap(cls);
+  hm.tab.swap(tab);
+  hm.n_class = nc;
+  hm.compact = true;
+}
+
+// cpu/RunSimulation.cpp:85-88, the parts that depend on CaNSR only.
+void serca_terms(double nsr, const double* P, double* out) {
+  double K_sr = nsr / P[SRS_R_K_D_SR];
+  K_sr = K_sr * K_sr;
+  out[0] = 0.104217 + 17.923 * K_sr;
+  out[1] = 1.75583e6 + K_sr * 7.61673e6;
+  out[2] = 6.08463e11 + K_sr * 4.50544e11;
+  out[3] = 9.17846e6 - 11478.2 * K_sr;
+  out[4] = 0.329904 * K_sr;
+}
 
 struct HostReal {
   int32_t mesh = 0;
@@ -473,6 +629,9 @@ struct srs_ctx {
   int nt_geometry = 0;   // non-temporal loads of operator rows and flags
   int nt_state = 0;      // non-temporal stores of the new state
   int plane_pad = 0;     // extra elements per SoA plane
+  int compact = 1;       // compress operator rows when a mesh allows it
+  int occ8 = 0;          // sweep compiled for eight wavefronts per SIMD
+  bool compact_all = false;
   bool layout_fixed = false;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -518,20 +677,26 @@ int check_real(srs_ctx* c, int32_t r) {
 
 using SweepFn = void (*)(const RealDesc*, const RealLive*, const int2*, int);
 
-template <int BS>
-SweepFn pick3(int ntg, int nts) {
-  if (ntg) return nts ? k_sweep<BS, true, true> : k_sweep<BS, true, false>;
-  return nts ? k_sweep<BS, false, true> : k_sweep<BS, false, false>;
+template <int BS, bool CMP>
+SweepFn pick3(int ntg, int nts, int occ8) {
+  if (occ8) {
+    if (ntg) return nts ? k_sweep_o8<BS, true, true, CMP> : k_sweep_o8<BS, true, false, CMP>;
+    return nts ? k_sweep_o8<BS, false, true, CMP> : k_sweep_o8<BS, false, false, CMP>;
+  }
+  if (ntg) return nts ? k_sweep<BS, true, true, CMP> : k_sweep<BS, true, false, CMP>;
+  return nts ? k_sweep<BS, false, true, CMP> : k_sweep<BS, false, false, CMP>;
 }
 
-SweepFn pick_sweep(const srs_ctx* c) {
+template <bool CMP>
+SweepFn pick_tile(const srs_ctx* c) {
   switch (c->tile) {
-    case 64: return pick3<64>(c->nt_geometry, c->nt_state);
-    case 128: return pick3<128>(c->nt_geometry, c->nt_state);
-    case 512: return pick3<512>(c->nt_geometry, c->nt_state);
-    default: return pick3<256>(c->nt_geometry, c->nt_state);
+    case 128: return pick3<128, CMP>(c->nt_geometry, c->nt_state, c->occ8);
+    case 512: return pick3<512, CMP>(c->nt_geometry, c->nt_state, c->occ8);
+    default: return pick3<256, CMP>(c->nt_geometry, c->nt_state, c->occ8);
   }
 }
+
+SweepFn pick_sweep(const srs_ctx* c) { return c->compact_all ? pick_tile<true>(c) : pick_tile<false>(c); }
 
 void launch_sweep(srs_ctx* c) {
   hipLaunchKernelGGL(pick_sweep(c), dim3(c->grid_sweep), dim3(c->tile), 0, c->stream, c->d_desc, c->d_live,
@@ -577,8 +742,7 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
   if (!c) return srs::fail(SRS_E_ARG, "null context");
   switch (key) {
     case SRS_TUNE_TILE:
-      if (value != 64 && value != 128 && value != 256 && value != 512)
-        return srs::fail(SRS_E_ARG, "tile must be 64, 128, 256 or 512");
+      if (value != 128 && value != 256 && value != 512) return srs::fail(SRS_E_ARG, "tile must be 128, 256 or 512");
       if (c->finalized) return srs::fail(SRS_E_STATE, "tile must be set before srs_ctx_finalize");
       c->tile = value;
       return SRS_OK;
@@ -587,7 +751,12 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
       if (c->layout_fixed) return srs::fail(SRS_E_STATE, "plane pad must be set before the first mesh");
       c->plane_pad = value;
       return SRS_OK;
+    case SRS_TUNE_COMPACT:
+      if (c->layout_fixed) return srs::fail(SRS_E_STATE, "compact layout must be chosen before the first mesh");
+      c->compact = value != 0;
+      return SRS_OK;
     case SRS_TUNE_NT_GEOMETRY: c->nt_geometry = value != 0; return SRS_OK;
+    case SRS_TUNE_OCCUPANCY: c->occ8 = value == 8; return SRS_OK;
     case SRS_TUNE_NT_STATE: c->nt_state = value != 0; return SRS_OK;
     default: return srs::fail(SRS_E_ARG, "unknown tuning key %d", key);
   }
@@ -673,6 +842,7 @@ int srs_ctx_add_mesh(srs_ctx* c, const srs_mesh_desc* m, int32_t* mesh_id) {
     }
     hm->flags[k] = f;
   }
+  if (c->compact) build_compact(*hm);
   *mesh_id = (int32_t)c->meshes.size();
   c->meshes.push_back(std::move(hm));
   return SRS_OK;
@@ -741,11 +911,30 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
   if (c->reals.empty()) return srs::fail(SRS_E_STATE, "no realizations");
   HIPCHK(hipSetDevice(c->device));
   int rc;
+  c->compact_all = !c->meshes.empty();
+  for (auto& hmp : c->meshes) c->compact_all &= hmp->compact;
+  for (auto& hmp : c->meshes) {  // per-wavefront flag summaries (flags are final now)
+    HostMesh& hm = *hmp;
+    hm.wsum.assign(hm.np / 64, 0);
+    for (int w = 0; w < hm.np / 64; ++w) {
+      uint8_t o = 0, a = 0xFF;
+      for (int k = 64 * w; k < std::min(64 * w + 64, hm.n); ++k) { o |= hm.flags[k]; a &= hm.flags[k]; }
+      hm.wsum[w] = (uint16_t)(o | (a << 8));
+    }
+  }
   for (auto& hmp : c->meshes) {
     HostMesh& hm = *hmp;
-    if ((rc = dev_upload(c, &hm.d_nbr, hm.nbr))) return rc;
-    if ((rc = dev_upload(c, &hm.d_coef, hm.coef))) return rc;
+    if (!c->compact_all) {
+      if ((rc = dev_upload(c, &hm.d_nbr, hm.nbr))) return rc;
+      if ((rc = dev_upload(c, &hm.d_coef, hm.coef))) return rc;
+    } else {
+      if ((rc = dev_upload(c, &hm.d_off, hm.off))) return rc;
+      if ((rc = dev_upload(c, &hm.d_cls, hm.cls))) return rc;
+      if ((rc = dev_upload(c, &hm.d_dq, hm.dq))) return rc;
+      if ((rc = dev_upload(c, &hm.d_tab, hm.tab))) return rc;
+    }
     if ((rc = dev_upload(c, &hm.d_flags, hm.flags))) return rc;
+    if ((rc = dev_upload(c, &hm.d_wsum, hm.wsum))) return rc;
     if ((rc = dev_upload(c, &hm.d_V0, hm.V0))) return rc;
     if ((rc = dev_upload(c, &hm.d_tt, hm.tt))) return rc;
     if (!hm.br.empty() && (rc = dev_upload(c, &hm.d_br, hm.br))) return rc;
@@ -758,6 +947,8 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
     HostMesh& hm = *c->meshes[hr.mesh];
     RealDesc d{};
     d.nbr = hm.d_nbr; d.coef = hm.d_coef; d.flags = hm.d_flags; d.V0 = hm.d_V0; d.tt = hm.d_tt; d.br = hm.d_br;
+    d.off = hm.d_off; d.cls = hm.d_cls; d.dq = hm.d_dq; d.tab = hm.d_tab; d.wsum = hm.d_wsum;
+    d.n_class = hm.n_class;
     d.n = hm.n; d.np = hm.np;
     double *x0, *x1;
     if ((rc = dev_alloc(c, &x0, 6 * (size_t)hm.np)) || (rc = dev_alloc(c, &x1, 6 * (size_t)hm.np))) return rc;
@@ -800,6 +991,7 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
     RealLive lv{};
     lv.vm = hr.reals[SRS_R_DEFAULT_VM];
     lv.ca_nsr = hr.reals[SRS_R_DEFAULT_CA_NSR];
+    serca_terms(lv.ca_nsr, hr.reals, lv.serca);
     lives.push_back(lv);
     const int nt = (hm.n + c->tile - 1) / c->tile;
     for (int t = 0; t < nt; ++t) tiles.push_back(make_int2((int)r, t));
@@ -892,6 +1084,7 @@ int srs_globals_set(srs_ctx* c, int32_t r, double vm, double ca_nsr) {
   if ((rc = live_rw(c, r, &lv, false))) return rc;
   lv.vm = vm;
   lv.ca_nsr = ca_nsr;
+  serca_terms(ca_nsr, c->reals[r].reals, lv.serca);
   return live_rw(c, r, &lv, true);
 }
 

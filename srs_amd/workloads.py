@@ -251,7 +251,7 @@ def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0
     members = ensemble_members(n_real, seed)
     ctx = Context(0)
     tune = dict(tune or {})
-    ctx.tune(**{k: tune[k] for k in ("tile", "plane_pad") if k in tune})
+    ctx.tune(**{k: tune[k] for k in ("tile", "plane_pad", "compact") if k in tune})
     meshes = []
     for m in range(3):
         g = box_grid(ensemble_mesh_spec(m))
@@ -288,7 +288,7 @@ def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0
         R = np.log(rng.random(nr + g.n_lcc) * (1 - 2**-53) + 2**-54)
         ctx.channels_set(r, rs, np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R)
         ctx.globals_set(r, 0.0, p.reals[16])
-    ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state") if k in tune})
+    ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state", "occupancy") if k in tune})
     return ctx, np.array(dts), n_ele, nbytes
 
 

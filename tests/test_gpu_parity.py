@@ -166,8 +166,8 @@ def test_no_channels_and_done_realizations():
     assert rel_err(X0, Fo.X) <= TOL
 
 
-@pytest.mark.parametrize("cfg", [dict(tile=64), dict(tile=128, nt_geometry=1), dict(tile=512, nt_state=1),
-                                 dict(tile=256, plane_pad=448, nt_geometry=1, nt_state=1)])
+@pytest.mark.parametrize("cfg", [dict(tile=128, occupancy=8), dict(tile=128, nt_geometry=1), dict(tile=512, nt_state=1),
+                                 dict(tile=256, plane_pad=448, nt_geometry=1, nt_state=1), dict(compact=0)])
 def test_tunables_do_not_change_results(cfg):
     pr = _tiny(tt=True)
     p = fast_gating(pr.params)
@@ -175,12 +175,12 @@ def test_tunables_do_not_change_results(cfg):
     outs = []
     for tune in ({}, cfg):
         ctx = S.Context(0)
-        ctx.tune(**{k: v for k, v in tune.items() if k in ("tile", "plane_pad")})
+        ctx.tune(**{k: v for k, v in tune.items() if k in ("tile", "plane_pad", "compact")})
         mid = ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, perm=W.kuhn_block_order(pr.grid))
         g = pr.grid
         ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=4, stream=0)
         ctx.finalize()
-        ctx.tune(**{k: v for k, v in tune.items() if k in ("nt_geometry", "nt_state")})
+        ctx.tune(**{k: v for k, v in tune.items() if k in ("nt_geometry", "nt_state", "occupancy")})
         ctx.state_set(0, F.X)
         ctx.channels_set(0, F.ryr_state, F.lcc_state, F.lccv_state, F.R)
         ctx.globals_set(0, F.gv[0], F.gv[1])
