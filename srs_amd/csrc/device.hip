@@ -44,6 +44,16 @@ constexpr int NXCD = 8;        // MI355X: 8 XCDs, workgroups dispatched round-ro
 
 enum : uint8_t { F_JSR = 1, F_TROPC = 2, F_SL = 4, F_BR = 8, F_CHAN = 16 };
 
+// State layout of one realization (6*np doubles): the four diffusing species
+// interleaved per element, (Ca, ATP, CaM, Fluo) in 32 bytes, so that one
+// neighbour gather fetches all four with one contiguous access; then the TnC
+// and CSQ planes, which are never gathered.
+constexpr int MOB_SLOT[6] = {0, 1, 2, -1, -1, 3};  // species -> slot in the interleaved quad
+__host__ __device__ inline size_t xoff(int s, size_t e, size_t np) {
+  return s == srs::TRPN ? 4 * np + e : s == srs::CSQN ? 5 * np + e : 4 * e + MOB_SLOT[s];
+}
+typedef double d4 __attribute__((ext_vector_type(4)));
+
 // Immutable per-realization descriptor (read with scalar loads).
 struct RealDesc {
   const int32_t* nbr;     // [4][np]
@@ -167,19 +177,18 @@ __device__ __forceinline__ void sweep_one(const RealDesc& d, const RealLive& l, 
     n4 = ldg<NTG>(nb + 3 * np + j);
   }
 
-  const SRS_GLOBAL double* Sca = Xo + srs::CA * np;
-  const SRS_GLOBAL double* Sdy = Xo + srs::DYE * np;
-  const SRS_GLOBAL double* Sat = Xo + srs::ATP * np;
-  const SRS_GLOBAL double* Scm = Xo + srs::CMDN * np;
-  const double c_ca = Sca[j], c_dy = Sdy[j], c_at = Sat[j], c_cm = Scm[j];
+  const SRS_GLOBAL d4* Mo = (const SRS_GLOBAL d4*)Xo;
+  const d4 m0 = Mo[j];
+  const double c_ca = m0.x, c_at = m0.y, c_cm = m0.z, c_dy = m0.w;
   // TnC has a nonzero rate only in cytosol elements with troponin, CSQ only
   // in the jSR (cpu/RunSimulation.cpp:45-46 reset both rates to zero).  A
   // zero rate leaves the value unchanged, and both state buffers hold the same
   // value, so those planes are neither read nor written elsewhere.
   const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
   const bool live_cs = (f & F_JSR) != 0;
-  const double c_tr = live_tr ? Xo[srs::TRPN * np + j] : 0.0;
-  const double c_cs = live_cs ? Xo[srs::CSQN * np + j] : 0.0;
+  const double c_tr = live_tr ? Xo[4 * np + j] : 0.0;
+  const double c_cs = live_cs ? Xo[5 * np + j] : 0.0;
+  const d4 m1 = Mo[n1], m2 = Mo[n2], m3 = Mo[n3], m4 = Mo[n4];
 
   double br_ca = 0.0, br_dy = 0.0, br_at = 0.0, br_cm = 0.0;
   if (f & F_BR) {
@@ -190,10 +199,10 @@ __device__ __forceinline__ void sweep_one(const RealDesc& d, const RealLive& l, 
     br_cm = br[3 * np + j];
   }
   // cpu/RunSimulation.cpp:61-71 (Ca uses D_CA in both domains)
-  double rca = P[SRS_R_D_CA] * (c_ca * a0 + Sca[n1] * a1 + Sca[n2] * a2 + Sca[n3] * a3 + Sca[n4] * a4) + br_ca;
-  double rdy = P[SRS_R_D_DYE] * (c_dy * a0 + Sdy[n1] * a1 + Sdy[n2] * a2 + Sdy[n3] * a3 + Sdy[n4] * a4) + br_dy;
-  double rat = P[SRS_R_D_ATP] * (c_at * a0 + Sat[n1] * a1 + Sat[n2] * a2 + Sat[n3] * a3 + Sat[n4] * a4) + br_at;
-  double rcm = P[SRS_R_D_CMDN] * (c_cm * a0 + Scm[n1] * a1 + Scm[n2] * a2 + Scm[n3] * a3 + Scm[n4] * a4) + br_cm;
+  double rca = P[SRS_R_D_CA] * (c_ca * a0 + m1.x * a1 + m2.x * a2 + m3.x * a3 + m4.x * a4) + br_ca;
+  double rdy = P[SRS_R_D_DYE] * (c_dy * a0 + m1.w * a1 + m2.w * a2 + m3.w * a3 + m4.w * a4) + br_dy;
+  double rat = P[SRS_R_D_ATP] * (c_at * a0 + m1.y * a1 + m2.y * a2 + m3.y * a3 + m4.y * a4) + br_at;
+  double rcm = P[SRS_R_D_CMDN] * (c_cm * a0 + m1.z * a1 + m2.z * a2 + m3.z * a3 + m4.z * a4) + br_cm;
   double rtr = 0, rcs = 0;
   const double nsr = l.ca_nsr;
 
@@ -238,7 +247,7 @@ __device__ __forceinline__ void sweep_one(const RealDesc& d, const RealLive& l, 
         if (c >= 0) {
           const int i = c >> 1;
           if (d.ryr_state[i]) {
-            const double J = P[SRS_R_V_RYR] * (Sca[d.ryr_jsr[i]] - Sca[d.ryr_ele[i]]) * P[SRS_R_V_CELL];
+            const double J = P[SRS_R_V_RYR] * (Xo[4 * d.ryr_jsr[i]] - Xo[4 * d.ryr_ele[i]]) * P[SRS_R_V_CELL];
             if (c & 1) rca -= J / v0; else rca += J / v0;
           }
         } else {
@@ -259,12 +268,10 @@ __device__ __forceinline__ void sweep_one(const RealDesc& d, const RealLive& l, 
   }
 
   // :640-644 forward Euler
-  stg<NTS>(Xn + srs::CA * np + j, c_ca + rca * dt);
-  stg<NTS>(Xn + srs::ATP * np + j, c_at + rat * dt);
-  stg<NTS>(Xn + srs::CMDN * np + j, c_cm + rcm * dt);
-  if (live_tr) stg<NTS>(Xn + srs::TRPN * np + j, c_tr + rtr * dt);
-  if (live_cs) stg<NTS>(Xn + srs::CSQN * np + j, c_cs + rcs * dt);
-  stg<NTS>(Xn + srs::DYE * np + j, c_dy + rdy * dt);
+  const d4 mn = {c_ca + rca * dt, c_at + rat * dt, c_cm + rcm * dt, c_dy + rdy * dt};
+  stg<NTS>((SRS_GLOBAL d4*)Xn + j, mn);
+  if (live_tr) stg<NTS>(Xn + 4 * np + j, c_tr + rtr * dt);
+  if (live_cs) stg<NTS>(Xn + 5 * np + j, c_cs + rcs * dt);
 }
 
 // Flags shared by every lane of a wavefront-sized element group: the low byte
@@ -310,7 +317,7 @@ __device__ void write_record(const RealDesc& d, RealLive& l, const double* Ca, d
   rec[k++] = (float)t;
   double i_ryr = 0;
   for (int i = 0; i < d.n_ryr; ++i)
-    i_ryr += d.ryr_state[i] * d.P[SRS_R_V_RYR] * (Ca[d.ryr_jsr[i]] - Ca[d.ryr_ele[i]]) * (d.P[SRS_R_V_CELL]) * 96485 *
+    i_ryr += d.ryr_state[i] * d.P[SRS_R_V_RYR] * (Ca[4 * d.ryr_jsr[i]] - Ca[4 * d.ryr_ele[i]]) * (d.P[SRS_R_V_CELL]) * 96485 *
              2 * (1e-6);
   rec[k++] = (float)i_ryr;
   int n_open = 0;
@@ -323,18 +330,18 @@ __device__ void write_record(const RealDesc& d, RealLive& l, const double* Ca, d
   double i_lcc = 0;
   for (int i = 0; i < d.n_lcc; ++i)
     if (srs_lcc_open(d.lcc_state[i], d.lccv_state[i]))
-      i_lcc += srs_lcc_flux_raw(l.vm, Ca[d.lcc_ele[i]], d.P[SRS_R_CA_0]) * 96485 * 2 * (1e-6);
+      i_lcc += srs_lcc_flux_raw(l.vm, Ca[4 * d.lcc_ele[i]], d.P[SRS_R_CA_0]) * 96485 * 2 * (1e-6);
   rec[k++] = (float)i_lcc;
   double m = 0;
-  for (int i = 0; i < d.n_ryr; ++i) m += Ca[d.ryr_jsr[i]];
+  for (int i = 0; i < d.n_ryr; ++i) m += Ca[4 * d.ryr_jsr[i]];
   m /= d.n_ryr;
   rec[k++] = (float)m;
   m = 0;
-  for (int i = 0; i < d.n_ryr; ++i) m += Ca[d.ryr_ele[i]];
+  for (int i = 0; i < d.n_ryr; ++i) m += Ca[4 * d.ryr_ele[i]];
   m /= d.n_ryr;
   rec[k++] = (float)m;
   if (d.gpu_layout)
-    for (int i = 0; i < d.n_ryr; ++i) rec[k++] = (float)Ca[d.ryr_ele[i]];
+    for (int i = 0; i < d.n_ryr; ++i) rec[k++] = (float)Ca[4 * d.ryr_ele[i]];
   if (d.ryr_out)
     for (int i = 0; i < d.n_ryr; ++i) rec[k++] = (float)d.ryr_state[i];
   l.rec_count++;
@@ -351,7 +358,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
   const int tid = threadIdx.x;
   const int cur = l.cur;
   const double* Xn = (mode & G_AFTER_SWEEP) ? d.X[cur ^ 1] : d.X[cur];
-  const double* Ca = Xn;  // species 0 plane
+  const double* Ca = Xn;  // Ca of element e at Ca[4 * e] (interleaved diffusing species)
   const double* P = d.P;
   const double dt = P[SRS_R_T_STEP];
   const int64_t step = l.step;
@@ -381,7 +388,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
               if (s) no++; else ncl++;
             }
           }
-          const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[d.ryr_ele[i]], Ca[d.ryr_jsr[i]], P);
+          const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[4 * d.ryr_ele[i]], Ca[4 * d.ryr_jsr[i]], P);
           const double Rn = d.R[i] + rate * dt;
           const int ns = (Rn > 0) ? (s_old[i] == 1 ? 0 : 1) : s_old[i];
           next[i] = ns;
@@ -403,7 +410,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
           if (s) no++; else ncl++;
         }
       }
-      const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[d.ryr_ele[i]], Ca[d.ryr_jsr[i]], P);
+      const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[4 * d.ryr_ele[i]], Ca[4 * d.ryr_jsr[i]], P);
       double Rn = d.R[i] + rate * dt;
       int ns = s_old[i];
       if (Rn > 0) {
@@ -420,7 +427,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
       const int s = d.lcc_state[i], vs = d.lccv_state[i];
       double tr[3];
       int32_t tg[3];
-      const double lr = srs_lcc_rates(s, vs, l.vm, Ca[d.lcc_ele[i]], tr, tg);
+      const double lr = srs_lcc_rates(s, vs, l.vm, Ca[4 * d.lcc_ele[i]], tr, tg);
       const int ic = nr + i;
       double Rn = d.R[ic] + (tr[0] + tr[1] + tr[2] + lr) * dt;
       if (Rn > 0) {
@@ -475,7 +482,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
     double cmax = 0;
     for (int i = 0; i < nr; ++i) {
       n_open += d.ryr_state[i];
-      const double c = Ca[d.ryr_ele[i]];
+      const double c = Ca[4 * d.ryr_ele[i]];
       if (c > cmax) cmax = c;
     }
     l.n_open_last = n_open;
@@ -487,12 +494,12 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
 // Gathers Ca at the channel elements of one realization.
 __global__ void k_channel_ca(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int r, double* out) {
   const RealDesc& d = D[r];
-  const double* Ca = d.X[L[r].cur];
+  const double* Ca = d.X[L[r].cur];  // Ca of element e at Ca[4 * e]
   for (int i = threadIdx.x; i < d.n_ryr; i += blockDim.x) {
-    out[i] = Ca[d.ryr_ele[i]];
-    out[d.n_ryr + i] = Ca[d.ryr_jsr[i]];
+    out[i] = Ca[4 * d.ryr_ele[i]];
+    out[d.n_ryr + i] = Ca[4 * d.ryr_jsr[i]];
   }
-  for (int i = threadIdx.x; i < d.n_lcc; i += blockDim.x) out[2 * d.n_ryr + i] = Ca[d.lcc_ele[i]];
+  for (int i = threadIdx.x; i < d.n_lcc; i += blockDim.x) out[2 * d.n_ryr + i] = Ca[4 * d.lcc_ele[i]];
 }
 
 // ----------------------------------------------------------------- host side
@@ -1017,7 +1024,7 @@ int srs_state_set(srs_ctx* c, int32_t r, const double* X) {
   const HostMesh& hm = *c->meshes[hr.mesh];
   std::vector<double> buf(6 * (size_t)hm.np, 0.0);
   for (int s = 0; s < 6; ++s)
-    for (int i = 0; i < hm.n; ++i) buf[(size_t)s * hm.np + hm.inv[i]] = X[(size_t)s * hm.n + i];
+    for (int i = 0; i < hm.n; ++i) buf[xoff(s, hm.inv[i], hm.np)] = X[(size_t)s * hm.n + i];
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(hr.desc.X[0], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -1038,7 +1045,7 @@ int srs_state_get(srs_ctx* c, int32_t r, double* X) {
   std::vector<double> buf(6 * (size_t)hm.np);
   HIPCHK(hipMemcpy(buf.data(), hr.desc.X[lv.cur], buf.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int s = 0; s < 6; ++s)
-    for (int i = 0; i < hm.n; ++i) X[(size_t)s * hm.n + i] = buf[(size_t)s * hm.np + hm.inv[i]];
+    for (int i = 0; i < hm.n; ++i) X[(size_t)s * hm.n + i] = buf[xoff(s, hm.inv[i], hm.np)];
   return SRS_OK;
 }
 
