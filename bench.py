@@ -9,8 +9,10 @@ that realization with its inputs resident in HBM.
 Printed as one JSON line:
   value        simulated spark time per wall second (spark-ms/s), plain path
   roofline     the fused sweep kernel against MI355X HBM bandwidth: algorithmic
-               bytes per launch (153 B per element) / mean kernel time from HIP
-               events on the context's stream
+               bytes per launch (161 B per element: 6 FP64 species read and
+               written, a 4 x int32 + 5 x FP64 operator row, 9 B of per-element
+               constants) / mean kernel time from HIP events on the context's
+               stream
   cpu_baseline the reference's own cpu/RunSimulation.cpp ExplicitStep
                (oracle/_ref) on the same mesh, 1 host core, bounded sample
   batched      the ensemble workload: many independent spark realizations on
@@ -38,14 +40,36 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(domain, tropc):
-    """HBM bytes one sweep must move per step: each live FP64 species read and
-    written once (Ca, ATP, CaM, Fluo everywhere; TnC in cytosol elements with
-    troponin; CSQ in the jSR), the ELL row (4 int32 + 5 FP64) and a 1-byte
-    flag word.  Neighbour gathers are counted as cache hits."""
+BYTES_PER_ELEMENT = 161
+
+
+def algorithmic_bytes(n_ele):
+    """HBM bytes one explicit step must move per element: 6 FP64 species read
+    and written (96 B), the ELL row of 4 int32 + 5 FP64 (56 B) and 9 B of
+    per-element constants (the SL buffer capacity and a domain/troponin flag
+    byte).  Neighbour gathers are counted as cache hits."""
+    return BYTES_PER_ELEMENT * int(n_ele)
+
+
+def live_bytes(domain, tropc):
+    """The bytes this build's sweep actually has to move: species that are
+    identically zero (TnC outside troponin-bearing cytosol, CSQ outside the
+    jSR) are neither read nor written."""
     jsr = np.asarray(domain) != 0
     live = 4 + ((~jsr) & (np.asarray(tropc) != 0)) + jsr
-    return int(np.sum(16 * live + 57))
+    return int(np.sum(16 * live + 65))
+
+
+def profiled_traffic(path):
+    """HBM bytes per sweep launch from a committed rocprofv3 summary: the line
+    'traffic_bytes_per_launch: N' (FETCH_SIZE + WRITE_SIZE, calibrated)."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def setup_plain(pr, order="block", tune=None):
@@ -151,7 +175,7 @@ def main():
                     help="'auto' (autotune the sweep on scratch contexts), 'default', or a JSON object of "
                          "tile/plane_pad/nt_geometry/nt_state")
     ap.add_argument("--calib-copy", action="store_true",
-                    help="also run one device copy of 512 MiB (HBM byte-counter calibration under rocprofv3 --pmc)")
+                    help="also run one device copy of 2 GiB (HBM byte-counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the spark step does not shard across GPUs "
@@ -163,11 +187,12 @@ def main():
     if args.calib_copy:
         import torch
 
-        a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        a = torch.ones(256 << 20, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
+        torch.cuda.synchronize()
         b.copy_(a)
         torch.cuda.synchronize()
-        log("calibration copy: 512 MiB read + 512 MiB written")
+        log("calibration copy: 2 GiB read + 2 GiB written")
         del a, b
     t0 = time.perf_counter()
     pr = W.prepare_dyad()
@@ -189,13 +214,9 @@ def main():
     log(f"plain: {args.steps} steps in {wall:.3f} s ({1e3 * wall / args.steps:.4f} ms/step), sweep {sweep_ms * 1e3:.2f} us,"
         f" gating {gate_ms * 1e3:.2f} us")
     value = args.steps * dt / wall
-    nbytes = algorithmic_bytes(pr.grid.domain, pr.grid.tropc)
+    nbytes = algorithmic_bytes(N)
     achieved = nbytes / (sweep_ms * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "sweep_dyad_traffic.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("traffic_bytes_per_launch")
+    traffic = profiled_traffic(os.path.join(ROOT, "profiles", "k_sweep_dyad_traffic.md"))
     out = {
         "metric": METRIC, "value": value, "unit": "spark-ms simulated/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -207,7 +228,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sweep", "kernel_ms": sweep_ms, "bytes_per_launch": nbytes,
-                     "bytes_per_element": nbytes / N},
+                     "bytes_per_element": BYTES_PER_ELEMENT,
+                     "live_bytes_per_launch": live_bytes(pr.grid.domain, pr.grid.tropc)},
     }
     if args.cpu_budget > 0:
         n, cwall, kind = bench_cpu(pr, args.cpu_budget)

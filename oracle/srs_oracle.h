@@ -13,7 +13,7 @@
  * Pinning (what checks each function, see tests/):
  *   orc_explicit_step / orc_gating   -> bit-exact against oracle/_ref (the
  *                                       reference cpu/RunSimulation.cpp itself,
- *                                       compiled by oracle/build_ref.sh), and
+ *                                       compiled by oracle/build.py), and
  *                                       against closed forms (equilibrium fixed
  *                                       point, mass conservation, Po law).
  *   orc_philox4x64                   -> numpy.random.Philox (independent impl.)
@@ -22,10 +22,15 @@
  *   orc_bfs_order                    -> BFS characterisation (levels, FIFO parent
  *                                       order, face-order tie break).
  *   orc_equilibrium_state            -> binding-reaction fixed point.
- *   orc_states_record                -> recomputation from fields in numpy.
- *   orc_open_random_ryr              -> parity unpinned beyond a rate-weighted
- *                                       selection frequency test (the reference
- *                                       loader/init file is not buildable here).
+ *   orc_boundary_rates               -> bit-exact against oracle/_ref, and the
+ *                                       closed form D Face_C b / V0 per face.
+ *   orc_states_record                -> a hand-computed record (every field
+ *                                       from its definition).
+ *   orc_open_random_ryr              -> selection law: RyR i is picked with
+ *                                       frequency rate_i / sum(rates), checked
+ *                                       in two limits where the rate is known
+ *                                       in closed form (linear phi alone, then
+ *                                       cytosolic Ca alone).
  */
 #ifndef SRS_ORACLE_H_
 #define SRS_ORACLE_H_

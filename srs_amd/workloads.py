@@ -244,8 +244,10 @@ def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0
                      order: str = "block", tune: dict | None = None):
     """The batched Monte Carlo workload in one context: returns (ctx, dt per
     realization, total elements over realizations, algorithmic sweep bytes
-    per step: live species of every realization plus each distinct mesh's
-    operator rows once)."""
+    per step).  Algorithmic bytes: 96 B per element of every realization (6
+    FP64 species read and written) plus 65 B per element of each distinct
+    mesh (the 4 int32 + 5 FP64 operator row and 9 B of per-element
+    constants), counted once per step however many realizations share it."""
     from . import ORDER_AS_GIVEN, ORDER_RCM, Context
 
     members = ensemble_members(n_real, seed)
@@ -265,7 +267,7 @@ def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0
     dts, n_ele, nbytes = [], 0, 0
     plist = []
     for g, *_ in meshes:
-        nbytes += 57 * g.n_ele
+        nbytes += 65 * g.n_ele
     for r, mem in enumerate(members):
         g, aij, nij, mid, dt = meshes[mem["mesh"]]
         idx, nb = lattice_subset(mem["mask"])
@@ -275,8 +277,7 @@ def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0
         ctx.add_realization(mid, g.ryr_ele[idx], g.ryr_jsr[idx], nb, g.lcc_ele, p, seed=seed + r, stream=r)
         dts.append(dt)
         n_ele += g.n_ele
-        jsr = g.domain != 0
-        nbytes += int(np.sum(16 * (4 + ((~jsr) & (g.tropc != 0)) + jsr)))
+        nbytes += 96 * g.n_ele
         plist.append((g, p, len(idx)))
     ctx.finalize(record_capacity)
     for r, (g, p, nr) in enumerate(plist):
