@@ -322,6 +322,9 @@ typedef struct srs_loop_state {
   double t_lcc_close;        /* T_LCC_Close */
   int32_t n_ryr_open_last;   /* N_RyR_Open as of the last states output */
   double ca_ss_max_last;     /* ca_ss_max as of the last states output */
+  int32_t ffwd;              /* 1 while the flag_ffwd gating-only phase runs
+                                (cpu/RunSimulation.cpp:469-519): steps of this
+                                realization gate channels on the unchanged fields */
 } srs_loop_state;
 int srs_loop_set(srs_ctx* ctx, int32_t real, const srs_loop_state* ls);
 int srs_loop_get(srs_ctx* ctx, int32_t real, srs_loop_state* ls);

@@ -171,6 +171,15 @@ class Oracle:
                 self.L.orc_gating(dt, C.byref(g), _f64(P), _f64(F.X), _f64(F.gv), C.byref(c), C.byref(rng))
         return F
 
+    def gating(self, G, reals, F: Fields, dt, rng_kind=RNG_PHILOX, key=0, stream=0, step=0):
+        """UpdateChannelGating alone (the flag_ffwd loop), Philox counter `step`."""
+        g = self._grid(G)
+        c = self._chan(G, F)
+        P = np.ascontiguousarray(reals, np.float64)
+        rng = OrcRng(rng_kind, key, PHILOX_KEY1, stream, step)
+        self.L.orc_gating(dt, C.byref(g), _f64(P), _f64(F.X), _f64(F.gv), C.byref(c), C.byref(rng))
+        return F
+
     def record(self, t, G, F: Fields, reals, ryr_out=False, gpu_layout=False):
         c = self._chan(G, F)
         rec = np.zeros(512, np.float32)

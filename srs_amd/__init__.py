@@ -401,8 +401,9 @@ class Context:
         check(lib().srs_globals_get(self._h, r, C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    def loop_set(self, r, t=0.0, dt_last=0.0, step=0, done=0, lcc_open=0, t_lcc_close=0.0, n_open=0, ca_max=0.0):
-        s = _abi.SrsLoopState(t, dt_last, step, done, lcc_open, t_lcc_close, n_open, ca_max)
+    def loop_set(self, r, t=0.0, dt_last=0.0, step=0, done=0, lcc_open=0, t_lcc_close=0.0, n_open=0, ca_max=0.0,
+                 ffwd=0):
+        s = _abi.SrsLoopState(t, dt_last, step, done, lcc_open, t_lcc_close, n_open, ca_max, ffwd)
         check(lib().srs_loop_set(self._h, r, C.byref(s)))
 
     def loop_get(self, r) -> dict:

@@ -70,7 +70,7 @@ class SrsLoopState(C.Structure):
     _fields_ = [
         ("t", C.c_double), ("dt_last", C.c_double), ("step", C.c_int64), ("done", C.c_int32),
         ("lcc_open", C.c_int32), ("t_lcc_close", C.c_double), ("n_ryr_open_last", C.c_int32),
-        ("ca_ss_max_last", C.c_double),
+        ("ca_ss_max_last", C.c_double), ("ffwd", C.c_int32),
     ]
 
 

@@ -102,6 +102,7 @@ struct RealLive {
   int64_t step;
   int64_t sched_base;
   int32_t done, lcc_open, n_open_last, cur, rec_count, rec_overflow;
+  int32_t ffwd;  // gating-only phase of flag_ffwd: the sweep skips this realization
 };
 
 enum : uint32_t {
@@ -287,7 +288,7 @@ __device__ __forceinline__ void sweep_tile(const RealDesc* __restrict__ D, const
   const int2 tl = tiles[b];
   const RealDesc& d = D[tl.x];
   const RealLive& l = L[tl.x];
-  if (l.done) return;
+  if (l.done | l.ffwd) return;
   const int j = tl.y * BS + threadIdx.x;
   if (j >= d.n) return;
   const int wave = __builtin_amdgcn_readfirstlane(j >> 6);
@@ -357,12 +358,18 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
   __shared__ int s_b[MAX_RYR];
   const int tid = threadIdx.x;
   const int cur = l.cur;
-  const double* Xn = (mode & G_AFTER_SWEEP) ? d.X[cur ^ 1] : d.X[cur];
+  // In the flag_ffwd phase the sweep skipped this realization: the fields are
+  // unchanged and only the channels gate (cpu/RunSimulation.cpp:474-476).
+  const bool ff = l.ffwd != 0;
+  const bool swept = (mode & G_AFTER_SWEEP) && !ff;
+  const double* Xn = swept ? d.X[cur ^ 1] : d.X[cur];
   const double* Ca = Xn;  // Ca of element e at Ca[4 * e] (interleaved diffusing species)
   const double* P = d.P;
   const double dt = P[SRS_R_T_STEP];
   const int64_t step = l.step;
-  const bool gating = !(mode & G_NO_GATING) && !d.I[SRS_I_FLAG_NO_GATING];
+  // ExplicitStep honours flag_nogating; the ffwd loop calls
+  // UpdateChannelGating directly (cpu/RunSimulation.cpp:476, 650-652).
+  const bool gating = !(mode & G_NO_GATING) && (ff || !d.I[SRS_I_FLAG_NO_GATING]);
   const int nr = d.n_ryr;
 
   if (gating) {
@@ -452,12 +459,30 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
   }
 
   if (tid != 0) return;
-  if (mode & G_AFTER_SWEEP) l.cur = cur ^ 1;
+  if (swept) l.cur = cur ^ 1;
   l.step = step + 1;
-  const double t = l.t + dt;  // cpu/RunSimulation.cpp:572-573
+  const double t = l.t + dt;  // cpu/RunSimulation.cpp:572-573 (ffwd: :477-478)
   l.t = t;
   l.dt_last = dt;
   if (!(mode & G_PROTOCOL)) return;
+
+  if (ff) {  // the rest of the ffwd loop body, cpu/RunSimulation.cpp:480-518
+    int n_open = 0, n_open_lcc = 0;
+    for (int i = 0; i < nr; ++i) n_open += d.ryr_state[i];
+    for (int i = 0; i < d.n_lcc; ++i)
+      if (d.lccv_state[i] == 2 && (d.lcc_state[i] == 6 || d.lcc_state[i] == 12)) n_open_lcc++;
+    const bool stop = n_open > 0 || n_open_lcc > 0 || t > P[SRS_R_T_FINAL];
+    if (!stop) {
+      const double iv = P[SRS_R_OUTPUT_STATES_INTERVAL];
+      if (floor(t / iv) > floor((t - dt) / iv)) write_record(d, l, Ca, t);
+    }
+    if (d.I[SRS_I_FLAG_V_CLAMP]) l.vm = (t <= P[SRS_R_T_CLAMP]) ? P[SRS_R_V_CLAMP] : P[SRS_R_DEFAULT_VM];
+    if (!stop) return;
+    // The main loop starts at this t with N_RyR_Open from the last ffwd
+    // count; ca_ss_max keeps its initial value.
+    l.ffwd = 0;
+    l.n_open_last = n_open;
+  }
 
   // While-condition at the top of the next iteration (:522).
   const bool fid = d.I[SRS_I_FLAG_FIDELITY_SIM] != 0;
@@ -1113,6 +1138,7 @@ int srs_loop_set(srs_ctx* c, int32_t r, const srs_loop_state* ls) {
   if ((rc = live_rw(c, r, &lv, false))) return rc;
   lv.t = ls->t; lv.dt_last = ls->dt_last; lv.step = ls->step; lv.done = ls->done; lv.lcc_open = ls->lcc_open;
   lv.t_lcc_close = ls->t_lcc_close; lv.n_open_last = ls->n_ryr_open_last; lv.ca_max_last = ls->ca_ss_max_last;
+  lv.ffwd = ls->ffwd ? 1 : 0;
   return live_rw(c, r, &lv, true);
 }
 
@@ -1124,6 +1150,7 @@ int srs_loop_get(srs_ctx* c, int32_t r, srs_loop_state* ls) {
   if ((rc = live_rw(c, r, &lv, false))) return rc;
   ls->t = lv.t; ls->dt_last = lv.dt_last; ls->step = lv.step; ls->done = lv.done; ls->lcc_open = lv.lcc_open;
   ls->t_lcc_close = lv.t_lcc_close; ls->n_ryr_open_last = lv.n_open_last; ls->ca_ss_max_last = lv.ca_max_last;
+  ls->ffwd = lv.ffwd;
   return SRS_OK;
 }
 

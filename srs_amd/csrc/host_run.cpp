@@ -105,7 +105,6 @@ extern "C" int srs_run_cru3d(const char* param_path, const char* mesh_base, cons
   char title[SRS_TITLE_LEN];
   int rc = srs_read_parameters(param_path, I, P, title);
   if (rc) return rc;
-  if (I[SRS_I_GATING_FFWD]) return srs::fail(SRS_E_ARG, "flag_ffwd (gating fast-forward) is not supported by this build");
   if (I[SRS_I_INITIALIZE_FROM_FILE]) return srs::fail(SRS_E_ARG, "initial states from file are not supported");
 
   // Output directory, cpu/main.cpp:30-133
@@ -266,10 +265,17 @@ extern "C" int srs_run_cru3d(const char* param_path, const char* mesh_base, cons
     double vm = P[SRS_R_DEFAULT_VM];
     int lcc_open = I[SRS_I_FLAG_LCC];
     const bool fid = I[SRS_I_FLAG_FIDELITY_SIM] != 0;
+    const bool ffwd = I[SRS_I_GATING_FFWD] != 0;
+    if (ffwd) printf("Using accelerated steady-state gating...\n");
     const double t0 = 0.0;
-    const bool cont = t0 <= P[SRS_R_T_FINAL] &&
-                      (!fid || ((lcc_open || n_open > 0 || 0.0 > P[SRS_R_C_MIN]) && n_open < I[SRS_I_N_RYR_FIDELITY]));
-    if (cont) {
+    // With flag_ffwd the gating-only loop runs first (:469-519, on the device);
+    // the main loop's first iteration follows at the time it stops.
+    const bool cont = ffwd || (t0 <= P[SRS_R_T_FINAL] &&
+                               (!fid || ((lcc_open || n_open > 0 || 0.0 > P[SRS_R_C_MIN]) &&
+                                         n_open < I[SRS_I_N_RYR_FIDELITY])));
+    if (ffwd) {
+      // Vm stays at its default until the first ffwd iteration ends (:510-517)
+    } else if (cont) {
       if (I[SRS_I_FLAG_LCC] && lcc_open && t0 >= t_lcc_close) { lcc_open = 0; ls[I[SRS_I_LCC_INDEX]] = 1; }
       if (I[SRS_I_FLAG_V_CLAMP]) vm = (t0 <= P[SRS_R_T_CLAMP]) ? P[SRS_R_V_CLAMP] : P[SRS_R_DEFAULT_VM];
       if (I[SRS_I_INIT_RYR] != -1 && t0 < P[SRS_R_INIT_RYR_HOLD] && !rs[I[SRS_I_INIT_RYR]]) rs[I[SRS_I_INIT_RYR]] = 1;
@@ -283,6 +289,7 @@ extern "C" int srs_run_cru3d(const char* param_path, const char* mesh_base, cons
     srs_loop_state lsd{};
     lsd.t = 0; lsd.dt_last = 0; lsd.step = 0; lsd.done = cont ? 0 : 1; lsd.lcc_open = lcc_open;
     lsd.t_lcc_close = t_lcc_close; lsd.n_ryr_open_last = n_open; lsd.ca_ss_max_last = 0;
+    lsd.ffwd = ffwd ? 1 : 0;
     if ((rc = srs_loop_set(G.ctx, k, &lsd))) return rc;
   }
 

@@ -87,6 +87,23 @@ def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, ma
     step = 0
     ir = int(Iv[S.I["INIT_RYR"]])
     ryr_out = bool(Iv[S.I["FLAG_RYR_OUT"]])
+    if Iv[S.I["GATING_FFWD"]]:  # cpu/RunSimulation.cpp:469-519
+        while True:
+            O.gating(G, P, F, dt, rng_kind=RNG_PHILOX, key=key, stream=stream, step=step)
+            step += 1
+            t += dt
+            dt_last = dt
+            n_open = int(F.ryr_state.sum())
+            n_lcc = int(np.sum((F.lccv_state == 2) & ((F.lcc_state == 6) | (F.lcc_state == 12))))
+            stop = n_open > 0 or n_lcc > 0 or t > P[S.R["T_FINAL"]]
+            if not stop and output_due(t, dt_last, P[S.R["OUTPUT_STATES_INTERVAL"]]):
+                recs.append(O.record(t, G, F, P, ryr_out=ryr_out))
+            if Iv[S.I["FLAG_V_CLAMP"]]:
+                F.gv[0] = P[S.R["V_CLAMP"]] if t <= P[S.R["T_CLAMP"]] else P[S.R["DEFAULT_VM"]]
+            if stop:
+                break
+            if step > max_steps:
+                raise RuntimeError("oracle ffwd loop did not terminate")
     while t <= P[S.R["T_FINAL"]] and (not fid or ((lcc_open or n_open > 0 or ca_max > P[S.R["C_MIN"]])
                                                   and n_open < Iv[S.I["N_RYR_FIDELITY"]])):
         if Iv[S.I["FLAG_V_CLAMP"]]:

@@ -6,11 +6,13 @@ runs are bit-identical; the bound leaves room for the one-ulp differences of
 device exp/pow/log against glibc in the LCC flux and the gating rates.
 Channel trajectories (integer states) must match exactly.
 """
+import os
+
 import numpy as np
 import pytest
 
 import srs_amd as S
-from oracle.oracle import RNG_PHILOX, Fields, Oracle, grid_dict
+from oracle.oracle import PHILOX_KEY1, RNG_PHILOX, Fields, Oracle, grid_dict
 from srs_amd import workloads as W
 from tests.helpers import fast_gating, gpu_context, initial_fields, oracle_loop, rel_err
 
@@ -129,6 +131,95 @@ def test_protocol_loop_records_match_oracle_loop():
     assert ls["done"] == 1 and ls["step"] == nsteps and ls["t"] == t_end
     assert recs_d.shape[0] == len(recs_o)
     np.testing.assert_array_equal(recs_d, np.array(recs_o))
+
+
+@pytest.mark.parametrize("case", ["opens", "t_final"])
+def test_ffwd_gating_phase_matches_oracle_loop(case):
+    """flag_ffwd: gating-only steps on the unchanged fields until a channel
+    opens (or T_FINAL passes), then the main loop from that time
+    (cpu/RunSimulation.cpp:469-519)."""
+    pr = _tiny(tt=True)
+    p = pr.params.copy()
+    dt = p.reals[1]
+    c0 = float(pr.X0[pr.grid.ryr_ele[0]])
+    # about one RyR opening per 60 steps over the whole cluster
+    kp = 1.0 / (60 * dt * pr.grid.n_ryr * c0 ** p.reals[S.R["RYR_ETA"]]) if case == "opens" else 0.0
+    p.setr(RYR_K_PLUS=kp, RYR_K_MINUS=20.0, T_FINAL=(400 if case == "opens" else 45) * dt,
+           OUTPUT_STATES_INTERVAL=7 * dt, T_CLAMP=1e9, V_CLAMP=0.0, DEFAULT_VM=-80.0)
+    p.seti(INIT_RYR=-1, FLAG_V_CLAMP=1, GATING_FFWD=1, FLAG_RYR_OUT=1)
+    F = initial_fields(pr, ryr_open=(), vm=-80.0, lcc_state=1, seed=9)
+    O = Oracle()
+    recs_o, t_end, nsteps = oracle_loop(O, pr, p, F.copy(), key=33, stream=0)
+    with gpu_context([pr], [F.copy()], params=[p], seeds=[33], record_capacity=256) as ctx:
+        ctx.loop_set(0, n_open=0, ffwd=1)
+        ctx.explicit_step(nsteps + 5, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        recs_d = ctx.records(0)
+        ls = ctx.loop_get(0)
+        X = ctx.state_get(0)
+    assert ls["done"] == 1 and ls["ffwd"] == 0 and ls["step"] == nsteps and ls["t"] == t_end
+    assert recs_d.shape[0] == len(recs_o)
+    np.testing.assert_array_equal(recs_d, np.array(recs_o))
+    if case == "t_final":
+        # nothing opened: the fields never moved
+        np.testing.assert_array_equal(X, F.X)
+    else:
+        assert nsteps > 20 and np.any(X != F.X)
+
+
+def _write_params(path, entries):
+    body = "".join(f"  <parameter><symbol>{k}</symbol><value>{v}</value></parameter>\n" for k, v in entries.items())
+    with open(path, "w") as f:
+        f.write(f'<?xml version="1.0" encoding="UTF-8"?>\n<simulation>\n{body}</simulation>\n')
+
+
+@pytest.mark.parametrize("ffwd", [False, True])
+def test_cru3d_run_matches_oracle_loop(tmp_path, ffwd):
+    """The cru3d program path end to end (parameter XML, mesh files, initial
+    channel protocol, Philox residuals, device loop, states file) against
+    the oracle-driven RunSimulation replica started from the same state."""
+    from types import SimpleNamespace
+
+    base = str(tmp_path / "mesh")
+    S.box_mesh_write(W.tiny_box_spec(tt=True), base)
+    g, _ = S.grid_read(base)
+    aij, nij = S.assemble_operator(g)
+    dt = W.stable_dt(aij, 0.25)
+    c0 = 0.1
+    kp = 1.0 / (60 * dt * g.n_ryr * c0 ** 2.1) if ffwd else 0.5
+    entries = dict(t_step=repr(dt * 1e6), t_final=repr(300 * dt), states_interval=repr(9 * dt),
+                   RYR_K_PLUS=repr(kp), RYR_K_MINUS="20", flag_use_clock="false", seed_base="1234",
+                   flag_output_ryr_states="true", sims_per_proc="1", paramset_title="tiny")
+    if ffwd:
+        entries.update(protocol="gain", flag_ffwd="true", v_clamp="0", t_clamp="1e9")
+    else:
+        entries.update(protocol="none", ryr_open_init="1", ryr_open_time=repr(50 * dt))
+    xml = str(tmp_path / "p.xml")
+    _write_params(xml, entries)
+    out = str(tmp_path / "out")
+    S.run_cru3d(xml, base, out)
+    p = S.read_parameters(xml)
+    assert p.reals[1] == float(repr(dt * 1e6)) * 1e-6
+    recs_d = np.fromfile(os.path.join(out, "states_0_0.out"), np.float32).reshape(-1, 7 + g.n_ryr)
+
+    O = Oracle()
+    P = p.reals
+    seed = (93843 + 1234) & 0xFFFFFFFF
+    nr, nl = g.n_ryr, g.n_lcc
+    rs = np.zeros(nr, np.int32)
+    if not ffwd:
+        rs[1] = 1
+    R = np.array([np.log(O.open01(O.philox([0, i, 0, 1], [seed, PHILOX_KEY1])[0])) for i in range(nr + nl)])
+    X0, _ = S.equilibrium_state(g, P)
+    F = Fields(X0, rs, np.ones(nl, np.int32), np.full(nl, 2, np.int32), R, [P[S.R["DEFAULT_VM"]],
+                                                                          P[S.R["DEFAULT_CA_NSR"]]])
+    G = grid_dict(g, aij, nij)
+    rec0 = O.record(0.0, G, F, P, ryr_out=True)
+    pr = SimpleNamespace(grid=g, aij=aij, nij=nij)
+    recs_o, t_end, nsteps = oracle_loop(O, pr, p, F, key=seed, stream=0)
+    exp = np.array([rec0] + recs_o)
+    assert recs_d.shape == exp.shape
+    np.testing.assert_array_equal(recs_d, exp)
+    assert nsteps > 100
 
 
 def test_spark_box_replay_full_mesh():
