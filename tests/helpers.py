@@ -70,7 +70,7 @@ def output_due(t, dt_last, iv):
     return math.floor(t / iv) > math.floor((t - dt_last) / iv)
 
 
-def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, max_steps=10 ** 7):
+def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, max_steps=10 ** 7, rng_kind=RNG_PHILOX):
     """RunSimulation's main loop with the oracle's step (Philox gating).
     Returns the records written inside and after the loop (the t = 0 record
     before the loop is the caller's)."""
@@ -89,7 +89,7 @@ def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, ma
     ryr_out = bool(Iv[S.I["FLAG_RYR_OUT"]])
     if Iv[S.I["GATING_FFWD"]]:  # cpu/RunSimulation.cpp:469-519
         while True:
-            O.gating(G, P, F, dt, rng_kind=RNG_PHILOX, key=key, stream=stream, step=step)
+            O.gating(G, P, F, dt, rng_kind=rng_kind, key=key, stream=stream, step=step)
             step += 1
             t += dt
             dt_last = dt
@@ -118,7 +118,7 @@ def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, ma
                 if Ca[e] > ca_max:
                     ca_max = Ca[e]
             recs.append(O.record(t, G, F, P, ryr_out=ryr_out))
-        O.steps(G, P, F, 1, dt, gating=not Iv[S.I["FLAG_NO_GATING"]], rng_kind=RNG_PHILOX, key=key, stream=stream,
+        O.steps(G, P, F, 1, dt, gating=not Iv[S.I["FLAG_NO_GATING"]], rng_kind=rng_kind, key=key, stream=stream,
                 step0=step)
         step += 1
         t += dt

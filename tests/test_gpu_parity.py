@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import srs_amd as S
-from oracle.oracle import PHILOX_KEY1, RNG_PHILOX, Fields, Oracle, grid_dict
+from oracle.oracle import PHILOX_KEY1, RNG_GLIBC, RNG_PHILOX, Fields, Oracle, grid_dict
 from srs_amd import workloads as W
 from tests.helpers import fast_gating, gpu_context, initial_fields, oracle_loop, rel_err
 
@@ -164,6 +164,69 @@ def test_ffwd_gating_phase_matches_oracle_loop(case):
         np.testing.assert_array_equal(X, F.X)
     else:
         assert nsteps > 20 and np.any(X != F.X)
+
+
+def _spark_stats(recs, nr):
+    """Per-realization statistics of a states-record sequence (one record per
+    step): first closing step of RyR 0 (its open time), integrated open RyR
+    count, peak mean cleft Ca at the RyRs (amplitude proxy), number of RyR
+    openings."""
+    st = recs[:, 7:7 + nr] > 0.5
+    closed0 = np.nonzero(~st[:, 0])[0]
+    first_close = int(closed0[0]) + 1 if closed0.size else recs.shape[0] + 1
+    openings = int(np.sum(st[1:] & ~st[:-1]))
+    return first_close, float(recs[:, 2].sum()), float(recs[:, 6].max()), openings
+
+
+def test_stochastic_gating_matches_reference_in_distribution():
+    """Philox gating on the device against the reference's own generator
+    (glibc rand(), as cpu/RunSimulation.cpp:208-209, 382-397 draw it; the
+    oracle's glibc mode is bit-exact with the reference step): two-sample KS
+    tests on open time, integrated open count, spark amplitude proxy and
+    number of openings, p > 0.01, plus the open-time law of a lone open RyR."""
+    from scipy import stats
+
+    import ctypes
+
+    n_real, n_steps = 160, 500
+    pr = _tiny()
+    p = pr.params.copy()
+    dt = p.reals[1]
+    # mean open time 120 steps; a closed RyR opens about every 300 steps at 0.2 uM
+    p.setr(RYR_K_MINUS=1 / (120 * dt), RYR_K_PLUS=1 / (300 * dt * 0.2 ** 2.1))
+    p.setr(T_FINAL=(n_steps - 0.5) * dt, OUTPUT_STATES_INTERVAL=dt * (1 - 1e-9), V_CLAMP=0.0, DEFAULT_VM=0.0)
+    p.seti(INIT_RYR=-1, FLAG_V_CLAMP=0, FLAG_RYR_OUT=1)
+    nr = pr.grid.n_ryr
+    fields = [initial_fields(pr, ryr_open=(0,), vm=0.0, lcc_state=1, seed=100 + k) for k in range(n_real)]
+    with gpu_context([pr] * n_real, [F.copy() for F in fields], params=[p] * n_real, seeds=[77] * n_real,
+                     record_capacity=n_steps + 8, share_mesh=True) as ctx:
+        for k in range(n_real):
+            ctx.loop_set(k, n_open=1)
+        ctx.explicit_step(n_steps + 4, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        gpu = [_spark_stats(ctx.records(k), nr) for k in range(n_real)]
+    O = Oracle()
+    libc = ctypes.CDLL(None)
+    ref = []
+    for k in range(n_real):
+        libc.srand(ctypes.c_uint(93843 + 38376427 + 83 * k))
+        recs, _, _ = oracle_loop(O, pr, p, fields[k].copy(), key=0, stream=0, rng_kind=RNG_GLIBC)
+        ref.append(_spark_stats(np.array(recs), nr))
+    gpu, ref = np.array(gpu), np.array(ref)
+    for q, name in enumerate(("open time", "open RyR-steps", "amplitude", "openings")):
+        pv = stats.ks_2samp(gpu[:, q], ref[:, q]).pvalue
+        assert pv > 0.01, (name, pv, gpu[:, q].mean(), ref[:, q].mean())
+    # a lone open RyR closes after ceil(E / (k_minus dt)) steps, E ~ Exp(1):
+    # geometric, here conditioned on closing within the run
+    q = 1.0 - np.exp(-p.reals[S.R["RYR_K_MINUS"]] * dt)
+    n = np.arange(1, n_steps + 1)
+    pmf = (1 - q) ** (n - 1) * q
+    pmf /= pmf.sum()
+    mean_law = float(np.sum(n * pmf))
+    sd_law = float(np.sqrt(np.sum((n - mean_law) ** 2 * pmf)))
+    for arr in (gpu, ref):
+        ft = arr[:, 0][arr[:, 0] <= n_steps]
+        assert abs(ft.mean() - mean_law) < 4 * sd_law / np.sqrt(ft.size)
+    assert ref[:, 3].mean() > 0.5  # the cluster does recruit openings
 
 
 def _write_params(path, entries):
