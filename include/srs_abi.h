@@ -370,6 +370,44 @@ int srs_real_info(srs_ctx* ctx, int32_t real, int32_t* n_ele, int32_t* n_ryr, in
 int srs_last_timing(srs_ctx* ctx, double* total_ms, double* sweep_ms);
 int srs_time_sweep(srs_ctx* ctx, int32_t reps, double* sweep_ms, double* gating_ms);
 
+/* ================================================== linescan (dF/F0 operator) */
+
+/* Parameters of the reference's linescan program (linescan/parameters.h:17-43;
+ * defaults and XML symbols linescan/main.cpp:353-508). */
+typedef struct srs_scan_desc {
+  int32_t n_scans;        /* scan_n */
+  int32_t n_scans_skip;   /* scan_n_per_line (flag_x_res_ca also lands here, as in the reference) */
+  int32_t axis;           /* scan_axis "X" 0, "Y" 1, "Z" 2 */
+  double res_x_f;         /* scan_x_res_f: fluorescence pixel [um] */
+  double res_x_ca;        /* Ca pixel [um] (no XML symbol reaches it; default 0.01) */
+  double res_t;
+  double x_min, x_max;    /* scan_x_min, scan_x_max */
+  double offset1, offset2;/* scan_offset_1, scan_offset_2 */
+  double domain[3];       /* scan_domain_x/y/z */
+  double pad_min, pad_max, pad_res;  /* scan_padding_* */
+  double var[3];          /* scan_var_x/y/z: PSF variances */
+  double f0, c0;          /* scan_f0, scan_c0: values of the padding cells */
+} srs_scan_desc;
+
+/* Replaces the linescan ReadParameters (linescan/main.cpp:353-508); path NULL
+ * gives the defaults.  The root element must be <analysis>. */
+int srs_scan_read_parameters(const char* path, srs_scan_desc* out);
+
+/* Prepares a scan line over mesh `mesh_id` of the context (after
+ * srs_ctx_finalize): centers[3*i + k] is coordinate k of element i's centroid
+ * (reference numbering).  Replaces the geometry half of the linescan main
+ * (linescan/main.cpp:85-141: cell centres, padding grid, scan points).
+ * n_f / n_ca: number of fluorescence and Ca points. */
+int srs_ctx_add_scan(srs_ctx* ctx, int32_t mesh_id, const double* centers, const srs_scan_desc* scan,
+                     int32_t* scan_id, int32_t* n_f, int32_t* n_ca);
+
+/* One scan line of realization `real`'s current state, computed on the device
+ * (linescan/main.cpp:195-249): ca[n_ca] = Ca of the nearest cell,
+ * caf[n_f] = Ca-bound Fluo of the nearest cell, cafb[n_f] = the PSF-blurred
+ * Ca-bound Fluo (sum of V0 * CaF * Gaussian PSF over cells within 1 um,
+ * padding cells included).  dF/F0 = cafb / f0 - 1. */
+int srs_scan_sample(srs_ctx* ctx, int32_t real, int32_t scan_id, double* ca, double* caf, double* cafb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,13 @@ def _load(path):
     return C.CDLL(path)
 
 
+class OrcScan(C.Structure):
+    _fields_ = [("axis", C.c_int), ("x_min", C.c_double), ("x_max", C.c_double), ("res_f", C.c_double),
+                ("res_ca", C.c_double), ("off1", C.c_double), ("off2", C.c_double), ("dom", C.c_double * 3),
+                ("pad_min", C.c_double), ("pad_max", C.c_double), ("pad_res", C.c_double), ("var", C.c_double * 3),
+                ("f0", C.c_double), ("c0", C.c_double)]
+
+
 class Fields:
     """Mutable simulation state shared by the oracle and the reference harness:
     fields X[6*N], channel states, residuals, globals."""
@@ -179,6 +186,21 @@ class Oracle:
         rng = OrcRng(rng_kind, key, PHILOX_KEY1, stream, step)
         self.L.orc_gating(dt, C.byref(g), _f64(P), _f64(F.X), _f64(F.gv), C.byref(c), C.byref(rng))
         return F
+
+    def linescan(self, scan, centers, V0, ca, caf):
+        """linescan/main.cpp:96-249 on one state; scan is any object with the
+        srs_scan_desc field names.  Returns (ca[n_ca], caf[n_f], cafb[n_f])."""
+        s = OrcScan(int(scan.axis), scan.x_min, scan.x_max, scan.res_x_f, scan.res_x_ca, scan.offset1, scan.offset2,
+                    (C.c_double * 3)(*scan.domain), scan.pad_min, scan.pad_max, scan.pad_res,
+                    (C.c_double * 3)(*scan.var), scan.f0, scan.c0)
+        nf, nca = C.c_int(), C.c_int()
+        self.L.orc_linescan_size(C.byref(s), C.byref(nf), C.byref(nca))
+        oca, ocaf, ocafb = np.zeros(max(nca.value, 1)), np.zeros(max(nf.value, 1)), np.zeros(max(nf.value, 1))
+        cen = np.ascontiguousarray(centers, np.float64).reshape(-1)
+        v, a, b = (np.ascontiguousarray(x, np.float64) for x in (V0, ca, caf))
+        self.L.orc_linescan(C.byref(s), len(v), _f64(cen), _f64(v), _f64(a), _f64(b), _f64(oca), _f64(ocaf),
+                            _f64(ocafb))
+        return oca[:nca.value], ocaf[:nf.value], ocafb[:nf.value]
 
     def record(self, t, G, F: Fields, reals, ryr_out=False, gpu_layout=False):
         c = self._chan(G, F)

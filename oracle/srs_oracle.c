@@ -483,3 +483,107 @@ int orc_open_random_ryr(const double* X, int n_ele, const double* P, orc_chan* c
   free(rates);
   return open;
 }
+
+/* ------------------------------------------------------------- linescan */
+/* linescan/parameters.h:45-62: the reference's own value of pi. */
+#define ORC_SCAN_PI 3.14159
+
+typedef struct { double* c; double* v; double* ca; double* caf; int n, cap; } orc_cells;
+
+static void orc_cells_push(orc_cells* m, double x, double y, double z, double v, double ca, double caf) {
+  if (m->n == m->cap) {
+    m->cap = m->cap ? 2 * m->cap : 1024;
+    m->c = (double*)realloc(m->c, 3 * (size_t)m->cap * sizeof(double));
+    m->v = (double*)realloc(m->v, (size_t)m->cap * sizeof(double));
+    m->ca = (double*)realloc(m->ca, (size_t)m->cap * sizeof(double));
+    m->caf = (double*)realloc(m->caf, (size_t)m->cap * sizeof(double));
+  }
+  m->c[3 * m->n] = x; m->c[3 * m->n + 1] = y; m->c[3 * m->n + 2] = z;
+  m->v[m->n] = v; m->ca[m->n] = ca; m->caf[m->n] = caf;
+  m->n++;
+}
+
+/* linescan/main.cpp:268-306 */
+static void orc_add_point_grid(orc_cells* m, const orc_scan* s, double xmin, double xmax, double ymin, double ymax,
+                               double zmin, double zmax) {
+  double dx = s->pad_res, dy = s->pad_res, dz = s->pad_res;
+  if (xmin > xmax) dx = -dx;
+  if (ymin > ymax) dy = -dy;
+  if (zmin > zmax) dz = -dz;
+  int nx = (int)((xmax - xmin) / dx), ny = (int)((ymax - ymin) / dy), nz = (int)((zmax - zmin) / dz);
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++)
+        orc_cells_push(m, xmin + i * dx + 0.5 * dx, ymin + j * dy + 0.5 * dy, zmin + k * dz + 0.5 * dz,
+                       fabs(dx * dy * dz), s->c0, s->f0);
+}
+
+void orc_linescan_size(const orc_scan* s, int* n_f, int* n_ca) {
+  *n_f = (int)((s->x_max - s->x_min) / s->res_f);
+  *n_ca = (int)((s->x_max - s->x_min) / s->res_ca);
+}
+
+void orc_linescan(const orc_scan* s, int n, const double* centers, const double* V0, const double* ca,
+                  const double* caf, double* out_ca, double* out_caf, double* out_cafb) {
+  orc_cells m = {0};
+  for (int j = 0; j < n; ++j)
+    orc_cells_push(&m, centers[3 * j], centers[3 * j + 1], centers[3 * j + 2], V0[j], ca[j], caf[j]);
+  /* padding, linescan/main.cpp:119-141 */
+  const double pmin = s->pad_min, pmax = s->pad_max;
+  const double xlo = -s->dom[0] / 2, xhi = s->dom[0] / 2, ylo = -s->dom[1] / 2, yhi = s->dom[1] / 2;
+  const double zlo = -s->dom[2] / 2, zhi = s->dom[2] / 2;
+  orc_add_point_grid(&m, s, pmin, pmax, pmin, pmax, zhi, pmax);
+  orc_add_point_grid(&m, s, pmin, pmax, pmin, pmax, zlo, pmin);
+  orc_add_point_grid(&m, s, xlo, pmin, pmin, pmax, zlo, zhi);
+  orc_add_point_grid(&m, s, xhi, pmax, pmin, pmax, zlo, zhi);
+  orc_add_point_grid(&m, s, xlo, xhi, ylo, pmin, zlo, zhi);
+  orc_add_point_grid(&m, s, xlo, xhi, yhi, pmax, zlo, zhi);
+  int n_f, n_ca;
+  orc_linescan_size(s, &n_f, &n_ca);
+  /* axis mapping, linescan/main.cpp:169-194 */
+  double c0[3] = {0, 0, 0}, sx, sy, sz;
+  if (s->axis == 0) {
+    c0[1] = s->off1; c0[2] = s->off2;
+    sx = sqrt(s->var[0]); sy = sqrt(s->var[1]); sz = sqrt(s->var[2]);
+  } else if (s->axis == 1) {
+    c0[0] = s->off1; c0[2] = s->off2;
+    sx = sqrt(s->var[0]); sy = sqrt(s->var[1]); sz = sqrt(s->var[2]);
+  } else {
+    c0[0] = s->off1; c0[1] = s->off2;
+    sx = sqrt(s->var[2]); sy = sqrt(s->var[0]); sz = sqrt(s->var[1]);
+  }
+  const double alpha = 1.0 / pow(2 * ORC_SCAN_PI, 1.5);
+  const double r_thresh = 1;
+  for (int i = 0; i < n_f; ++i) { /* :197-225 */
+    c0[s->axis] = s->x_min + i * s->res_f;
+    double sum = 0;
+    double rx = m.c[0] - c0[0], ry = m.c[1] - c0[1], rz = m.c[2] - c0[2];
+    double r_min = rx * rx + ry * ry + rz * rz;
+    int cell_min = 0;
+    for (int j = 0; j < m.n; ++j) {
+      rx = m.c[3 * j] - c0[0]; ry = m.c[3 * j + 1] - c0[1]; rz = m.c[3 * j + 2] - c0[2];
+      double dist = rx * rx + ry * ry + rz * rz;
+      if (dist < r_thresh) {
+        double w = alpha * exp(-0.5 * ((rx * rx / (sx * sx)) + (ry * ry / (sy * sy)) + (rz * rz / (sz * sz)))) /
+                   (sx * sy * sz);
+        sum += m.v[j] * m.caf[j] * w;
+      }
+      if (dist < r_min) { r_min = dist; cell_min = j; }
+    }
+    out_cafb[i] = sum;
+    out_caf[i] = m.caf[cell_min];
+  }
+  for (int i = 0; i < n_ca; ++i) { /* :227-249 */
+    c0[s->axis] = s->x_min + i * s->res_ca;
+    double rx = m.c[0] - c0[0], ry = m.c[1] - c0[1], rz = m.c[2] - c0[2];
+    double r_min = rx * rx + ry * ry + rz * rz;
+    int cell_min = 0;
+    for (int j = 0; j < m.n; ++j) {
+      rx = m.c[3 * j] - c0[0]; ry = m.c[3 * j + 1] - c0[1]; rz = m.c[3 * j + 2] - c0[2];
+      double dist = rx * rx + ry * ry + rz * rz;
+      if (dist < r_min) { r_min = dist; cell_min = j; }
+    }
+    out_ca[i] = m.ca[cell_min];
+  }
+  free(m.c); free(m.v); free(m.ca); free(m.caf);
+}

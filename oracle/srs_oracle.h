@@ -26,6 +26,9 @@
  *                                       closed form D Face_C b / V0 per face.
  *   orc_states_record                -> a hand-computed record (every field
  *                                       from its definition).
+ *   orc_linescan                     -> closed forms: one cell, the PSF
+ *                                       integral over a uniform padding grid,
+ *                                       brute-force nearest cell.
  *   orc_open_random_ryr              -> selection law: RyR i is picked with
  *                                       frequency rate_i / sum(rates), checked
  *                                       in two limits where the rate is known
@@ -137,6 +140,19 @@ int orc_states_record(double t, const double* X, int n_ele, const double* gv, co
 
 /* cpu/InitFuncs.cpp:1420-1475 with the uniform supplied by the caller. */
 int orc_open_random_ryr(const double* X, int n_ele, const double* reals, orc_chan* ch, double u01);
+
+/* Linescan of one state (linescan/main.cpp:96-249, AddPointGrid :268-306,
+ * normpdf3d linescan/parameters.h:60-62).  axis 0/1/2 = X/Y/Z. */
+typedef struct orc_scan {
+  int axis;
+  double x_min, x_max, res_f, res_ca, off1, off2;
+  double dom[3], pad_min, pad_max, pad_res, var[3], f0, c0;
+} orc_scan;
+/* Number of fluorescence and Ca points. */
+void orc_linescan_size(const orc_scan* s, int* n_f, int* n_ca);
+/* centers [n][3]; outputs ca[n_ca], caf[n_f], cafb[n_f]. */
+void orc_linescan(const orc_scan* s, int n, const double* centers, const double* V0, const double* ca,
+                  const double* caf, double* out_ca, double* out_caf, double* out_cafb);
 
 #ifdef __cplusplus
 }

@@ -130,6 +130,13 @@ class Grid:
     lcc_ele: np.ndarray
     perm: np.ndarray
 
+    def centroids(self) -> np.ndarray:
+        """Element centres [n_ele][3], the mean of the four nodes in the order
+        the linescan program adds them (linescan/main.cpp:96-103)."""
+        X = self.node_xyz.reshape(-1, 3)
+        t = self.ele_nodes.reshape(-1, 4)
+        return (((X[t[:, 0]] + X[t[:, 1]]) + X[t[:, 2]]) + X[t[:, 3]]) / 4.0
+
     @property
     def n_ryr(self):
         return len(self.ryr_ele)
@@ -279,6 +286,18 @@ def run_cru3d(param: str, mesh: str, out_dir: str | None, rank: int = 0, start: 
     return secs.value
 
 
+def read_scan_parameters(path: str | None = None) -> _abi.SrsScanDesc:
+    """The linescan program's ReadParameters (linescan/main.cpp:353-508)."""
+    d = _abi.SrsScanDesc()
+    check(lib().srs_scan_read_parameters(path.encode() if path else None, C.byref(d)))
+    return d
+
+
+def dff0(cafb: np.ndarray, f0: float) -> np.ndarray:
+    """dF/F0 of a blurred fluorescence scan line against the resting signal."""
+    return np.asarray(cafb) / f0 - 1.0
+
+
 class Context:
     """A device context: meshes, realizations, and the step.
 
@@ -292,6 +311,7 @@ class Context:
         self._keep = []
         self.n_real = 0
         self._sizes = []
+        self._scans = {}
 
     def tune(self, **kw):
         """tile=, nt_geometry=, nt_state=, plane_pad= (see srs_ctx_tune)."""
@@ -443,12 +463,28 @@ class Context:
         check(lib().srs_last_timing(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def add_scan(self, mesh_id: int, centers: np.ndarray, desc: _abi.SrsScanDesc) -> tuple[int, int, int]:
+        """A scan line over a mesh (after finalize); returns (scan id, n_f, n_ca)."""
+        c = np.ascontiguousarray(centers, np.float64).reshape(-1)
+        sid, nf, nca = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().srs_ctx_add_scan(self._h, mesh_id, _f64(c), C.byref(desc), C.byref(sid), C.byref(nf),
+                                     C.byref(nca)))
+        self._scans[sid.value] = (nf.value, nca.value)
+        return sid.value, nf.value, nca.value
+
+    def scan_sample(self, r: int, scan_id: int):
+        """(Ca nearest [n_ca], CaF nearest [n_f], CaF blurred [n_f]) of the current state."""
+        nf, nca = self._scans[scan_id]
+        ca, caf, cafb = np.zeros(nca), np.zeros(nf), np.zeros(nf)
+        check(lib().srs_scan_sample(self._h, r, scan_id, _f64(ca), _f64(caf), _f64(cafb)))
+        return ca, caf, cafb
+
     def time_sweep(self, reps: int):
         a, b = C.c_double(), C.c_double()
         check(lib().srs_time_sweep(self._h, reps, C.byref(a), C.byref(b)))
         return a.value, b.value
 
 
-__all__ = ["Params", "Grid", "BoxSpec", "Context", "SrsError", "read_parameters", "grid_read", "box_grid",
+__all__ = ["Params", "Grid", "BoxSpec", "Context", "SrsError", "read_parameters", "read_scan_parameters", "dff0", "grid_read", "box_grid",
            "box_mesh_write", "bfs_order", "assemble_operator", "equilibrium_state", "boundary_rates",
            "states_record", "run_cru3d", "I", "R"]

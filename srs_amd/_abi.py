@@ -59,6 +59,16 @@ class SrsMeshDesc(C.Structure):
     ]
 
 
+class SrsScanDesc(C.Structure):
+    _fields_ = [
+        ("n_scans", C.c_int32), ("n_scans_skip", C.c_int32), ("axis", C.c_int32),
+        ("res_x_f", C.c_double), ("res_x_ca", C.c_double), ("res_t", C.c_double),
+        ("x_min", C.c_double), ("x_max", C.c_double), ("offset1", C.c_double), ("offset2", C.c_double),
+        ("domain", C.c_double * 3), ("pad_min", C.c_double), ("pad_max", C.c_double), ("pad_res", C.c_double),
+        ("var", C.c_double * 3), ("f0", C.c_double), ("c0", C.c_double),
+    ]
+
+
 class SrsChanDesc(C.Structure):
     _fields_ = [
         ("n_ryr", C.c_int32), ("ryr_ele", i32p), ("ryr_jsr", i32p), ("ryr_nb", i32p),
@@ -114,6 +124,9 @@ _PROTOS = {
     "srs_real_info": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, i32p]),
     "srs_last_timing": (C.c_int, [C.c_void_p, f64p, f64p]),
     "srs_time_sweep": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p]),
+    "srs_scan_read_parameters": (C.c_int, [C.c_char_p, C.POINTER(SrsScanDesc)]),
+    "srs_ctx_add_scan": (C.c_int, [C.c_void_p, C.c_int32, f64p, C.POINTER(SrsScanDesc), i32p, i32p, i32p]),
+    "srs_scan_sample": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p, f64p, f64p]),
 }
 
 EXPORTED = tuple(_PROTOS)

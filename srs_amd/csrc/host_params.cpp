@@ -291,3 +291,88 @@ extern "C" int srs_read_parameters(const char* path, int32_t ints[SRS_N_INT], do
   }
   return SRS_OK;
 }
+
+// --------------------------------------------------------- linescan parameters
+namespace {
+
+void scan_defaults(srs_scan_desc* d) {
+  // linescan/main.cpp:356-380
+  memset(d, 0, sizeof(*d));
+  d->n_scans = 26;
+  d->n_scans_skip = 2;
+  d->res_x_f = 0.15;
+  d->res_x_ca = 0.01;
+  d->res_t = 2.0;
+  d->x_min = -4.0;
+  d->x_max = 4.0;
+  d->offset1 = 0.1075;
+  d->offset2 = 0;
+  d->domain[0] = d->domain[1] = d->domain[2] = 4;
+  d->pad_min = -5.0;
+  d->pad_max = 5.0;
+  d->pad_res = 0.3;
+  d->var[0] = 0.0289;
+  d->var[1] = 0.1154;
+  d->var[2] = 0.0289;
+  d->f0 = 4.1667;
+  d->c0 = 0.10;
+  d->axis = 2;
+}
+
+}  // namespace
+
+extern "C" int srs_scan_read_parameters(const char* path, srs_scan_desc* d) {
+  srs::clear_error();
+  if (!d) return srs::fail(SRS_E_ARG, "null output");
+  scan_defaults(d);
+  if (!path) return SRS_OK;
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return srs::fail(SRS_E_IO, "cannot open scan parameter file %s", path);
+  std::stringstream ss;
+  ss << in.rdbuf();
+  const std::string src = ss.str();
+  Parser ps(src);
+  Elem root;
+  if (!ps.skip_misc() || !ps.element(root)) return srs::fail(SRS_E_IO, "%s: XML error: %s", path, ps.err.c_str());
+  if (root.name != "analysis") return srs::fail(SRS_E_IO, "%s: Invalid XML root", path);
+  // only <parameter> children of the root count (linescan/main.cpp:426-447)
+  for (const Elem& pe : root.kids) {
+    if (pe.name != "parameter") continue;
+    const std::string* symbol = nullptr;
+    const std::string* value = nullptr;
+    for (const Elem& k : pe.kids) {
+      if (k.name == "symbol") symbol = &k.text;
+      else if (k.name == "value") value = &k.text;
+    }
+    if (!symbol || !value) continue;
+    const std::string& sy = *symbol;
+    const char* v = value->c_str();
+    // linescan/main.cpp:461-501
+    if (sy == "scan_n") d->n_scans = atoi(v);
+    else if (sy == "scan_n_per_line") d->n_scans_skip = atoi(v);
+    else if (sy == "scan_x_res_f") d->res_x_f = atof(v);
+    else if (sy == "flag_x_res_ca") d->n_scans_skip = atoi(v);  // Integers[I_RES_X_Ca] is Integers[1]
+    else if (sy == "scan_x_min") d->x_min = atof(v);
+    else if (sy == "scan_x_max") d->x_max = atof(v);
+    else if (sy == "scan_offset_1") d->offset1 = atof(v);
+    else if (sy == "scan_offset_2") d->offset2 = atof(v);
+    else if (sy == "scan_domain_x") d->domain[0] = atof(v);
+    else if (sy == "scan_domain_y") d->domain[1] = atof(v);
+    else if (sy == "scan_domain_z") d->domain[2] = atof(v);
+    else if (sy == "scan_padding_min") d->pad_min = atof(v);
+    else if (sy == "scan_padding_max") d->pad_max = atof(v);
+    else if (sy == "scan_padding_res") d->pad_res = atof(v);
+    else if (sy == "scan_var_x") d->var[0] = atof(v);
+    else if (sy == "scan_var_y") d->var[1] = atof(v);
+    else if (sy == "scan_var_z") d->var[2] = atof(v);
+    else if (sy == "scan_f0") d->f0 = atof(v);
+    else if (sy == "scan_c0") d->c0 = atof(v);
+    else if (sy == "scan_axis") {
+      if (strcmp(v, "X") == 0) d->axis = 0;
+      else if (strcmp(v, "Y") == 0) d->axis = 1;
+      else if (strcmp(v, "Z") == 0) d->axis = 2;
+      else return srs::fail(SRS_E_ARG, "invalid scan axis: %s", v);
+    }
+  }
+  return SRS_OK;
+}

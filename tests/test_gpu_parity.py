@@ -229,6 +229,47 @@ def test_stochastic_gating_matches_reference_in_distribution():
     assert ref[:, 3].mean() > 0.5  # the cluster does recruit openings
 
 
+@pytest.mark.parametrize("axis", [0, 2])
+def test_linescan_matches_oracle(axis):
+    """The device linescan (nearest-cell Ca and CaF, PSF-blurred CaF with the
+    padding) on the 56,250-element spark box against the oracle's restatement
+    of linescan/main.cpp.  Nearest values are copies (bit-exact); the blurred
+    sum is summed in another order (rel <= 1e-12)."""
+    pr = W.prepare_spark()
+    g = pr.grid
+    n = g.n_ele
+    cen = g.centroids() - np.array([0.5, 0.5, 0.3])  # the linescan program assumes a centred mesh
+    rng = np.random.default_rng(11)
+    X = pr.X0.copy()
+    X[:n] *= 1 + 50 * rng.random(n)
+    X[5 * n:] *= 1 + 3 * rng.random(n)
+    d = S.read_scan_parameters(None)
+    d.axis = axis
+    d.domain[0], d.domain[1], d.domain[2] = 1.0, 1.0, 0.6
+    d.pad_min, d.pad_max, d.pad_res = -1.5, 1.5, 0.1
+    if axis == 0:
+        d.x_min, d.x_max, d.res_x_f, d.res_x_ca = -0.7, 0.7, 0.025, 0.01
+        d.offset1, d.offset2 = 0.02, -0.2925
+    else:
+        d.x_min, d.x_max, d.res_x_f, d.res_x_ca = -0.45, 0.45, 0.02, 0.005
+        d.offset1, d.offset2 = 0.02, 0.01
+    with S.Context(0) as ctx:
+        mid = ctx.add_mesh_grid(g, pr.aij, pr.nij, perm=W.kuhn_block_order(g))
+        ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, pr.params, seed=1, stream=0)
+        ctx.finalize()
+        ctx.state_set(0, X)
+        sid, nf, nca = ctx.add_scan(mid, cen, d)
+        ca, caf, cafb = ctx.scan_sample(0, sid)
+    oca, ocaf, ocafb = Oracle().linescan(d, cen, g.V0, X[:n], X[5 * n:])
+    assert nf == len(ocafb) > 40 and nca == len(oca) > 100
+    np.testing.assert_array_equal(ca, oca)
+    np.testing.assert_array_equal(caf, ocaf)
+    assert rel_err(cafb, ocafb) <= 1e-12
+    assert np.any(caf != d.f0) and np.any(ca != d.c0)
+    dff = S.dff0(cafb, d.f0)
+    assert np.all(np.isfinite(dff)) and dff.max() > 0
+
+
 def _write_params(path, entries):
     body = "".join(f"  <parameter><symbol>{k}</symbol><value>{v}</value></parameter>\n" for k, v in entries.items())
     with open(path, "w") as f:
