@@ -17,7 +17,8 @@ Printed as one JSON line:
                (oracle/_ref) on the same mesh, 1 host core, bounded sample
   batched      the ensemble workload: many independent spark realizations on
                h = 40/30/25 nm meshes in one launch per step, against the
-               plain path run one realization at a time on the h = 40 nm mesh
+               plain path run one realization at a time over the same mix of
+               meshes
 """
 from __future__ import annotations
 
@@ -147,18 +148,23 @@ def bench_batched(n_real, steps, warmup, order, tune):
     wall = time.perf_counter() - t0
     sweep_ms, gate_ms = ctx.time_sweep(5)
     ctx.close()
-    # plain path, one realization at a time, on the h = 40 nm member mesh
-    pr = W.prepare_spark()
-    pctx = setup_plain(pr, order, tune)
-    pctx.explicit_step(200, S.STEP_GRAPH)
-    pctx.sync()
-    t0 = time.perf_counter()
-    pctx.explicit_step(2000, S.STEP_GRAPH)
-    pctx.sync()
-    pwall = time.perf_counter() - t0
-    pctx.close()
-    return dict(wall=wall, sim_ms=float(np.sum(dts)) * steps, sweep_ms=sweep_ms, gate_ms=gate_ms, n_ele=n_ele, nbytes=nbytes,
-                plain_rate=2000 * pr.params.reals[1] / pwall, plain_dt=float(pr.params.reals[1]))
+    # The plain path, one realization at a time: one member on each of the
+    # three meshes, then the batch's mix of meshes run back to back.
+    members = W.ensemble_members(n_real)
+    rate = {}
+    for m in range(3):
+        r = next(k for k, mem in enumerate(members) if mem["mesh"] == m)
+        pctx, pdts, _, _ = W.ensemble_context(n_real, order=order, tune=tune, only=[r])
+        pctx.explicit_step(200, S.STEP_GRAPH)
+        pctx.sync()
+        t0 = time.perf_counter()
+        pctx.explicit_step(2000, S.STEP_GRAPH)
+        pctx.sync()
+        rate[m] = 2000 * float(pdts[0]) / (time.perf_counter() - t0)
+        pctx.close()
+    seq = n_real / sum(1.0 / rate[mem["mesh"]] for mem in members)
+    return dict(wall=wall, sim_ms=float(np.sum(dts)) * steps, sweep_ms=sweep_ms, gate_ms=gate_ms, n_ele=n_ele,
+                nbytes=nbytes, plain_rate=seq, plain_per_mesh=[rate[m] for m in range(3)])
 
 
 def main():
@@ -174,6 +180,8 @@ def main():
     ap.add_argument("--tune", default="auto",
                     help="'auto' (autotune the sweep on scratch contexts), 'default', or a JSON object of "
                          "tile/plane_pad/nt_geometry/nt_state")
+    ap.add_argument("--workload", choices=("dyad", "spark"), default="dyad",
+                    help="plain-path workload: the 2.95M-element dyad box (default) or the 56,250-element spark box")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device copy of 2 GiB (HBM byte-counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -195,9 +203,9 @@ def main():
         log("calibration copy: 2 GiB read + 2 GiB written")
         del a, b
     t0 = time.perf_counter()
-    pr = W.prepare_dyad()
-    log(f"dyad box: {pr.grid.n_ele} elements, {pr.grid.n_ryr} RyR, {pr.grid.n_lcc} LCC, dt {pr.params.reals[1]:.4e} ms "
-        f"(setup {time.perf_counter() - t0:.1f} s)")
+    pr = W.prepare_dyad() if args.workload == "dyad" else W.prepare_spark()
+    log(f"{args.workload} box: {pr.grid.n_ele} elements, {pr.grid.n_ryr} RyR, {pr.grid.n_lcc} LCC, "
+        f"dt {pr.params.reals[1]:.4e} ms (setup {time.perf_counter() - t0:.1f} s)")
     dt = float(pr.params.reals[1])
     N = pr.grid.n_ele
     trials = None
@@ -216,13 +224,16 @@ def main():
     value = args.steps * dt / wall
     nbytes = algorithmic_bytes(N)
     achieved = nbytes / (sweep_ms * 1e-3) / 1e9
-    traffic = profiled_traffic(os.path.join(ROOT, "profiles", "k_sweep_dyad_traffic.md"))
+    traffic = profiled_traffic(os.path.join(ROOT, "profiles", f"k_sweep_{args.workload}_traffic.md"))
     out = {
         "metric": METRIC, "value": value, "unit": "spark-ms simulated/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"dyad_box 2x2x1 um, T-tubule r=100 nm, jSR 1.0x1.0x0.04 um, h=20 nm Kuhn tets "
-                               f"({N} elements), {pr.grid.n_ryr} RyR + {pr.grid.n_lcc} LCC, 1 realization, FP64",
+        "config": {"workload": (f"dyad_box 2x2x1 um, T-tubule r=100 nm, jSR 1.0x1.0x0.04 um, h=20 nm Kuhn tets "
+                                f"({N} elements), {pr.grid.n_ryr} RyR + {pr.grid.n_lcc} LCC, 1 realization, FP64"
+                                if args.workload == "dyad" else
+                                f"spark_box 1.0x1.0x0.6 um, jSR 0.4x0.4x0.04 um, h=40 nm Kuhn tets ({N} elements), "
+                                f"{pr.grid.n_ryr} RyR + {pr.grid.n_lcc} LCC, 1 realization, FP64"),
                    "n_elements": N, "dt_ms": dt, "element_order": args.order, "tuning": tune,
                    "parallelism": "single realization, one GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -247,6 +258,10 @@ def main():
             "steps": args.batch_steps, "ms_per_step": 1e3 * b["wall"] / args.batch_steps,
             "sweep_ms": b["sweep_ms"], "sweep_gbs": bach, "sweep_frac": bach / HBM_PEAK_GBS,
             "plain_sequential_value": b["plain_rate"], "speedup_vs_plain_sequential": brate / b["plain_rate"],
+            "plain_per_mesh": {"h40": b["plain_per_mesh"][0], "h30": b["plain_per_mesh"][1],
+                               "h25": b["plain_per_mesh"][2]},
+            "plain_sequential": "the plain path, one realization per context, over the batch's mix of meshes "
+                                "(spark-ms/s = n / sum of 1/rate of each member's mesh)",
         }
         log(f"batched: {brate:.3f} spark-ms/s vs plain {b['plain_rate']:.4f} ({brate / b['plain_rate']:.1f}x)")
     if trials:

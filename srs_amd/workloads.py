@@ -241,16 +241,19 @@ def initial_channels(pr: Prepared, open_ryr=None, seed=0):
 
 
 def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0, t_final: float = 50.0,
-                     order: str = "block", tune: dict | None = None):
+                     order: str = "block", tune: dict | None = None, only=None):
     """The batched Monte Carlo workload in one context: returns (ctx, dt per
     realization, total elements over realizations, algorithmic sweep bytes
     per step).  Algorithmic bytes: 96 B per element of every realization (6
     FP64 species read and written) plus 65 B per element of each distinct
     mesh (the 4 int32 + 5 FP64 operator row and 9 B of per-element
-    constants), counted once per step however many realizations share it."""
+    constants), counted once per step however many realizations share it.
+    only: indices of the members to include (default all n_real), e.g. one
+    member at a time for the plain sequential comparison."""
     from . import ORDER_AS_GIVEN, ORDER_RCM, Context
 
     members = ensemble_members(n_real, seed)
+    picked = list(range(n_real)) if only is None else list(only)
     ctx = Context(0)
     tune = dict(tune or {})
     ctx.tune(**{k: tune[k] for k in ("tile", "plane_pad", "compact") if k in tune})
@@ -266,9 +269,10 @@ def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0
         meshes.append((g, aij, nij, mid, stable_dt(aij, 0.22)))
     dts, n_ele, nbytes = [], 0, 0
     plist = []
-    for g, *_ in meshes:
-        nbytes += 65 * g.n_ele
-    for r, mem in enumerate(members):
+    for m in sorted({members[r]["mesh"] for r in picked}):
+        nbytes += 65 * meshes[m][0].n_ele
+    for r in picked:
+        mem = members[r]
         g, aij, nij, mid, dt = meshes[mem["mesh"]]
         idx, nb = lattice_subset(mem["mask"])
         p = spark_params(t_final=t_final)
@@ -280,15 +284,16 @@ def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0
         nbytes += 96 * g.n_ele
         plist.append((g, p, len(idx)))
     ctx.finalize(record_capacity)
-    for r, (g, p, nr) in enumerate(plist):
+    for k, (g, p, nr) in enumerate(plist):
+        r = picked[k]
         X0, _ = equilibrium_state(g, p.reals)
-        ctx.state_set(r, X0)
+        ctx.state_set(k, X0)
         rs = np.zeros(nr, np.int32)
         rs[nr // 2] = 1
         rng = np.random.default_rng(seed + r)
         R = np.log(rng.random(nr + g.n_lcc) * (1 - 2**-53) + 2**-54)
-        ctx.channels_set(r, rs, np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R)
-        ctx.globals_set(r, 0.0, p.reals[16])
+        ctx.channels_set(k, rs, np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R)
+        ctx.globals_set(k, 0.0, p.reals[16])
     ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state", "occupancy") if k in tune})
     return ctx, np.array(dts), n_ele, nbytes
 
