@@ -79,7 +79,7 @@ def setup_plain(pr, order="block", tune=None):
 
     ctx = S.Context(0)
     tune = dict(tune or {})
-    ctx.tune(**{k: tune[k] for k in ("tile", "plane_pad", "compact") if k in tune})
+    ctx.tune(**{k: tune[k] for k in ("tile", "plane_pad", "compact", "group") if k in tune})
     if order == "block":
         mid = ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, perm=kuhn_block_order(pr.grid))
     else:
@@ -91,7 +91,7 @@ def setup_plain(pr, order="block", tune=None):
     ctx.state_set(0, pr.X0)
     ctx.channels_set(0, rs, ls, lvs, R)
     ctx.globals_set(0, 0.0, pr.params.reals[16])
-    ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state", "occupancy") if k in tune})
+    ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state", "occupancy", "lds") if k in tune})
     return ctx
 
 
@@ -139,7 +139,8 @@ def bench_batched(n_real, steps, warmup, order, tune):
     import srs_amd as S
     from srs_amd import workloads as W
 
-    ctx, dts, n_ele, nbytes = W.ensemble_context(n_real, order=order, tune=tune)
+    ctx, dts, n_ele, nbytes = W.ensemble_context(n_real, order=order, tune=tune, log=log)
+    log(f"batched: {n_real} realizations, {n_ele} elements, group {tune.get('group', 1)}")
     ctx.explicit_step(warmup, S.STEP_GRAPH)
     ctx.sync()
     t0 = time.perf_counter()
@@ -154,7 +155,7 @@ def bench_batched(n_real, steps, warmup, order, tune):
     rate = {}
     for m in range(3):
         r = next(k for k, mem in enumerate(members) if mem["mesh"] == m)
-        pctx, pdts, _, _ = W.ensemble_context(n_real, order=order, tune=tune, only=[r])
+        pctx, pdts, _, _ = W.ensemble_context(n_real, order=order, tune={**tune, "group": 1}, only=[r])
         pctx.explicit_step(200, S.STEP_GRAPH)
         pctx.sync()
         t0 = time.perf_counter()
@@ -174,6 +175,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--batch", type=int, default=2048, help="realizations in the batched leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-group", type=int, default=4, choices=(1, 2, 4),
+                    help="realizations per realization group in the batched leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work (0: skip)")
     ap.add_argument("--order", choices=("block", "rcm", "given"), default="block",
                     help="device element order: Kuhn cube blocks, reverse Cuthill-McKee, or the reference BFS order")
@@ -248,14 +251,14 @@ def main():
                                "sample": f"{n} ExplicitStep calls with gating on the same {N}-element mesh"}
         log(f"cpu {kind}: {n} steps in {cwall:.2f} s")
     if args.batch > 0:
-        b = bench_batched(args.batch, args.batch_steps, 2, args.order, tune)
+        b = bench_batched(args.batch, args.batch_steps, 2, args.order, {**tune, "group": args.batch_group})
         brate = b["sim_ms"] / b["wall"]
         bach = b["nbytes"] / (b["sweep_ms"] * 1e-3) / 1e9
         out["batched"] = {
             "workload": f"{args.batch} realizations on h=40/30/25 nm spark boxes (56,250/130,680/230,400 elements), "
                         "9-100 RyR each, SR load 0.5-1.25 mM, FP64",
             "value": brate, "unit": "spark-ms simulated/s", "sparks_per_s": brate / 50.0,
-            "steps": args.batch_steps, "ms_per_step": 1e3 * b["wall"] / args.batch_steps,
+            "steps": args.batch_steps, "ms_per_step": 1e3 * b["wall"] / args.batch_steps, "group": args.batch_group,
             "sweep_ms": b["sweep_ms"], "sweep_gbs": bach, "sweep_frac": bach / HBM_PEAK_GBS,
             "plain_sequential_value": b["plain_rate"], "speedup_vs_plain_sequential": brate / b["plain_rate"],
             "plain_per_mesh": {"h40": b["plain_per_mesh"][0], "h30": b["plain_per_mesh"][1],

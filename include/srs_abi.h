@@ -255,6 +255,11 @@ int srs_ctx_create(int device, srs_ctx** out);
 #define SRS_TUNE_COMPACT 5       /* 1 (default): store operator rows as class + int16 ulp deltas and
                                     neighbours as int16 offsets where that reproduces them exactly */
 #define SRS_TUNE_OCCUPANCY 6     /* 8: sweep built for eight wavefronts per SIMD (<= 64 VGPRs); 0: default */
+#define SRS_TUNE_GROUP 7         /* 1 (default), 2 or 4: realizations of one mesh whose states are interleaved
+                                    per element so that each sweep tile loads the operator rows once for all
+                                    of them (set before srs_ctx_finalize) */
+#define SRS_TUNE_LDS 8           /* 1: stage each sweep tile's diffusing species in LDS and read neighbours
+                                    inside the tile from there (groups of 1) */
 int srs_ctx_tune(srs_ctx* ctx, int32_t key, int32_t value);
 
 /* Replaces FreeDomainStruct / FreeGlobalStruct (gpu/RunSimulation.cu:874-901). */

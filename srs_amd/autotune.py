@@ -16,22 +16,26 @@ from typing import Callable
 from . import Context
 
 LAYOUT_KEYS = ("tile", "plane_pad", "compact")
-KERNEL_KEYS = ("nt_geometry", "nt_state", "occupancy")
+KERNEL_KEYS = ("nt_geometry", "nt_state", "occupancy", "lds")
 
 
 def autotune(make: Callable[[dict], Context], tiles=(128, 256, 512), pads=(0,), compact=(1, 0), nt_geometry=(0, 1),
-             nt_state=(0, 1), occupancy=(0, 8), reps: int = 20, warm: int = 3, log=None) -> tuple[dict, list]:
+             nt_state=(0, 1), occupancy=(0, 8), lds=(0, 1), reps: int = 20, warm: int = 3,
+             log=None) -> tuple[dict, list]:
     """Returns (best configuration, all trials as (config, sweep_ms))."""
     trials = []
     for tile, pad, cmp in itertools.product(tiles, pads, compact):
         ctx = make({"tile": tile, "plane_pad": pad, "compact": cmp})
         try:
-            for g, s, o in itertools.product(nt_geometry, nt_state, occupancy):
-                ctx.tune(nt_geometry=g, nt_state=s, occupancy=o)
+            for g, s, o, ld in itertools.product(nt_geometry, nt_state, occupancy, lds):
+                if ld and o:
+                    continue  # the LDS-staged sweep has one register budget
+                ctx.tune(nt_geometry=g, nt_state=s, occupancy=o, lds=ld)
                 ctx.explicit_step(warm)
                 t0 = time.perf_counter()
                 ms, _ = ctx.time_sweep(reps)
-                cfg = {"tile": tile, "plane_pad": pad, "compact": cmp, "nt_geometry": g, "nt_state": s, "occupancy": o}
+                cfg = {"tile": tile, "plane_pad": pad, "compact": cmp, "nt_geometry": g, "nt_state": s, "occupancy": o,
+                       "lds": ld}
                 trials.append((cfg, ms))
                 if log:
                     log(f"autotune {cfg}: sweep {ms * 1e3:.2f} us ({time.perf_counter() - t0:.2f} s)")

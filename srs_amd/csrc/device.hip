@@ -49,8 +49,17 @@ enum : uint8_t { F_JSR = 1, F_TROPC = 2, F_SL = 4, F_BR = 8, F_CHAN = 16 };
 // neighbour gather fetches all four with one contiguous access; then the TnC
 // and CSQ planes, which are never gathered.
 constexpr int MOB_SLOT[6] = {0, 1, 2, -1, -1, 3};  // species -> slot in the interleaved quad
-__host__ __device__ inline size_t xoff(int s, size_t e, size_t np) {
-  return s == srs::TRPN ? 4 * np + e : s == srs::CSQN ? 5 * np + e : 4 * e + MOB_SLOT[s];
+// A realization group holds K realizations of one mesh in blocks of 64
+// elements: block b holds the 64 quads of slot 0, then slot 1, ..., so a
+// wavefront's loads of one realization stay contiguous.  Element e of slot
+// `slot` is entry ((e/64)*K + slot)*64 + e%64 of the quad array, and of the
+// TnC and CSQ planes (K*np entries each) that follow the 4*K*np quad doubles.
+__host__ __device__ inline size_t gidx(size_t e, int K, int slot) {
+  return ((e >> 6) * (size_t)K + (size_t)slot) * 64 + (e & 63);
+}
+__host__ __device__ inline size_t xoff(int s, size_t e, size_t np, int K = 1, int slot = 0) {
+  const size_t q = gidx(e, K, slot);
+  return s == srs::TRPN ? 4 * (size_t)K * np + q : s == srs::CSQN ? 5 * (size_t)K * np + q : 4 * q + MOB_SLOT[s];
 }
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -70,6 +79,7 @@ struct RealDesc {
   const double* br;       // [4][np] Ca, Fluo, ATP, CaM boundary rates, or null
   double* X[2];           // [6][np] ping-pong state
   int32_t n, np;
+  int32_t kst, slot;      // realization group: K realizations interleaved, this one in slot `slot`
   int32_t n_ryr, n_lcc;
   const int32_t* ryr_ele;
   const int32_t* ryr_jsr;
@@ -91,6 +101,10 @@ struct RealDesc {
   double P[SRS_N_REAL];
   int32_t I[SRS_N_INT];
 };
+
+// Offset of element e's Ca in a realization's state buffer (its quad's first
+// double); Fluo is at +3.
+__device__ __forceinline__ size_t cq(const RealDesc& d, int e) { return 4 * gidx((size_t)e, d.kst, d.slot); }
 
 // Mutable per-realization loop state.
 struct RealLive {
@@ -139,57 +153,126 @@ __device__ __forceinline__ void stg(SRS_GLOBAL T* p, T v) {
   else *p = v;
 }
 
-// One element of the fused step.  FK >= 0: the caller knows the element's
-// flags (a wavefront whose lanes are all alike), so none are loaded.
-template <bool NTG, bool NTS, bool COMPACT, int FK>
-__device__ __forceinline__ void sweep_one(const RealDesc& d, const RealLive& l, const int j) {
-  const int np = d.np;
-  const int cur = l.cur;
-  const SRS_GLOBAL double* __restrict__ Xo = gp(d.X[cur]);
-  SRS_GLOBAL double* __restrict__ Xn = gpw(d.X[cur ^ 1]);
-  const double* P = d.P;
-  const double dt = P[SRS_R_T_STEP];
-  const uint8_t f = FK >= 0 ? (uint8_t)FK : ldg<NTG>(gp(d.flags) + j);
-
+// Operator row of one element: coefficients (diagonal first), neighbour
+// indices and flags.  Mesh-level data, shared by every realization of a
+// realization group.
+struct Geo {
   double a0, a1, a2, a3, a4;
   int n1, n2, n3, n4;
+  uint8_t f;
+};
+
+// FK >= 0: the caller knows the element's flags (a wavefront whose lanes are
+// all alike), so none are loaded.
+template <bool NTG, bool COMPACT, int FK>
+__device__ __forceinline__ Geo load_geo(const RealDesc& d, const int j) {
+  Geo g;
+  const int np = d.np;
+  g.f = FK >= 0 ? (uint8_t)FK : ldg<NTG>(gp(d.flags) + j);
   if constexpr (COMPACT) {
     // Operator row = class row + small integer multiples of its ulps, which
     // reproduces the assembled FP64 values exactly (every row was checked
     // when the mesh was added).
     const SRS_GLOBAL int16_t* off = gp(d.off);
-    n1 = j + ldg<NTG>(off + j);
-    n2 = j + ldg<NTG>(off + np + j);
-    n3 = j + ldg<NTG>(off + 2 * np + j);
-    n4 = j + ldg<NTG>(off + 3 * np + j);
+    g.n1 = j + ldg<NTG>(off + j);
+    g.n2 = j + ldg<NTG>(off + np + j);
+    g.n3 = j + ldg<NTG>(off + 2 * np + j);
+    g.n4 = j + ldg<NTG>(off + 3 * np + j);
     const SRS_GLOBAL double* t = gp(d.tab) + 10 * (int)ldg<NTG>(gp(d.cls) + j);
     const SRS_GLOBAL int16_t* dq = gp(d.dq);
-    a0 = t[0] + (double)ldg<NTG>(dq + j) * t[5];
-    a1 = t[1] + (double)ldg<NTG>(dq + np + j) * t[6];
-    a2 = t[2] + (double)ldg<NTG>(dq + 2 * np + j) * t[7];
-    a3 = t[3] + (double)ldg<NTG>(dq + 3 * np + j) * t[8];
-    a4 = t[4] + (double)ldg<NTG>(dq + 4 * np + j) * t[9];
+    g.a0 = t[0] + (double)ldg<NTG>(dq + j) * t[5];
+    g.a1 = t[1] + (double)ldg<NTG>(dq + np + j) * t[6];
+    g.a2 = t[2] + (double)ldg<NTG>(dq + 2 * np + j) * t[7];
+    g.a3 = t[3] + (double)ldg<NTG>(dq + 3 * np + j) * t[8];
+    g.a4 = t[4] + (double)ldg<NTG>(dq + 4 * np + j) * t[9];
   } else {
     const SRS_GLOBAL double* cf = gp(d.coef);
     const SRS_GLOBAL int32_t* nb = gp(d.nbr);
-    a0 = ldg<NTG>(cf + j); a1 = ldg<NTG>(cf + np + j); a2 = ldg<NTG>(cf + 2 * np + j);
-    a3 = ldg<NTG>(cf + 3 * np + j); a4 = ldg<NTG>(cf + 4 * np + j);
-    n1 = ldg<NTG>(nb + j); n2 = ldg<NTG>(nb + np + j); n3 = ldg<NTG>(nb + 2 * np + j);
-    n4 = ldg<NTG>(nb + 3 * np + j);
+    g.a0 = ldg<NTG>(cf + j); g.a1 = ldg<NTG>(cf + np + j); g.a2 = ldg<NTG>(cf + 2 * np + j);
+    g.a3 = ldg<NTG>(cf + 3 * np + j); g.a4 = ldg<NTG>(cf + 4 * np + j);
+    g.n1 = ldg<NTG>(nb + j); g.n2 = ldg<NTG>(nb + np + j); g.n3 = ldg<NTG>(nb + 2 * np + j);
+    g.n4 = ldg<NTG>(nb + 3 * np + j);
   }
+  return g;
+}
 
+// The state of one element of one realization: its own and its neighbours'
+// diffusing quads, TnC and CSQ where they are live.
+struct Loc {
+  d4 m0, m1, m2, m3, m4;
+  double c_tr, c_cs;
+};
+
+template <int FK>
+__device__ __forceinline__ Loc load_loc(const RealDesc& d, const RealLive& l, const int j, const Geo& g) {
+  // Realization-group layout: element e of the realization in slot s of a
+  // group of K is quad gidx(e, K, s); the TnC and CSQ planes follow the quads.
+  const int K = d.kst, s = d.slot;
+  const size_t np = d.np;
+  const SRS_GLOBAL double* __restrict__ Xo = gp(d.X[l.cur]);
   const SRS_GLOBAL d4* Mo = (const SRS_GLOBAL d4*)Xo;
-  const d4 m0 = Mo[j];
-  const double c_ca = m0.x, c_at = m0.y, c_cm = m0.z, c_dy = m0.w;
+  Loc c;
+  c.m0 = Mo[gidx(j, K, s)];
   // TnC has a nonzero rate only in cytosol elements with troponin, CSQ only
   // in the jSR (cpu/RunSimulation.cpp:45-46 reset both rates to zero).  A
   // zero rate leaves the value unchanged, and both state buffers hold the same
   // value, so those planes are neither read nor written elsewhere.
+  const uint8_t f = FK >= 0 ? (uint8_t)FK : g.f;
   const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
   const bool live_cs = (f & F_JSR) != 0;
-  const double c_tr = live_tr ? Xo[4 * np + j] : 0.0;
-  const double c_cs = live_cs ? Xo[5 * np + j] : 0.0;
-  const d4 m1 = Mo[n1], m2 = Mo[n2], m3 = Mo[n3], m4 = Mo[n4];
+  c.c_tr = live_tr ? Xo[4 * K * np + gidx(j, K, s)] : 0.0;
+  c.c_cs = live_cs ? Xo[5 * K * np + gidx(j, K, s)] : 0.0;
+  c.m1 = Mo[gidx(g.n1, K, s)];
+  c.m2 = Mo[gidx(g.n2, K, s)];
+  c.m3 = Mo[gidx(g.n3, K, s)];
+  c.m4 = Mo[gidx(g.n4, K, s)];
+  return c;
+}
+
+// As load_loc for a single-realization layout, with the neighbour quads that
+// lie in the workgroup's own tile read from the copy staged in LDS.
+template <int FK, int BS>
+__device__ __forceinline__ Loc load_loc_lds(const RealDesc& d, const RealLive& l, const int j, const Geo& g,
+                                            const d4* tq, const int base, const d4& m0) {
+  const size_t np = d.np;
+  const SRS_GLOBAL double* __restrict__ Xo = gp(d.X[l.cur]);
+  const SRS_GLOBAL d4* Mo = (const SRS_GLOBAL d4*)Xo;
+  Loc c;
+  c.m0 = m0;
+  const uint8_t f = FK >= 0 ? (uint8_t)FK : g.f;
+  const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
+  const bool live_cs = (f & F_JSR) != 0;
+  c.c_tr = live_tr ? Xo[4 * np + j] : 0.0;
+  c.c_cs = live_cs ? Xo[5 * np + j] : 0.0;
+  auto nq = [&](int nb) -> d4 {
+    const unsigned k = (unsigned)(nb - base);
+    return k < (unsigned)BS ? tq[k] : Mo[nb];
+  };
+  c.m1 = nq(g.n1);
+  c.m2 = nq(g.n2);
+  c.m3 = nq(g.n3);
+  c.m4 = nq(g.n4);
+  return c;
+}
+
+// The fused step of one element of one realization, from its loaded row and
+// state, in the reference's operation order (cpu/RunSimulation.cpp:591-648).
+template <bool NTS, int FK>
+__device__ __forceinline__ void step_elem(const RealDesc& d, const RealLive& l, const int j, const Geo& g,
+                                          const Loc& c) {
+  const int K = d.kst, s = d.slot;
+  const size_t np = d.np;
+  const SRS_GLOBAL double* __restrict__ Xo = gp(d.X[l.cur]);
+  SRS_GLOBAL double* __restrict__ Xn = gpw(d.X[l.cur ^ 1]);
+  const double* P = d.P;
+  const double dt = P[SRS_R_T_STEP];
+  const uint8_t f = FK >= 0 ? (uint8_t)FK : g.f;
+  const double a0 = g.a0, a1 = g.a1, a2 = g.a2, a3 = g.a3, a4 = g.a4;
+  const double c_ca = c.m0.x, c_at = c.m0.y, c_cm = c.m0.z, c_dy = c.m0.w;
+  const double c_tr = c.c_tr, c_cs = c.c_cs;
+  const d4 m1 = c.m1, m2 = c.m2, m3 = c.m3, m4 = c.m4;
+  const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
+  const bool live_cs = (f & F_JSR) != 0;
 
   double br_ca = 0.0, br_dy = 0.0, br_at = 0.0, br_cm = 0.0;
   if (f & F_BR) {
@@ -244,15 +327,16 @@ __device__ __forceinline__ void sweep_one(const RealDesc& d, const RealLive& l, 
     if (lo < d.n_celem && d.celem[lo] == j) {
       const double v0 = d.V0[j];
       for (int e = d.cstart[lo]; e < d.cstart[lo + 1]; ++e) {
-        const int c = d.centry[e];
-        if (c >= 0) {
-          const int i = c >> 1;
+        const int ch = d.centry[e];
+        if (ch >= 0) {
+          const int i = ch >> 1;
           if (d.ryr_state[i]) {
-            const double J = P[SRS_R_V_RYR] * (Xo[4 * d.ryr_jsr[i]] - Xo[4 * d.ryr_ele[i]]) * P[SRS_R_V_CELL];
-            if (c & 1) rca -= J / v0; else rca += J / v0;
+            const double J = P[SRS_R_V_RYR] * (Xo[4 * gidx(d.ryr_jsr[i], K, s)] -
+                                               Xo[4 * gidx(d.ryr_ele[i], K, s)]) * P[SRS_R_V_CELL];
+            if (ch & 1) rca -= J / v0; else rca += J / v0;
           }
         } else {
-          const int i = -c - 1;
+          const int i = -ch - 1;
           if (srs_lcc_open(d.lcc_state[i], d.lccv_state[i])) rca += srs_lcc_flux_raw(l.vm, c_ca, P[SRS_R_CA_0]) / v0;
         }
       }
@@ -270,9 +354,10 @@ __device__ __forceinline__ void sweep_one(const RealDesc& d, const RealLive& l, 
 
   // :640-644 forward Euler
   const d4 mn = {c_ca + rca * dt, c_at + rat * dt, c_cm + rcm * dt, c_dy + rdy * dt};
-  stg<NTS>((SRS_GLOBAL d4*)Xn + j, mn);
-  if (live_tr) stg<NTS>(Xn + 4 * np + j, c_tr + rtr * dt);
-  if (live_cs) stg<NTS>(Xn + 5 * np + j, c_cs + rcs * dt);
+  const size_t q = gidx(j, K, s);
+  stg<NTS>((SRS_GLOBAL d4*)Xn + q, mn);
+  if (live_tr) stg<NTS>(Xn + 4 * K * np + q, c_tr + rtr * dt);
+  if (live_cs) stg<NTS>(Xn + 5 * K * np + q, c_cs + rcs * dt);
 }
 
 // Flags shared by every lane of a wavefront-sized element group: the low byte
@@ -280,34 +365,108 @@ __device__ __forceinline__ void sweep_one(const RealDesc& d, const RealLive& l, 
 // cytosol elements with troponin takes the path with the flags known.
 constexpr uint8_t F_SPECIAL = F_JSR | F_SL | F_BR | F_CHAN;
 
+// One tile of one realization group: the operator rows are loaded once and
+// applied to the KG realizations of the group, one after another.  Realizations that are done or in the ffwd phase
+// are computed too, into the buffer they do not read next: their state
+// pointer does not flip, so the result is never used.
+template <bool NTS, int KG, int FK>
+__device__ __forceinline__ void sweep_group(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                                            const int* mem, const int j, const Geo& g) {
+  if constexpr (KG == 1) {
+    const Loc c = load_loc<FK>(D[mem[0]], L[mem[0]], j, g);
+    step_elem<NTS, FK>(D[mem[0]], L[mem[0]], j, g, c);
+  } else {
+    // members one after another: the register budget of a single realization
+    for (int s = 0; s < KG; ++s)
+      if (mem[s] >= 0) {
+        const Loc c = load_loc<FK>(D[mem[s]], L[mem[s]], j, g);
+        step_elem<NTS, FK>(D[mem[s]], L[mem[s]], j, g, c);
+      }
+  }
+}
+
+// The element's flags are loaded with its operator row; a wavefront whose
+// active lanes are all plain cytosol with troponin (the bulk of a mesh) then
+// runs the specialisation with the flags known at compile time.
+// Single-realization groups with LDS staging: the tile's own quads are
+// loaded coalesced into LDS, and neighbour quads inside the tile come from
+// there instead of through the L1 cache.
 template <int BS, bool NTG, bool NTS, bool COMPACT>
+__device__ __forceinline__ void sweep_tile_lds(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int r,
+                                               int t) {
+  __shared__ d4 tq[BS];
+  const RealDesc& d = D[r];
+  const RealLive& l = L[r];
+  const int base = t * BS;
+  const int j = base + threadIdx.x;
+  const bool in = j < d.n;
+  Geo g{};
+  d4 m0 = {0.0, 0.0, 0.0, 0.0};
+  if (in) {
+    g = load_geo<NTG, COMPACT, -1>(d, j);
+    m0 = ((const SRS_GLOBAL d4*)gp(d.X[l.cur]))[j];
+  }
+  tq[threadIdx.x] = m0;
+  __syncthreads();
+  if (!in) return;
+  const bool plain = (g.f & F_SPECIAL) == 0 && (g.f & F_TROPC);
+  if (__builtin_amdgcn_ballot_w64(!plain) == 0) {
+    const Loc c = load_loc_lds<F_TROPC, BS>(d, l, j, g, tq, base, m0);
+    step_elem<NTS, F_TROPC>(d, l, j, g, c);
+  } else {
+    const Loc c = load_loc_lds<-1, BS>(d, l, j, g, tq, base, m0);
+    step_elem<NTS, -1>(d, l, j, g, c);
+  }
+}
+
+template <int BS, bool NTG, bool NTS, bool COMPACT, int KG>
 __device__ __forceinline__ void sweep_tile(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
-                                           const int2* __restrict__ tiles, int n_tiles) {
+                                           const int2* __restrict__ tiles, int n_tiles, const int* __restrict__ groups,
+                                           int single) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   if (b >= n_tiles) return;
-  const int2 tl = tiles[b];
-  const RealDesc& d = D[tl.x];
-  const RealLive& l = L[tl.x];
-  if (l.done | l.ffwd) return;
+  const int2 tl = single ? make_int2(0, b) : tiles[b];  // {group, tile}
+  int mem[KG];
+  bool any = false;
+#pragma unroll
+  for (int s = 0; s < KG; ++s) {
+    mem[s] = single ? (s == 0 ? 0 : -1) : groups[KG * tl.x + s];
+    if (mem[s] >= 0) any |= !(L[mem[s]].done | L[mem[s]].ffwd);
+  }
+  if (!any) return;
+  const RealDesc& d = D[mem[0]];
   const int j = tl.y * BS + threadIdx.x;
   if (j >= d.n) return;
-  const int wave = __builtin_amdgcn_readfirstlane(j >> 6);
-  const uint16_t ws = d.wsum[wave];
-  if ((ws & F_SPECIAL) == 0 && ((ws >> 8) & F_TROPC)) sweep_one<NTG, NTS, COMPACT, F_TROPC>(d, l, j);
-  else sweep_one<NTG, NTS, COMPACT, -1>(d, l, j);
+  const Geo g = load_geo<NTG, COMPACT, -1>(d, j);
+  const bool plain = (g.f & F_SPECIAL) == 0 && (g.f & F_TROPC);
+  if (__builtin_amdgcn_ballot_w64(!plain) == 0) sweep_group<NTS, KG, F_TROPC>(D, L, mem, j, g);
+  else sweep_group<NTS, KG, -1>(D, L, mem, j, g);
 }
 
 // Two register budgets: the compiler's choice, or at most 64 VGPRs so that
 // eight wavefronts share each SIMD.
-template <int BS, bool NTG, bool NTS, bool COMPACT>
+template <int BS, bool NTG, bool NTS, bool COMPACT, int KG>
 __global__ __launch_bounds__(BS) void k_sweep(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
-                                              const int2* __restrict__ tiles, int n_tiles) {
-  sweep_tile<BS, NTG, NTS, COMPACT>(D, L, tiles, n_tiles);
+                                              const int2* __restrict__ tiles, int n_tiles,
+                                              const int* __restrict__ groups, int single) {
+  sweep_tile<BS, NTG, NTS, COMPACT, KG>(D, L, tiles, n_tiles, groups, single);
 }
 template <int BS, bool NTG, bool NTS, bool COMPACT>
+__global__ __launch_bounds__(BS) void k_sweep_lds(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                                                  const int2* __restrict__ tiles, int n_tiles,
+                                                  const int* __restrict__ groups, int single) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  if (b >= n_tiles) return;
+  const int2 tl = single ? make_int2(0, b) : tiles[b];
+  const int r = single ? 0 : groups[tl.x];
+  if (L[r].done | L[r].ffwd) return;
+  sweep_tile_lds<BS, NTG, NTS, COMPACT>(D, L, r, tl.y);
+}
+template <int BS, bool NTG, bool NTS, bool COMPACT, int KG>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_sweep_o8(
-    const RealDesc* __restrict__ D, const RealLive* __restrict__ L, const int2* __restrict__ tiles, int n_tiles) {
-  sweep_tile<BS, NTG, NTS, COMPACT>(D, L, tiles, n_tiles);
+    const RealDesc* __restrict__ D, const RealLive* __restrict__ L, const int2* __restrict__ tiles, int n_tiles,
+    const int* __restrict__ groups, int single) {
+  sweep_tile<BS, NTG, NTS, COMPACT, KG>(D, L, tiles, n_tiles, groups, single);
 }
 
 __device__ void write_record(const RealDesc& d, RealLive& l, const double* Ca, double t) {
@@ -318,7 +477,7 @@ __device__ void write_record(const RealDesc& d, RealLive& l, const double* Ca, d
   rec[k++] = (float)t;
   double i_ryr = 0;
   for (int i = 0; i < d.n_ryr; ++i)
-    i_ryr += d.ryr_state[i] * d.P[SRS_R_V_RYR] * (Ca[4 * d.ryr_jsr[i]] - Ca[4 * d.ryr_ele[i]]) * (d.P[SRS_R_V_CELL]) * 96485 *
+    i_ryr += d.ryr_state[i] * d.P[SRS_R_V_RYR] * (Ca[cq(d, d.ryr_jsr[i])] - Ca[cq(d, d.ryr_ele[i])]) * (d.P[SRS_R_V_CELL]) * 96485 *
              2 * (1e-6);
   rec[k++] = (float)i_ryr;
   int n_open = 0;
@@ -331,18 +490,18 @@ __device__ void write_record(const RealDesc& d, RealLive& l, const double* Ca, d
   double i_lcc = 0;
   for (int i = 0; i < d.n_lcc; ++i)
     if (srs_lcc_open(d.lcc_state[i], d.lccv_state[i]))
-      i_lcc += srs_lcc_flux_raw(l.vm, Ca[4 * d.lcc_ele[i]], d.P[SRS_R_CA_0]) * 96485 * 2 * (1e-6);
+      i_lcc += srs_lcc_flux_raw(l.vm, Ca[cq(d, d.lcc_ele[i])], d.P[SRS_R_CA_0]) * 96485 * 2 * (1e-6);
   rec[k++] = (float)i_lcc;
   double m = 0;
-  for (int i = 0; i < d.n_ryr; ++i) m += Ca[4 * d.ryr_jsr[i]];
+  for (int i = 0; i < d.n_ryr; ++i) m += Ca[cq(d, d.ryr_jsr[i])];
   m /= d.n_ryr;
   rec[k++] = (float)m;
   m = 0;
-  for (int i = 0; i < d.n_ryr; ++i) m += Ca[4 * d.ryr_ele[i]];
+  for (int i = 0; i < d.n_ryr; ++i) m += Ca[cq(d, d.ryr_ele[i])];
   m /= d.n_ryr;
   rec[k++] = (float)m;
   if (d.gpu_layout)
-    for (int i = 0; i < d.n_ryr; ++i) rec[k++] = (float)Ca[4 * d.ryr_ele[i]];
+    for (int i = 0; i < d.n_ryr; ++i) rec[k++] = (float)Ca[cq(d, d.ryr_ele[i])];
   if (d.ryr_out)
     for (int i = 0; i < d.n_ryr; ++i) rec[k++] = (float)d.ryr_state[i];
   l.rec_count++;
@@ -363,7 +522,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
   const bool ff = l.ffwd != 0;
   const bool swept = (mode & G_AFTER_SWEEP) && !ff;
   const double* Xn = swept ? d.X[cur ^ 1] : d.X[cur];
-  const double* Ca = Xn;  // Ca of element e at Ca[4 * e] (interleaved diffusing species)
+  const double* Ca = Xn;  // Ca of element e at Ca[cq(d, e)] (interleaved diffusing species)
   const double* P = d.P;
   const double dt = P[SRS_R_T_STEP];
   const int64_t step = l.step;
@@ -395,7 +554,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
               if (s) no++; else ncl++;
             }
           }
-          const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[4 * d.ryr_ele[i]], Ca[4 * d.ryr_jsr[i]], P);
+          const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[cq(d, d.ryr_ele[i])], Ca[cq(d, d.ryr_jsr[i])], P);
           const double Rn = d.R[i] + rate * dt;
           const int ns = (Rn > 0) ? (s_old[i] == 1 ? 0 : 1) : s_old[i];
           next[i] = ns;
@@ -417,7 +576,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
           if (s) no++; else ncl++;
         }
       }
-      const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[4 * d.ryr_ele[i]], Ca[4 * d.ryr_jsr[i]], P);
+      const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[cq(d, d.ryr_ele[i])], Ca[cq(d, d.ryr_jsr[i])], P);
       double Rn = d.R[i] + rate * dt;
       int ns = s_old[i];
       if (Rn > 0) {
@@ -434,7 +593,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
       const int s = d.lcc_state[i], vs = d.lccv_state[i];
       double tr[3];
       int32_t tg[3];
-      const double lr = srs_lcc_rates(s, vs, l.vm, Ca[4 * d.lcc_ele[i]], tr, tg);
+      const double lr = srs_lcc_rates(s, vs, l.vm, Ca[cq(d, d.lcc_ele[i])], tr, tg);
       const int ic = nr + i;
       double Rn = d.R[ic] + (tr[0] + tr[1] + tr[2] + lr) * dt;
       if (Rn > 0) {
@@ -507,7 +666,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
     double cmax = 0;
     for (int i = 0; i < nr; ++i) {
       n_open += d.ryr_state[i];
-      const double c = Ca[4 * d.ryr_ele[i]];
+      const double c = Ca[cq(d, d.ryr_ele[i])];
       if (c > cmax) cmax = c;
     }
     l.n_open_last = n_open;
@@ -519,12 +678,12 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
 // Gathers Ca at the channel elements of one realization.
 __global__ void k_channel_ca(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int r, double* out) {
   const RealDesc& d = D[r];
-  const double* Ca = d.X[L[r].cur];  // Ca of element e at Ca[4 * e]
+  const double* Ca = d.X[L[r].cur];  // Ca of element e at Ca[cq(d, e)]
   for (int i = threadIdx.x; i < d.n_ryr; i += blockDim.x) {
-    out[i] = Ca[4 * d.ryr_ele[i]];
-    out[d.n_ryr + i] = Ca[4 * d.ryr_jsr[i]];
+    out[i] = Ca[cq(d, d.ryr_ele[i])];
+    out[d.n_ryr + i] = Ca[cq(d, d.ryr_jsr[i])];
   }
-  for (int i = threadIdx.x; i < d.n_lcc; i += blockDim.x) out[2 * d.n_ryr + i] = Ca[4 * d.lcc_ele[i]];
+  for (int i = threadIdx.x; i < d.n_lcc; i += blockDim.x) out[2 * d.n_ryr + i] = Ca[cq(d, d.lcc_ele[i])];
 }
 
 // ------------------------------------------------------------------ linescan
@@ -556,7 +715,7 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_blur(const RealDesc* __restric
     const int j = (blockIdx.x * SCAN_CELLS_PER_THREAD + q) * SCAN_BS + threadIdx.x;
     if (j < n) {
       cx[q] = cen[j]; cy[q] = cen[np + j]; cz[q] = cen[2 * np + j];
-      vf[q] = d.V0[j] * X[4 * (size_t)j + 3];  // V0 * Ca-bound Fluo
+      vf[q] = d.V0[j] * X[cq(d, j) + 3];  // V0 * Ca-bound Fluo
     } else {
       cx[q] = cy[q] = cz[q] = 1e30;  // beyond the 1 um cut-off of every point
       vf[q] = 0.0;
@@ -597,12 +756,26 @@ __global__ void k_scan_final(const RealDesc* __restrict__ D, const RealLive* __r
     for (int b = 0; b < n_blocks; ++b) s += part[(size_t)b * n_f + i];
     out[n_ca + n_f + i] = s + pad[i];
     const int e = near_f[i];
-    out[n_ca + i] = e >= 0 ? X[4 * (size_t)e + 3] : f0;
+    out[n_ca + i] = e >= 0 ? X[cq(D[r], e) + 3] : f0;
   }
   for (int i = threadIdx.x; i < n_ca; i += blockDim.x) {
     const int e = near_ca[i];
-    out[i] = e >= 0 ? X[4 * (size_t)e] : c0;
+    out[i] = e >= 0 ? X[cq(D[r], e)] : c0;
   }
+}
+
+// Moves one realization's state between the single-realization layout
+// (np quads, TnC[np], CSQ[np]) and its slot of a group buffer.
+__global__ void k_slot_copy(const double* __restrict__ src, double* __restrict__ dst, size_t np, int K, int slot,
+                            int to_group) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 6 * np) return;
+  size_t g;
+  if (i < 4 * np) g = 4 * gidx(i / 4, K, slot) + i % 4;
+  else if (i < 5 * np) g = 4 * (size_t)K * np + gidx(i - 4 * np, K, slot);
+  else g = 5 * (size_t)K * np + gidx(i - 5 * np, K, slot);
+  if (to_group) dst[g] = src[i];
+  else dst[i] = src[g];
 }
 
 // ----------------------------------------------------------------- host side
@@ -749,6 +922,8 @@ struct srs_ctx {
   int plane_pad = 0;     // extra elements per SoA plane
   int compact = 1;       // compress operator rows when a mesh allows it
   int occ8 = 0;          // sweep compiled for eight wavefronts per SIMD
+  int group = 1;         // realizations per realization group (1, 2 or 4)
+  int lds = 0;           // stage each tile's quads in LDS for the neighbour gathers (groups of 1)
   bool compact_all = false;
   bool layout_fixed = false;
   hipStream_t stream = nullptr;
@@ -759,8 +934,11 @@ struct srs_ctx {
   std::vector<void*> allocs;
   RealDesc* d_desc = nullptr;
   RealLive* d_live = nullptr;
-  int2* d_tiles = nullptr;
+  int2* d_tiles = nullptr;   // {group, tile}
   int n_tiles = 0, grid_sweep = 0;
+  int32_t* d_groups = nullptr;  // [n_groups][group] realization ids, -1 pads
+  double* stage = nullptr;       // staging buffer for state transfers of grouped realizations
+  size_t stage_len = 0;
   double last_total_ms = 0, last_sweep_ms = 0;
   // captured step blocks, keyed by (steps, gate mode)
   std::map<std::pair<int64_t, uint32_t>, hipGraphExec_t> graphs;
@@ -794,32 +972,43 @@ int check_real(srs_ctx* c, int32_t r) {
   return SRS_OK;
 }
 
-using SweepFn = void (*)(const RealDesc*, const RealLive*, const int2*, int);
+using SweepFn = void (*)(const RealDesc*, const RealLive*, const int2*, int, const int*, int);
 
-template <int BS, bool CMP>
-SweepFn pick3(int ntg, int nts, int occ8) {
-  if (occ8) {
-    if (ntg) return nts ? k_sweep_o8<BS, true, true, CMP> : k_sweep_o8<BS, true, false, CMP>;
-    return nts ? k_sweep_o8<BS, false, true, CMP> : k_sweep_o8<BS, false, false, CMP>;
+template <int BS, bool CMP, int KG>
+SweepFn pick3(int ntg, int nts, int occ8, int lds) {
+  if (KG == 1 && lds) {
+    if (ntg) return nts ? k_sweep_lds<BS, true, true, CMP> : k_sweep_lds<BS, true, false, CMP>;
+    return nts ? k_sweep_lds<BS, false, true, CMP> : k_sweep_lds<BS, false, false, CMP>;
   }
-  if (ntg) return nts ? k_sweep<BS, true, true, CMP> : k_sweep<BS, true, false, CMP>;
-  return nts ? k_sweep<BS, false, true, CMP> : k_sweep<BS, false, false, CMP>;
+  if (occ8) {
+    if (ntg) return nts ? k_sweep_o8<BS, true, true, CMP, KG> : k_sweep_o8<BS, true, false, CMP, KG>;
+    return nts ? k_sweep_o8<BS, false, true, CMP, KG> : k_sweep_o8<BS, false, false, CMP, KG>;
+  }
+  if (ntg) return nts ? k_sweep<BS, true, true, CMP, KG> : k_sweep<BS, true, false, CMP, KG>;
+  return nts ? k_sweep<BS, false, true, CMP, KG> : k_sweep<BS, false, false, CMP, KG>;
 }
 
-template <bool CMP>
+template <bool CMP, int KG>
 SweepFn pick_tile(const srs_ctx* c) {
   switch (c->tile) {
-    case 128: return pick3<128, CMP>(c->nt_geometry, c->nt_state, c->occ8);
-    case 512: return pick3<512, CMP>(c->nt_geometry, c->nt_state, c->occ8);
-    default: return pick3<256, CMP>(c->nt_geometry, c->nt_state, c->occ8);
+    case 128: return pick3<128, CMP, KG>(c->nt_geometry, c->nt_state, c->occ8, c->lds);
+    case 512: return pick3<512, CMP, KG>(c->nt_geometry, c->nt_state, c->occ8, c->lds);
+    default: return pick3<256, CMP, KG>(c->nt_geometry, c->nt_state, c->occ8, c->lds);
   }
 }
 
-SweepFn pick_sweep(const srs_ctx* c) { return c->compact_all ? pick_tile<true>(c) : pick_tile<false>(c); }
+template <int KG>
+SweepFn pick_kg(const srs_ctx* c) { return c->compact_all ? pick_tile<true, KG>(c) : pick_tile<false, KG>(c); }
+
+SweepFn pick_sweep(const srs_ctx* c) {
+  return c->group == 4 ? pick_kg<4>(c) : c->group == 2 ? pick_kg<2>(c) : pick_kg<1>(c);
+}
 
 void launch_sweep(srs_ctx* c) {
+  // one realization: tile b is tile b of realization 0, no table lookups
+  const int single = c->reals.size() == 1 ? 1 : 0;
   hipLaunchKernelGGL(pick_sweep(c), dim3(c->grid_sweep), dim3(c->tile), 0, c->stream, c->d_desc, c->d_live,
-                     c->d_tiles, c->n_tiles);
+                     c->d_tiles, c->n_tiles, (const int*)c->d_groups, single);
 }
 
 int launch_step(srs_ctx* c, uint32_t gmode) {
@@ -877,6 +1066,12 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
     case SRS_TUNE_NT_GEOMETRY: c->nt_geometry = value != 0; return SRS_OK;
     case SRS_TUNE_OCCUPANCY: c->occ8 = value == 8; return SRS_OK;
     case SRS_TUNE_NT_STATE: c->nt_state = value != 0; return SRS_OK;
+    case SRS_TUNE_LDS: c->lds = value != 0; return SRS_OK;
+    case SRS_TUNE_GROUP:
+      if (value != 1 && value != 2 && value != 4) return srs::fail(SRS_E_ARG, "group must be 1, 2 or 4");
+      if (c->finalized) return srs::fail(SRS_E_STATE, "group must be set before srs_ctx_finalize");
+      c->group = value;
+      return SRS_OK;
     default: return srs::fail(SRS_E_ARG, "unknown tuning key %d", key);
   }
 }
@@ -887,6 +1082,7 @@ int srs_ctx_destroy(srs_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stage) (void)hipFree(c->stage);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
   (void)hipStreamDestroy(c->stream);
@@ -1061,6 +1257,39 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
   std::vector<RealDesc> descs;
   std::vector<RealLive> lives;
   std::vector<int2> tiles;
+  // Realization groups: up to c->group realizations of one mesh, in the
+  // order they were added, share interleaved state buffers and each sweep
+  // tile of the group loads the operator rows once for all of them.
+  const int KG = c->group;
+  std::vector<int32_t> groups;     // [n_groups][KG], -1 pads
+  std::vector<int32_t> real_group(c->reals.size()), real_slot(c->reals.size());
+  std::vector<int32_t> group_mesh;
+  for (size_t m = 0; m < c->meshes.size(); ++m) {
+    int cur_g = -1, fill = KG;
+    for (size_t r = 0; r < c->reals.size(); ++r) {
+      if (c->reals[r].mesh != (int)m) continue;
+      if (fill == KG) {
+        cur_g = (int)group_mesh.size();
+        group_mesh.push_back((int)m);
+        groups.insert(groups.end(), KG, -1);
+        fill = 0;
+      }
+      groups[(size_t)KG * cur_g + fill] = (int32_t)r;
+      real_group[r] = cur_g;
+      real_slot[r] = fill++;
+    }
+  }
+  const int n_groups = (int)group_mesh.size();
+  std::vector<double*> gx0(n_groups), gx1(n_groups);
+  for (int gi = 0; gi < n_groups; ++gi) {
+    const HostMesh& hm = *c->meshes[group_mesh[gi]];
+    const size_t cnt = 6 * (size_t)KG * hm.np;
+    if ((rc = dev_alloc(c, &gx0[gi], cnt)) || (rc = dev_alloc(c, &gx1[gi], cnt))) return rc;
+    HIPCHK(hipMemset(gx0[gi], 0, cnt * sizeof(double)));
+    HIPCHK(hipMemset(gx1[gi], 0, cnt * sizeof(double)));
+    const int nt = (hm.n + c->tile - 1) / c->tile;
+    for (int t = 0; t < nt; ++t) tiles.push_back(make_int2(gi, t));
+  }
   for (size_t r = 0; r < c->reals.size(); ++r) {
     HostReal& hr = c->reals[r];
     HostMesh& hm = *c->meshes[hr.mesh];
@@ -1069,11 +1298,9 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
     d.off = hm.d_off; d.cls = hm.d_cls; d.dq = hm.d_dq; d.tab = hm.d_tab; d.wsum = hm.d_wsum;
     d.n_class = hm.n_class;
     d.n = hm.n; d.np = hm.np;
-    double *x0, *x1;
-    if ((rc = dev_alloc(c, &x0, 6 * (size_t)hm.np)) || (rc = dev_alloc(c, &x1, 6 * (size_t)hm.np))) return rc;
-    HIPCHK(hipMemset(x0, 0, 6 * (size_t)hm.np * sizeof(double)));
-    HIPCHK(hipMemset(x1, 0, 6 * (size_t)hm.np * sizeof(double)));
-    d.X[0] = x0; d.X[1] = x1;
+    d.X[0] = gx0[real_group[r]]; d.X[1] = gx1[real_group[r]];
+    d.kst = KG;
+    d.slot = real_slot[r];
     d.n_ryr = (int32_t)hr.ryr_ele.size();
     d.n_lcc = (int32_t)hr.lcc_ele.size();
     int32_t *p;
@@ -1112,19 +1339,54 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
     lv.ca_nsr = hr.reals[SRS_R_DEFAULT_CA_NSR];
     serca_terms(lv.ca_nsr, hr.reals, lv.serca);
     lives.push_back(lv);
-    const int nt = (hm.n + c->tile - 1) / c->tile;
-    for (int t = 0; t < nt; ++t) tiles.push_back(make_int2((int)r, t));
   }
   c->n_tiles = (int)tiles.size();
   c->grid_sweep = (c->n_tiles + NXCD - 1) / NXCD * NXCD;
   if ((rc = dev_upload(c, &c->d_desc, descs))) return rc;
   if ((rc = dev_upload(c, &c->d_live, lives))) return rc;
   if ((rc = dev_upload(c, &c->d_tiles, tiles))) return rc;
+  if ((rc = dev_upload(c, &c->d_groups, groups))) return rc;
   // meshes' channel flags were updated by add_realization; refresh them
   for (auto& hmp : c->meshes)
     HIPCHK(hipMemcpy(hmp->d_flags, hmp->flags.data(), hmp->flags.size(), hipMemcpyHostToDevice));
   HIPCHK(hipDeviceSynchronize());
   c->finalized = true;
+  return SRS_OK;
+}
+
+// Copies one realization's state between a host buffer in the single
+// realization layout (np quads, then the TnC and CSQ planes) and its slot of
+// the group buffer: contiguous through a device staging buffer, then a
+// scatter (or gather) kernel for the slot.
+static int slot_copy(srs_ctx* c, const RealDesc& d, double* dev, double* host, bool to_dev) {
+  const size_t np = d.np, n6 = 6 * np;
+  const hipMemcpyKind kind = to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  if (d.kst == 1) {
+    if (to_dev) HIPCHK(hipMemcpy(dev, host, n6 * sizeof(double), kind));
+    else HIPCHK(hipMemcpy(host, dev, n6 * sizeof(double), kind));
+    return SRS_OK;
+  }
+  if (c->stage_len < n6) {
+    if (c->stage) HIPCHK(hipFree(c->stage));
+    c->stage = nullptr;
+    c->stage_len = 0;
+    HIPCHK(hipMalloc((void**)&c->stage, n6 * sizeof(double)));
+    c->stage_len = n6;
+  }
+  const unsigned blocks = (unsigned)((n6 + 255) / 256);
+  if (to_dev) {
+    HIPCHK(hipMemcpy(c->stage, host, n6 * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_slot_copy, dim3(blocks), dim3(256), 0, c->stream, (const double*)c->stage, dev, np, d.kst,
+                       d.slot, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    hipLaunchKernelGGL(k_slot_copy, dim3(blocks), dim3(256), 0, c->stream, (const double*)dev, c->stage, np, d.kst,
+                       d.slot, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(host, c->stage, n6 * sizeof(double), hipMemcpyDeviceToHost));
+  }
   return SRS_OK;
 }
 
@@ -1139,9 +1401,8 @@ int srs_state_set(srs_ctx* c, int32_t r, const double* X) {
     for (int i = 0; i < hm.n; ++i) buf[xoff(s, hm.inv[i], hm.np)] = X[(size_t)s * hm.n + i];
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpy(hr.desc.X[0], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(hr.desc.X[1], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
-  return SRS_OK;
+  if ((rc = slot_copy(c, hr.desc, hr.desc.X[0], buf.data(), true))) return rc;
+  return slot_copy(c, hr.desc, hr.desc.X[1], buf.data(), true);
 }
 
 int srs_state_get(srs_ctx* c, int32_t r, double* X) {
@@ -1155,7 +1416,7 @@ int srs_state_get(srs_ctx* c, int32_t r, double* X) {
   RealLive lv;
   HIPCHK(hipMemcpy(&lv, c->d_live + r, sizeof(lv), hipMemcpyDeviceToHost));
   std::vector<double> buf(6 * (size_t)hm.np);
-  HIPCHK(hipMemcpy(buf.data(), hr.desc.X[lv.cur], buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if ((rc = slot_copy(c, hr.desc, hr.desc.X[lv.cur], buf.data(), false))) return rc;
   for (int s = 0; s < 6; ++s)
     for (int i = 0; i < hm.n; ++i) X[(size_t)s * hm.n + i] = buf[xoff(s, hm.inv[i], hm.np)];
   return SRS_OK;

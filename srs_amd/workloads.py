@@ -241,7 +241,7 @@ def initial_channels(pr: Prepared, open_ryr=None, seed=0):
 
 
 def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0, t_final: float = 50.0,
-                     order: str = "block", tune: dict | None = None, only=None):
+                     order: str = "block", tune: dict | None = None, only=None, log=None):
     """The batched Monte Carlo workload in one context: returns (ctx, dt per
     realization, total elements over realizations, algorithmic sweep bytes
     per step).  Algorithmic bytes: 96 B per element of every realization (6
@@ -256,7 +256,7 @@ def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0
     picked = list(range(n_real)) if only is None else list(only)
     ctx = Context(0)
     tune = dict(tune or {})
-    ctx.tune(**{k: tune[k] for k in ("tile", "plane_pad", "compact") if k in tune})
+    ctx.tune(**{k: tune[k] for k in ("tile", "plane_pad", "compact", "group") if k in tune})
     meshes = []
     for m in range(3):
         g = box_grid(ensemble_mesh_spec(m))
@@ -283,6 +283,8 @@ def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0
         n_ele += g.n_ele
         nbytes += 96 * g.n_ele
         plist.append((g, p, len(idx)))
+        if log and len(plist) % 256 == 0:
+            log(f"  added {len(plist)} realizations")
     ctx.finalize(record_capacity)
     for k, (g, p, nr) in enumerate(plist):
         r = picked[k]
@@ -294,7 +296,9 @@ def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0
         R = np.log(rng.random(nr + g.n_lcc) * (1 - 2**-53) + 2**-54)
         ctx.channels_set(k, rs, np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R)
         ctx.globals_set(k, 0.0, p.reals[16])
-    ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state", "occupancy") if k in tune})
+        if log and (k + 1) % 256 == 0:
+            log(f"  initialised {k + 1} realizations")
+    ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state", "occupancy", "lds") if k in tune})
     return ctx, np.array(dts), n_ele, nbytes
 
 

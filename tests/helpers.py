@@ -32,9 +32,11 @@ def initial_fields(pr, vm=0.0, ryr_open=(0,), lcc_state=6, lccv_state=2, seed=0)
                   [vm, pr.params.reals[S.R["DEFAULT_CA_NSR"]]])
 
 
-def gpu_context(preps, fields, params=None, seeds=None, order=S.ORDER_RCM, record_capacity=0, share_mesh=False):
+def gpu_context(preps, fields, params=None, seeds=None, order=S.ORDER_RCM, record_capacity=0, share_mesh=False,
+                group=1):
     """A context with one realization per prepared workload."""
     ctx = S.Context(0)
+    ctx.tune(group=group)
     mids = []
     for k, pr in enumerate(preps):
         if share_mesh and k > 0:

@@ -89,7 +89,8 @@ def test_graph_replay_equals_launches():
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
-def test_batch_equals_single_runs():
+@pytest.mark.parametrize("lds", [0, 1])
+def test_batch_equals_single_runs(lds):
     """Realizations of different meshes and parameters in one launch reproduce
     their single-realization runs bit for bit."""
     preps = [_tiny(tt=False), _tiny(tt=True), W.prepare(W.tiny_box_spec(nx=5, ny=3), W.spark_params())]
@@ -98,6 +99,7 @@ def test_batch_equals_single_runs():
     fields = [initial_fields(pr, seed=k) for k, pr in enumerate(preps)]
     seeds = [11, 12, 13]
     with gpu_context(preps, fields, params=params, seeds=seeds) as ctx:
+        ctx.tune(lds=lds)
         ctx.explicit_step(250, S.STEP_GRAPH)
         batch = [(ctx.state_get(k), ctx.channels_get(k)[0]) for k in range(3)]
     for k, pr in enumerate(preps):
@@ -105,6 +107,33 @@ def test_batch_equals_single_runs():
                             params[k].reals[1], gating=True, rng_kind=RNG_PHILOX, key=seeds[k], stream=k)
         assert np.array_equal(batch[k][1], Fo.ryr_state)
         assert rel_err(batch[k][0], Fo.X) <= TOL
+
+
+@pytest.mark.parametrize("group", [2, 4])
+def test_realization_groups_equal_single_runs(group):
+    """Realizations of one mesh interleaved in groups (operator rows loaded
+    once per group): each equals its oracle run; a realization marked done
+    halfway keeps its state while its group partners go on."""
+    pr = _tiny(tt=True)
+    n_real = 5
+    params = [fast_gating(pr.params) for _ in range(n_real)]
+    for k, p in enumerate(params):
+        p.setr(C_0_CA_JSR=600.0 + 100 * k, DEFAULT_CA_NSR=600.0 + 100 * k)
+    fields = [initial_fields(pr, ryr_open=(k % pr.grid.n_ryr,), seed=20 + k) for k in range(n_real)]
+    seeds = [40 + k for k in range(n_real)]
+    with gpu_context([pr] * n_real, fields, params=params, seeds=seeds, share_mesh=True, group=group) as ctx:
+        ctx.explicit_step(120, S.STEP_GRAPH)
+        X1 = ctx.state_get(1)
+        ctx.loop_set(1, step=120, done=1)
+        ctx.explicit_step(130, S.STEP_GRAPH)
+        got = [(ctx.state_get(k), ctx.channels_get(k)[0]) for k in range(n_real)]
+    assert np.array_equal(got[1][0], X1)
+    for k in range(n_real):
+        n = 120 if k == 1 else 250
+        Fo = Oracle().steps(grid_dict(pr.grid, pr.aij, pr.nij), params[k].reals, fields[k].copy(), n,
+                            params[k].reals[1], gating=True, rng_kind=RNG_PHILOX, key=seeds[k], stream=k)
+        assert np.array_equal(got[k][1], Fo.ryr_state), k
+        assert rel_err(got[k][0], Fo.X) <= TOL, k
 
 
 def test_protocol_loop_records_match_oracle_loop():
@@ -362,7 +391,9 @@ def test_no_channels_and_done_realizations():
 
 
 @pytest.mark.parametrize("cfg", [dict(tile=128, occupancy=8), dict(tile=128, nt_geometry=1), dict(tile=512, nt_state=1),
-                                 dict(tile=256, plane_pad=448, nt_geometry=1, nt_state=1), dict(compact=0)])
+                                 dict(tile=256, plane_pad=448, nt_geometry=1, nt_state=1), dict(compact=0),
+                                 dict(group=2), dict(group=4, compact=0, occupancy=8), dict(lds=1),
+                                 dict(lds=1, tile=128, compact=0, nt_state=1), dict(lds=1, tile=512, nt_geometry=1)])
 def test_tunables_do_not_change_results(cfg):
     pr = _tiny(tt=True)
     p = fast_gating(pr.params)
@@ -370,12 +401,12 @@ def test_tunables_do_not_change_results(cfg):
     outs = []
     for tune in ({}, cfg):
         ctx = S.Context(0)
-        ctx.tune(**{k: v for k, v in tune.items() if k in ("tile", "plane_pad", "compact")})
+        ctx.tune(**{k: v for k, v in tune.items() if k in ("tile", "plane_pad", "compact", "group")})
         mid = ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, perm=W.kuhn_block_order(pr.grid))
         g = pr.grid
         ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=4, stream=0)
         ctx.finalize()
-        ctx.tune(**{k: v for k, v in tune.items() if k in ("nt_geometry", "nt_state", "occupancy")})
+        ctx.tune(**{k: v for k, v in tune.items() if k in ("nt_geometry", "nt_state", "occupancy", "lds")})
         ctx.state_set(0, F.X)
         ctx.channels_set(0, F.ryr_state, F.lcc_state, F.lccv_state, F.R)
         ctx.globals_set(0, F.gv[0], F.gv[1])
