@@ -413,6 +413,35 @@ int srs_ctx_add_scan(srs_ctx* ctx, int32_t mesh_id, const double* centers, const
  * padding cells included).  dF/F0 = cafb / f0 - 1. */
 int srs_scan_sample(srs_ctx* ctx, int32_t real, int32_t scan_id, double* ca, double* caf, double* cafb);
 
+/* ====================================================== adjoint (fitting) */
+/* These calls have no counterpart in the reference: they serve parameter
+ * fitting against line scans with the channel trajectory held fixed. */
+
+/* Number of parameters with gradients, and their SRS_R_* indices in order:
+ * D_CA, D_DYE, K_ON/K_OFF of CaM, TnC, ATP and Fluo, V_RYR, A_P (SERCA pump
+ * density, the Vmax scale), K_D_SL_HI. */
+#define SRS_N_THETA 13
+int srs_adjoint_theta(int32_t idx[SRS_N_THETA]);
+
+/* Replaces a realization's parameter vector after srs_ctx_finalize. */
+int srs_params_set(srs_ctx* ctx, int32_t real, const double reals[SRS_N_REAL]);
+
+/* n_steps plain steps with gating of a one-realization context, recording
+ * schedule[k*n_ryr + i] = state of RyR i during step k (row 0 = the states
+ * at the call). */
+int srs_record_schedule(srs_ctx* ctx, int32_t real, int64_t n_steps, int32_t* schedule);
+
+/* From the realization's current state, n_steps steps with the RyR states of
+ * each step taken from `schedule` and gating off (LCC states held), sampling
+ * scan line `scan_id` after sample_steps[k] steps (increasing, 1..n_steps).
+ * loss = sum over samples and fluorescence points of (cafb/f0 - 1 - target)^2.
+ * If grad is not NULL, the adjoint computes d loss / d theta (srs_adjoint_theta
+ * order) with checkpoints every `ckpt` steps and recomputation.  The
+ * realization's state, channel states and loop state are restored. */
+int srs_linescan_loss_grad(srs_ctx* ctx, int32_t real, int32_t scan_id, int64_t n_steps, const int32_t* schedule,
+                           int32_t n_samples, const int64_t* sample_steps, const double* target, int32_t ckpt,
+                           double* loss, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -295,6 +295,16 @@ def read_scan_parameters(path: str | None = None) -> _abi.SrsScanDesc:
     return d
 
 
+N_THETA = 13
+
+
+def adjoint_theta() -> np.ndarray:
+    """SRS_R_* indices of the parameters srs_linescan_loss_grad differentiates."""
+    idx = np.zeros(N_THETA, np.int32)
+    check(lib().srs_adjoint_theta(_i32(idx)))
+    return idx
+
+
 def dff0(cafb: np.ndarray, f0: float) -> np.ndarray:
     """dF/F0 of a blurred fluorescence scan line against the resting signal."""
     return np.asarray(cafb) / f0 - 1.0
@@ -480,6 +490,32 @@ class Context:
         ca, caf, cafb = np.zeros(nca), np.zeros(nf), np.zeros(nf)
         check(lib().srs_scan_sample(self._h, r, scan_id, _f64(ca), _f64(caf), _f64(cafb)))
         return ca, caf, cafb
+
+    def params_set(self, r: int, reals: np.ndarray):
+        """Replaces realization r's parameter vector (SRS_R_* layout)."""
+        P = np.ascontiguousarray(reals, np.float64)
+        check(lib().srs_params_set(self._h, r, _f64(P)))
+
+    def record_schedule(self, r: int, n_steps: int) -> np.ndarray:
+        """n_steps steps with gating; row k = RyR states during step k."""
+        _, nr, _ = self.info(r)
+        out = np.zeros((n_steps, nr), np.int32)
+        check(lib().srs_record_schedule(self._h, r, int(n_steps), _i32(out)))
+        return out
+
+    def loss_grad(self, r: int, scan_id: int, schedule: np.ndarray, sample_steps, target: np.ndarray, ckpt: int = 256,
+                  grad: bool = True):
+        """Line-scan loss of a replayed run and (optionally) its adjoint
+        gradient in adjoint_theta() order; the realization is left as it was."""
+        sch = np.ascontiguousarray(schedule, np.int32)
+        ss = np.ascontiguousarray(sample_steps, np.int64)
+        tg = np.ascontiguousarray(target, np.float64)
+        loss = C.c_double()
+        g = np.zeros(N_THETA) if grad else None
+        check(lib().srs_linescan_loss_grad(self._h, r, scan_id, int(sch.shape[0]), _i32(sch), len(ss),
+                                           ss.ctypes.data_as(C.POINTER(C.c_int64)), _f64(tg), int(ckpt),
+                                           C.byref(loss), _f64(g) if grad else None))
+        return (loss.value, g) if grad else loss.value
 
     def time_sweep(self, reps: int):
         a, b = C.c_double(), C.c_double()

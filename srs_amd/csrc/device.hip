@@ -764,6 +764,248 @@ __global__ void k_scan_final(const RealDesc* __restrict__ D, const RealLive* __r
   }
 }
 
+// ------------------------------------------------------------------- adjoint
+// Reverse mode of the explicit step with the channel trajectory frozen
+// (RyR states from a recorded schedule, LCC states held): for
+// X_{n+1} = X_n + dt r(X_n; theta), lambda_n = lambda_{n+1} + dt J^T lambda_{n+1}
+// and dL/dtheta += dt (dr/dtheta)^T lambda_{n+1}.  The diffusion operator
+// satisfies V_i A_ij = V_j A_ji (both equal the face coefficient), so
+// (A^T y)_j = V_j (A (y / V))_j: the transpose is another gather over the same
+// rows.  Single-realization layout (quads, then the TnC and CSQ planes).
+constexpr int ADJ_NT = 13;   // D_CA, D_DYE, kon/koff of CaM, TnC, ATP, Fluo, V_RYR, A_P, K_D_SL_HI
+constexpr int ADJ_BS = 256;
+__constant__ const int kAdjTheta[ADJ_NT] = {SRS_R_D_CA,     SRS_R_D_DYE,     SRS_R_K_ON_CMDN, SRS_R_K_OFF_CMDN,
+                                            SRS_R_K_ON_TRPN, SRS_R_K_OFF_TRPN, SRS_R_K_ON_ATP, SRS_R_K_OFF_ATP,
+                                            SRS_R_K_ON_DYE, SRS_R_K_OFF_DYE, SRS_R_V_RYR,     SRS_R_A_P,
+                                            SRS_R_K_D_SL_HI};
+
+// 1 / beta of the rapid SL buffer at element j and its derivatives in c and
+// in K_D_SL_HI (cpu/RunSimulation.cpp:15-22).
+struct SlBuf { double beta, dbeta_dc, dbeta_dkh; };
+__device__ __forceinline__ SlBuf sl_buffer(const RealDesc& d, int j, double c, uint8_t f) {
+  SlBuf s{1.0, 0.0, 0.0};
+  if (!(f & F_SL)) return s;
+  const double* P = d.P;
+  const double ttj = d.tt[j], v0j = d.V0[j];
+  const double bhi = ttj * P[SRS_R_B_TOT_SL_HI] / (v0j * (1e-15));
+  const double blo = ttj * P[SRS_R_B_TOT_SL_LO] / (v0j * (1e-15));
+  const double kh = P[SRS_R_K_D_SL_HI], kl = P[SRS_R_K_D_SL_LO];
+  const double eh = kh + c, el = kl + c;
+  s.beta = 1 + (bhi * kh / (eh * eh)) + (blo * kl / (el * el));
+  s.dbeta_dc = -2 * bhi * kh / (eh * eh * eh) - 2 * blo * kl / (el * el * el);
+  s.dbeta_dkh = bhi * (c - kh) / (eh * eh * eh);
+  return s;
+}
+
+// Pass 1: w = (lambda_Ca / beta, lambda_ATP, lambda_CaM, lambda_Fluo) / V0.
+__global__ __launch_bounds__(ADJ_BS) void k_adj_w(const RealDesc* __restrict__ D, const double* __restrict__ X,
+                                                  const double* __restrict__ lam, d4* __restrict__ w) {
+  const RealDesc& d = D[0];
+  const int j = blockIdx.x * ADJ_BS + threadIdx.x;
+  if (j >= d.n) return;
+  const uint8_t f = d.flags[j];
+  const d4 m = ((const d4*)lam)[j];
+  const double c = X[4 * (size_t)j];
+  const SlBuf sb = sl_buffer(d, j, c, f);
+  const double iv = 1.0 / d.V0[j];
+  w[j] = d4{m.x / sb.beta * iv, m.y * iv, m.z * iv, m.w * iv};
+}
+
+// Pass 2: lambda_n and the theta-gradient partials of one step.
+template <bool COMPACT>
+__global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                                                     const double* __restrict__ X, const double* __restrict__ lam,
+                                                     const d4* __restrict__ w, const int32_t* __restrict__ ryr,
+                                                     double* __restrict__ lam_out, double* __restrict__ gpart) {
+  __shared__ double red[ADJ_BS / 64][ADJ_NT];
+  const RealDesc& d = D[0];
+  const RealLive& l = L[0];
+  const int j = blockIdx.x * ADJ_BS + threadIdx.x;
+  const size_t np = d.np;
+  double gr[ADJ_NT];
+#pragma unroll
+  for (int k = 0; k < ADJ_NT; ++k) gr[k] = 0.0;
+  if (j < d.n) {
+    const double* P = d.P;
+    const double dt = P[SRS_R_T_STEP];
+    const Geo g = load_geo<false, COMPACT, -1>(d, j);
+    const uint8_t f = g.f;
+    const d4* Q = (const d4*)X;
+    const d4 m0 = Q[j], m1 = Q[g.n1], m2 = Q[g.n2], m3 = Q[g.n3], m4 = Q[g.n4];
+    const d4 w0 = w[j], w1 = w[g.n1], w2 = w[g.n2], w3 = w[g.n3], w4 = w[g.n4];
+    const d4 mu = ((const d4*)lam)[j];
+    const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
+    const bool live_cs = (f & F_JSR) != 0;
+    const double bT = live_tr ? X[4 * np + j] : 0.0, bC = live_cs ? X[5 * np + j] : 0.0;
+    const double muT = lam[4 * np + j], muC = lam[5 * np + j];
+    const double c = m0.x, bA = m0.y, bM = m0.z, bD = m0.w;
+    const double V0 = d.V0[j];
+    const SlBuf sb = sl_buffer(d, j, c, f);
+    const double u = mu.x / sb.beta;
+
+    // transposed diffusion: dt D_s V0 (A w_s)_j
+    const double aw_c = g.a0 * w0.x + g.a1 * w1.x + g.a2 * w2.x + g.a3 * w3.x + g.a4 * w4.x;
+    const double aw_a = g.a0 * w0.y + g.a1 * w1.y + g.a2 * w2.y + g.a3 * w3.y + g.a4 * w4.y;
+    const double aw_m = g.a0 * w0.z + g.a1 * w1.z + g.a2 * w2.z + g.a3 * w3.z + g.a4 * w4.z;
+    const double aw_d = g.a0 * w0.w + g.a1 * w1.w + g.a2 * w2.w + g.a3 * w3.w + g.a4 * w4.w;
+    double lc = mu.x + dt * P[SRS_R_D_CA] * V0 * aw_c;
+    double la = mu.y + dt * P[SRS_R_D_ATP] * V0 * aw_a;
+    double lm = mu.z + dt * P[SRS_R_D_CMDN] * V0 * aw_m;
+    double ld = mu.w + dt * P[SRS_R_D_DYE] * V0 * aw_d;
+    double lT = muT, lC = muC;
+
+    // forward Ca rate before the SL division (for the SL terms) and the
+    // diffusion sums (for the D gradients)
+    const double ac = c * g.a0 + m1.x * g.a1 + m2.x * g.a2 + m3.x * g.a3 + m4.x * g.a4;
+    const double ad = bD * g.a0 + m1.w * g.a1 + m2.w * g.a2 + m3.w * g.a3 + m4.w * g.a4;
+    double rca = P[SRS_R_D_CA] * ac + ((f & F_BR) ? d.br[j] : 0.0);
+    gr[0] += dt * u * ac;          // D_CA
+    gr[1] += dt * mu.w * ad;       // D_DYE
+    if (!(f & F_JSR)) {
+      if (f & F_TROPC) {
+        const double kon = P[SRS_R_K_ON_TRPN], koff = P[SRS_R_K_OFF_TRPN], B = P[SRS_R_B_TOT_TRPN];
+        const double r = kon * (B - bT) * c - koff * bT;
+        rca -= r;
+        lc += dt * (muT - u) * kon * (B - bT);
+        lT += dt * (muT - u) * (-kon * c - koff);
+        gr[4] += dt * (muT - u) * (B - bT) * c;
+        gr[5] += dt * (muT - u) * (-bT);
+        // SERCA, cpu/RunSimulation.cpp:82-90
+        const double kdi = P[SRS_R_K_D_I];
+        double K = c / kdi;
+        K = K * K;
+        const double dK = 2.0 * (c / kdi) / kdi;
+        const double Nn = K * K * 3.24873e12 + K * l.serca[3] - l.serca[4];
+        const double Dn = l.serca[0] + K * l.serca[1] + K * K * l.serca[2];
+        const double v = Nn / Dn;
+        const double dv = ((2 * K * 3.24873e12 + l.serca[3]) * Dn - Nn * (l.serca[1] + 2 * K * l.serca[2])) / (Dn * Dn);
+        rca -= (2e-3) * v * P[SRS_R_A_P];
+        lc += dt * (-u) * (2e-3) * P[SRS_R_A_P] * dv * dK;
+        gr[11] += dt * u * (-(2e-3) * v);
+      }
+      const double konD = P[SRS_R_K_ON_DYE], koffD = P[SRS_R_K_OFF_DYE], BD = P[SRS_R_B_TOT_DYE];
+      const double rD = konD * (BD - bD) * c - koffD * bD;
+      rca -= rD;
+      lc += dt * (mu.w - u) * konD * (BD - bD);
+      ld += dt * (mu.w - u) * (-konD * c - koffD);
+      gr[8] += dt * (mu.w - u) * (BD - bD) * c;
+      gr[9] += dt * (mu.w - u) * (-bD);
+      const double konA = P[SRS_R_K_ON_ATP], koffA = P[SRS_R_K_OFF_ATP], BA = P[SRS_R_B_TOT_ATP];
+      const double rA = konA * (BA - bA) * c - koffA * bA;
+      rca -= rA;
+      lc += dt * (mu.y - u) * konA * (BA - bA);
+      la += dt * (mu.y - u) * (-konA * c - koffA);
+      gr[6] += dt * (mu.y - u) * (BA - bA) * c;
+      gr[7] += dt * (mu.y - u) * (-bA);
+      const double konM = P[SRS_R_K_ON_CMDN], koffM = P[SRS_R_K_OFF_CMDN], BM = P[SRS_R_B_TOT_CMDN];
+      const double rM = konM * (BM - bM) * c - koffM * bM;
+      rca -= rM;
+      lc += dt * (mu.z - u) * konM * (BM - bM);
+      lm += dt * (mu.z - u) * (-konM * c - koffM);
+      gr[2] += dt * (mu.z - u) * (BM - bM) * c;
+      gr[3] += dt * (mu.z - u) * (-bM);
+    } else {
+      const double kon = P[SRS_R_K_ON_CSQN], koff = P[SRS_R_K_OFF_CSQN], B = P[SRS_R_B_T_JSR];
+      const double r = kon * (B - bC) * c - koff * bC;
+      rca -= r;
+      rca += (l.ca_nsr - c) * P[SRS_R_V_REFILL];
+      lc += dt * ((muC - u) * kon * (B - bC) - u * P[SRS_R_V_REFILL]);
+      lC += dt * (muC - u) * (-kon * c - koff);
+    }
+    if (f & F_CHAN) {
+      int lo = 0, hi = d.n_celem;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (d.celem[mid] < j) lo = mid + 1; else hi = mid;
+      }
+      if (lo < d.n_celem && d.celem[lo] == j) {
+        const double a = P[SRS_R_V_RYR] * P[SRS_R_V_CELL];
+        for (int e = d.cstart[lo]; e < d.cstart[lo + 1]; ++e) {
+          const int ch = d.centry[e];
+          if (ch >= 0) {
+            const int i = ch >> 1;
+            if (!ryr[i]) continue;
+            const int ce = d.ryr_ele[i], je = d.ryr_jsr[i];
+            const int p = (ch & 1) ? ce : je;  // the partner element
+            const double cp = X[4 * (size_t)p];
+            const double up = lam[4 * (size_t)p] / sl_buffer(d, p, cp, d.flags[p]).beta;
+            const double dc = X[4 * (size_t)je] - X[4 * (size_t)ce];
+            const double J = a * dc;
+            if (ch & 1) {  // jSR side: r -= J / V0_jsr
+              rca -= J / V0;
+              lc += dt * (u * (-a / V0) + up * (a / d.V0[p]));
+            } else {       // cleft side: r += J / V0_cyt
+              rca += J / V0;
+              lc += dt * (u * (-a / V0) + up * (a / d.V0[p]));
+              gr[10] += dt * (u * P[SRS_R_V_CELL] * dc / V0 - up * P[SRS_R_V_CELL] * dc / d.V0[p]);
+            }
+          } else {
+            const int i = -ch - 1;
+            if (srs_lcc_open(d.lcc_state[i], d.lccv_state[i])) {
+              rca += srs_lcc_flux_raw(l.vm, c, P[SRS_R_CA_0]) / V0;
+              lc += dt * u * srs_lcc_flux_raw(l.vm, 1.0, 0.0) / V0;
+            }
+          }
+        }
+      }
+    }
+    if (f & F_SL) {
+      const double b2 = sb.beta * sb.beta;
+      lc += dt * mu.x * (-rca * sb.dbeta_dc / b2);
+      gr[12] += dt * mu.x * (-rca * sb.dbeta_dkh / b2);
+    }
+    ((d4*)lam_out)[j] = d4{lc, la, lm, ld};
+    lam_out[4 * np + j] = lT;
+    lam_out[5 * np + j] = lC;
+  }
+  // block reduction of the gradient partials, accumulated per block
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < ADJ_NT; ++k) {
+    double v = gr[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) red[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < ADJ_NT) {
+    double s = 0.0;
+    for (int q = 0; q < ADJ_BS / 64; ++q) s += red[q][threadIdx.x];
+    gpart[(size_t)blockIdx.x * ADJ_NT + threadIdx.x] += s;
+  }
+}
+
+// The loss gradient of one scan sample in the Fluo adjoint:
+// lambda_Fluo,j += V0_j sum_i gcoef_i w(c_j - p_i) over points within 1 um.
+__global__ __launch_bounds__(ADJ_BS) void k_scan_adj(const RealDesc* __restrict__ D, const double* __restrict__ cen,
+                                                     ScanGeo g, int n_f, const double* __restrict__ gcoef,
+                                                     double* __restrict__ lam) {
+  const RealDesc& d = D[0];
+  const int j = blockIdx.x * ADJ_BS + threadIdx.x;
+  if (j >= d.n) return;
+  const size_t np = d.np;
+  const double cx = cen[j], cy = cen[np + j], cz = cen[2 * np + j];
+  double s = 0.0;
+  for (int i = 0; i < n_f; ++i) {
+    double p[3] = {g.base[0], g.base[1], g.base[2]};
+    p[g.axis] = g.x_min + i * g.res;
+    const double rx = cx - p[0], ry = cy - p[1], rz = cz - p[2];
+    if (rx * rx + ry * ry + rz * rz < 1.0)
+      s += gcoef[i] * (g.alpha * exp(-0.5 * ((rx * rx / (g.sx * g.sx)) + (ry * ry / (g.sy * g.sy)) +
+                                             (rz * rz / (g.sz * g.sz)))) / (g.sx * g.sy * g.sz));
+  }
+  lam[4 * (size_t)j + 3] += d.V0[j] * s;
+}
+
+// Writes each step's RyR states (the ones the next step's fluxes use) into a
+// schedule, for srs_record_schedule.
+__global__ void k_sched_out(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int32_t* out,
+                            int64_t n_rows) {
+  const RealDesc& d = D[0];
+  const int64_t row = L[0].step;  // k_gate already advanced the step counter
+  if (row >= n_rows) return;
+  for (int i = threadIdx.x; i < d.n_ryr; i += blockDim.x) out[row * d.n_ryr + i] = d.ryr_state[i];
+}
+
 // Moves one realization's state between the single-realization layout
 // (np quads, TnC[np], CSQ[np]) and its slot of a group buffer.
 __global__ void k_slot_copy(const double* __restrict__ src, double* __restrict__ dst, size_t np, int K, int slot,
@@ -1018,6 +1260,20 @@ int launch_step(srs_ctx* c, uint32_t gmode) {
   return SRS_OK;
 }
 
+}  // namespace
+
+namespace {
+// Temporary device buffers freed when the call returns.
+struct DevMem {
+  std::vector<void*> ptrs;
+  ~DevMem() { for (void* p : ptrs) (void)hipFree(p); }
+  template <class T>
+  hipError_t alloc(T** p, size_t count) {
+    hipError_t e = hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) ptrs.push_back((void*)*p);
+    return e;
+  }
+};
 }  // namespace
 
 extern "C" {
@@ -1828,6 +2084,236 @@ int srs_scan_sample(srs_ctx* c, int32_t r, int32_t scan_id, double* ca, double* 
   if (ca) std::copy(out.begin(), out.begin() + S.n_ca, ca);
   if (caf) std::copy(out.begin() + S.n_ca, out.begin() + S.n_ca + S.n_f, caf);
   if (cafb) std::copy(out.begin() + S.n_ca + S.n_f, out.end(), cafb);
+  return SRS_OK;
+}
+
+// ----------------------------------------------------------------- adjoint
+
+int srs_adjoint_theta(int32_t* idx) {
+  srs::clear_error();
+  if (!idx) return srs::fail(SRS_E_ARG, "null output");
+  const int t[ADJ_NT] = {SRS_R_D_CA,     SRS_R_D_DYE,     SRS_R_K_ON_CMDN, SRS_R_K_OFF_CMDN, SRS_R_K_ON_TRPN,
+                         SRS_R_K_OFF_TRPN, SRS_R_K_ON_ATP, SRS_R_K_OFF_ATP, SRS_R_K_ON_DYE,  SRS_R_K_OFF_DYE,
+                         SRS_R_V_RYR,     SRS_R_A_P,       SRS_R_K_D_SL_HI};
+  for (int k = 0; k < ADJ_NT; ++k) idx[k] = t[k];
+  return SRS_N_THETA == ADJ_NT ? SRS_OK : srs::fail(SRS_E_STATE, "theta count mismatch");
+}
+
+int srs_params_set(srs_ctx* c, int32_t r, const double* reals) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  if (!reals) return srs::fail(SRS_E_ARG, "null parameters");
+  HostReal& hr = c->reals[r];
+  memcpy(hr.reals, reals, sizeof(hr.reals));
+  memcpy(hr.desc.P, reals, sizeof(hr.desc.P));
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->d_desc + r, &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice));
+  RealLive lv;
+  if ((rc = live_rw(c, r, &lv, false))) return rc;
+  serca_terms(lv.ca_nsr, hr.reals, lv.serca);
+  return live_rw(c, r, &lv, true);
+}
+
+static int single_real(srs_ctx* c, int32_t r) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  if (c->reals.size() != 1 || c->group != 1)
+    return srs::fail(SRS_E_ARG, "this call needs a context with one realization and group size 1");
+  return SRS_OK;
+}
+
+int srs_record_schedule(srs_ctx* c, int32_t r, int64_t n_steps, int32_t* schedule) {
+  int rc = single_real(c, r);
+  if (rc) return rc;
+  if (n_steps <= 0) return SRS_OK;
+  const int nr = (int)c->reals[r].ryr_ele.size();
+  if (nr == 0) return srs::fail(SRS_E_ARG, "realization has no RyRs");
+  if (!schedule) return srs::fail(SRS_E_ARG, "null schedule");
+  HIPCHK(hipSetDevice(c->device));
+  RealLive lv;
+  if ((rc = live_rw(c, r, &lv, false))) return rc;
+  int32_t* d_out = nullptr;
+  HIPCHK(hipMalloc(&d_out, (size_t)n_steps * nr * sizeof(int32_t)));
+  const RealDesc& d = c->reals[r].desc;
+  hipError_t e = hipMemcpy(d_out, d.ryr_state, nr * sizeof(int32_t), hipMemcpyDeviceToDevice);
+  for (int64_t k = 0; k < n_steps && e == hipSuccess; ++k) {
+    launch_step(c, G_AFTER_SWEEP);
+    hipLaunchKernelGGL(k_sched_out, dim3(1), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_out + 0,
+                       (int64_t)n_steps);
+    if ((k & 1023) == 1023) e = hipStreamSynchronize(c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(schedule, d_out, (size_t)n_steps * nr * sizeof(int32_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d_out);
+  if (e != hipSuccess) return srs::fail(SRS_E_HIP, "record schedule: %s", hipGetErrorString(e));
+  (void)lv;
+  return SRS_OK;
+}
+
+int srs_linescan_loss_grad(srs_ctx* c, int32_t r, int32_t scan_id, int64_t n_steps, const int32_t* schedule,
+                           int32_t n_samples, const int64_t* sample_steps, const double* target, int32_t ckpt,
+                           double* loss, double* grad) {
+  int rc = single_real(c, r);
+  if (rc) return rc;
+  if (scan_id < 0 || scan_id >= (int)c->scans.size()) return srs::fail(SRS_E_ARG, "bad scan id %d", scan_id);
+  const HostScan& S = c->scans[scan_id];
+  if (c->reals[r].mesh != S.mesh) return srs::fail(SRS_E_ARG, "realization is not on the scan's mesh");
+  if (n_steps <= 0 || n_samples <= 0 || !sample_steps || !target || !loss) return srs::fail(SRS_E_ARG, "bad arguments");
+  if (ckpt <= 0 || ckpt > 4096) return srs::fail(SRS_E_ARG, "checkpoint interval must be 1..4096");
+  for (int k = 0; k < n_samples; ++k)
+    if (sample_steps[k] < 1 || sample_steps[k] > n_steps || (k && sample_steps[k] <= sample_steps[k - 1]))
+      return srs::fail(SRS_E_ARG, "sample steps must increase within 1..n_steps");
+  HostReal& hr = c->reals[r];
+  const HostMesh& hm = *c->meshes[hr.mesh];
+  const int nr = (int)hr.ryr_ele.size();
+  if (nr > 0 && !schedule) return srs::fail(SRS_E_ARG, "null schedule");
+  const size_t np = hm.np, n6 = 6 * np;
+  const int nf = S.n_f, nca = S.n_ca, nout = nca + 2 * nf;
+  if (nf <= 0) return srs::fail(SRS_E_ARG, "the scan has no fluorescence points");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+
+  // saved state, restored at the end
+  RealLive lv0;
+  if ((rc = live_rw(c, r, &lv0, false))) return rc;
+  std::vector<int32_t> ryr0(nr);
+  if (nr) HIPCHK(hipMemcpy(ryr0.data(), hr.desc.ryr_state, nr * sizeof(int32_t), hipMemcpyDeviceToHost));
+
+  DevMem M;
+  const int64_t n_ck = (n_steps + ckpt - 1) / ckpt;
+  double *ck = nullptr, *samp = nullptr;
+  int32_t* d_sched = nullptr;
+  HIPCHK(M.alloc(&ck, (size_t)n_ck * n6));
+  HIPCHK(M.alloc(&samp, (size_t)n_samples * nout));
+  if (nr) {
+    HIPCHK(M.alloc(&d_sched, (size_t)n_steps * nr));
+    HIPCHK(hipMemcpy(d_sched, schedule, (size_t)n_steps * nr * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  RealDesc dsc = hr.desc;
+  dsc.sched = d_sched;
+  dsc.n_sched = n_steps;
+  HIPCHK(hipMemcpy(c->d_desc + r, &dsc, sizeof(dsc), hipMemcpyHostToDevice));
+  const uint32_t gmode = G_AFTER_SWEEP | G_NO_GATING | G_REPLAY;
+  auto set_live = [&](int64_t step, int cur) -> int {
+    RealLive lv = lv0;
+    lv.step = step;
+    lv.sched_base = 0;  // schedule row = step index
+    lv.cur = cur;
+    lv.done = 0;
+    lv.ffwd = 0;
+    HIPCHK(hipMemcpy(c->d_live + r, &lv, sizeof(lv), hipMemcpyHostToDevice));
+    if (nr) HIPCHK(hipMemcpy(hr.desc.ryr_state, d_sched + (size_t)step * nr, nr * sizeof(int32_t),
+                             hipMemcpyDeviceToDevice));
+    return SRS_OK;
+  };
+  auto scan_into = [&](double* out) {
+    hipLaunchKernelGGL(k_scan_blur, dim3(S.n_blocks), dim3(SCAN_BS), 0, c->stream, c->d_desc, c->d_live, r, S.d_cen,
+                       S.geo, nf, S.d_part);
+    hipLaunchKernelGGL(k_scan_final, dim3(1), dim3(256), 0, c->stream, c->d_desc, c->d_live, r, S.n_blocks, nf, nca,
+                       S.d_part, S.d_pad, S.d_near_f, S.d_near_ca, S.desc.f0, S.desc.c0, out);
+  };
+
+  // forward, with checkpoints and scan samples
+  const int cur0 = lv0.cur;
+  if ((rc = set_live(0, cur0))) return rc;
+  hipError_t e = hipSuccess;
+  int ks = 0;
+  for (int64_t n = 0; n < n_steps && e == hipSuccess; ++n) {
+    const int cur = cur0 ^ (int)(n & 1);
+    if (n % ckpt == 0)
+      e = hipMemcpyAsync(ck + (size_t)(n / ckpt) * n6, hr.desc.X[cur], n6 * sizeof(double), hipMemcpyDeviceToDevice,
+                         c->stream);
+    launch_step(c, gmode);
+    if (ks < n_samples && sample_steps[ks] == n + 1) scan_into(samp + (size_t)(ks++) * nout);
+    if (e == hipSuccess) e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return srs::fail(SRS_E_HIP, "adjoint forward: %s", hipGetErrorString(e));
+  std::vector<double> sh((size_t)n_samples * nout);
+  HIPCHK(hipMemcpy(sh.data(), samp, sh.size() * sizeof(double), hipMemcpyDeviceToHost));
+  // loss and its gradient in the blurred fluorescence
+  std::vector<double> gco((size_t)n_samples * nf);
+  double Ls = 0.0;
+  const double f0 = S.desc.f0;
+  for (int k = 0; k < n_samples; ++k)
+    for (int i = 0; i < nf; ++i) {
+      const double dff = sh[(size_t)k * nout + nca + nf + i] / f0 - 1.0;
+      const double res = dff - target[(size_t)k * nf + i];
+      Ls += res * res;
+      gco[(size_t)k * nf + i] = 2.0 * res / f0;
+    }
+  *loss = Ls;
+
+  if (grad) {
+    double *lam0, *lam1, *seg, *gpart, *dg;
+    d4* w;
+    const int nblk = (hm.n + ADJ_BS - 1) / ADJ_BS;
+    HIPCHK(M.alloc(&lam0, n6));
+    HIPCHK(M.alloc(&lam1, n6));
+    HIPCHK(M.alloc(&w, np));
+    HIPCHK(M.alloc(&seg, (size_t)(ckpt + 1) * n6));
+    HIPCHK(M.alloc(&gpart, (size_t)nblk * ADJ_NT));
+    HIPCHK(M.alloc(&dg, gco.size()));
+    HIPCHK(hipMemcpy(dg, gco.data(), gco.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(lam0, 0, n6 * sizeof(double)));
+    HIPCHK(hipMemset(lam1, 0, n6 * sizeof(double)));
+    HIPCHK(hipMemset(gpart, 0, (size_t)nblk * ADJ_NT * sizeof(double)));
+    double* lam = lam0;   // lambda_{n+1}
+    double* lout = lam1;  // lambda_n
+    ks = n_samples - 1;
+    for (int64_t s0 = (n_ck - 1) * ckpt; s0 >= 0 && e == hipSuccess; s0 -= ckpt) {
+      const int64_t s1 = std::min<int64_t>(n_steps, s0 + ckpt);
+      // recompute the segment from its checkpoint, keeping every state
+      const int curs = cur0 ^ (int)(s0 & 1);
+      if ((rc = set_live(s0, curs))) return rc;
+      e = hipMemcpyAsync(hr.desc.X[curs], ck + (size_t)(s0 / ckpt) * n6, n6 * sizeof(double), hipMemcpyDeviceToDevice,
+                         c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(seg, ck + (size_t)(s0 / ckpt) * n6, n6 * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream);
+      for (int64_t n = s0; n < s1 - 1 && e == hipSuccess; ++n) {
+        launch_step(c, gmode);
+        const int curn = cur0 ^ (int)((n + 1) & 1);
+        e = hipMemcpyAsync(seg + (size_t)(n + 1 - s0) * n6, hr.desc.X[curn], n6 * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->stream);
+      }
+      // backward through the segment
+      for (int64_t n = s1 - 1; n >= s0 && e == hipSuccess; --n) {
+        if (ks >= 0 && sample_steps[ks] == n + 1) {
+          hipLaunchKernelGGL(k_scan_adj, dim3(nblk), dim3(ADJ_BS), 0, c->stream, c->d_desc, S.d_cen, S.geo, nf,
+                             dg + (size_t)ks * nf, lam);
+          --ks;
+        }
+        const double* Xn = seg + (size_t)(n - s0) * n6;
+        const int32_t* ry = nr ? d_sched + (size_t)n * nr : nullptr;
+        hipLaunchKernelGGL(k_adj_w, dim3(nblk), dim3(ADJ_BS), 0, c->stream, c->d_desc, Xn, lam, w);
+        if (c->compact_all)
+          hipLaunchKernelGGL(k_adj_step<true>, dim3(nblk), dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, Xn, lam,
+                             w, ry, lout, gpart);
+        else
+          hipLaunchKernelGGL(k_adj_step<false>, dim3(nblk), dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, Xn,
+                             lam, w, ry, lout, gpart);
+        std::swap(lam, lout);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    if (e != hipSuccess) return srs::fail(SRS_E_HIP, "adjoint backward: %s", hipGetErrorString(e));
+    std::vector<double> gp((size_t)nblk * ADJ_NT);
+    HIPCHK(hipMemcpy(gp.data(), gpart, gp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < ADJ_NT; ++k) {
+      double s = 0.0;
+      for (int b = 0; b < nblk; ++b) s += gp[(size_t)b * ADJ_NT + k];
+      grad[k] = s;
+    }
+  }
+  // restore the realization as it was
+  HIPCHK(hipMemcpy(hr.desc.X[cur0], ck, n6 * sizeof(double), hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(c->d_desc + r, &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_live + r, &lv0, sizeof(lv0), hipMemcpyHostToDevice));
+  if (nr) HIPCHK(hipMemcpy(hr.desc.ryr_state, ryr0.data(), nr * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipDeviceSynchronize());
   return SRS_OK;
 }
 
