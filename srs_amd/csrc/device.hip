@@ -73,7 +73,6 @@ struct RealDesc {
   const double* tab;      // compact layout: [n_class][10] class row (5 values) and its ulps (5 values)
   int32_t n_class;
   const uint8_t* flags;   // [np]
-  const uint16_t* wsum;   // [np/64] per-wavefront flag summary: OR | AND << 8
   const double* V0;       // [np]
   const double* tt;       // [np]
   const double* br;       // [4][np] Ca, Fluo, ATP, CaM boundary rates, or null
@@ -385,9 +384,6 @@ __device__ __forceinline__ void sweep_group(const RealDesc* __restrict__ D, cons
   }
 }
 
-// The element's flags are loaded with its operator row; a wavefront whose
-// active lanes are all plain cytosol with troponin (the bulk of a mesh) then
-// runs the specialisation with the flags known at compile time.
 // Single-realization groups with LDS staging: the tile's own quads are
 // loaded coalesced into LDS, and neighbour quads inside the tile come from
 // there instead of through the L1 cache.
@@ -419,6 +415,9 @@ __device__ __forceinline__ void sweep_tile_lds(const RealDesc* __restrict__ D, c
   }
 }
 
+// The element's flags are loaded with its operator row; a wavefront whose
+// active lanes are all plain cytosol with troponin (the bulk of a mesh) then
+// runs the specialisation with the flags known at compile time.
 template <int BS, bool NTG, bool NTS, bool COMPACT, int KG>
 __device__ __forceinline__ void sweep_tile(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
                                            const int2* __restrict__ tiles, int n_tiles, const int* __restrict__ groups,
@@ -532,20 +531,52 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
   const int nr = d.n_ryr;
 
   if (gating) {
+    // Everything a channel's update reads is loaded before the first barrier
+    // (RyR i by thread i mod 256, LCC i by thread 255 - i mod 256, so RyRs
+    // and LCCs are updated side by side), leaving one memory round trip in
+    // front of the arithmetic.
+    constexpr int RPT = MAX_RYR / GATE_THREADS, LPT = MAX_LCC / GATE_THREADS;
+    double ca_c[RPT], ca_j[RPT], Rr[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const int i = tid + k * GATE_THREADS;
+      if (i < nr) {
+        const int so = d.ryr_state[i];
+        s_old[i] = so;
+        s_a[i] = so;
+        ca_c[k] = Ca[cq(d, d.ryr_ele[i])];
+        ca_j[k] = Ca[cq(d, d.ryr_jsr[i])];
+        Rr[k] = d.R[i];
+      }
+    }
+    int ls[LPT], lvs[LPT];
+    double ca_l[LPT], Rl[LPT];
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) {
+      const int i = (GATE_THREADS - 1 - tid) + k * GATE_THREADS;
+      if (i < d.n_lcc) {
+        ls[k] = d.lcc_state[i];
+        lvs[k] = d.lccv_state[i];
+        ca_l[k] = Ca[cq(d, d.lcc_ele[i])];
+        Rl[k] = d.R[nr + i];
+      }
+    }
+    __syncthreads();
     // RyRs: the reference updates them one after another (later RyRs see
     // earlier flips through the neighbour counts, cpu/RunSimulation.cpp:661-666).
     // Each sweep below recomputes every RyR with lower-index neighbours taken
     // from the previous sweep's result and higher-index ones from the old
     // states; the first sweep that changes nothing is the sequential result.
-    for (int i = tid; i < nr; i += GATE_THREADS) { s_old[i] = d.ryr_state[i]; s_a[i] = s_old[i]; }
-    __syncthreads();
     const bool coupled = P[SRS_R_RYR_A_STAR] != 0.0;
     int* prev = s_a;
     int* next = s_b;
     if (coupled) {
       for (int it = 0; it <= nr; ++it) {
         int changed = 0;
-        for (int i = tid; i < nr; i += GATE_THREADS) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+          const int i = tid + k * GATE_THREADS;
+          if (i >= nr) continue;
           int no = 0, ncl = 0;
           for (int q = 0; q < 4; ++q) {
             const int nb = d.ryr_nb[4 * i + q];
@@ -554,8 +585,8 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
               if (s) no++; else ncl++;
             }
           }
-          const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[cq(d, d.ryr_ele[i])], Ca[cq(d, d.ryr_jsr[i])], P);
-          const double Rn = d.R[i] + rate * dt;
+          const double rate = srs_ryr_rate(s_old[i], no, ncl, ca_c[k], ca_j[k], P);
+          const double Rn = Rr[k] + rate * dt;
           const int ns = (Rn > 0) ? (s_old[i] == 1 ? 0 : 1) : s_old[i];
           next[i] = ns;
           changed |= (ns != prev[i]);
@@ -567,17 +598,22 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
     }
     // Commit with neighbour states from the fixed point (for uncoupled RyRs
     // the neighbour counts do not enter the rate, A* = 0 gives X = 1).
-    for (int i = tid; i < nr; i += GATE_THREADS) {
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const int i = tid + k * GATE_THREADS;
+      if (i >= nr) continue;
       int no = 0, ncl = 0;
-      for (int q = 0; q < 4; ++q) {
-        const int nb = d.ryr_nb[4 * i + q];
-        if (nb >= 0) {
-          const int s = (nb < i) ? prev[nb] : s_old[nb];
-          if (s) no++; else ncl++;
+      if (coupled) {
+        for (int q = 0; q < 4; ++q) {
+          const int nb = d.ryr_nb[4 * i + q];
+          if (nb >= 0) {
+            const int s = (nb < i) ? prev[nb] : s_old[nb];
+            if (s) no++; else ncl++;
+          }
         }
       }
-      const double rate = srs_ryr_rate(s_old[i], no, ncl, Ca[cq(d, d.ryr_ele[i])], Ca[cq(d, d.ryr_jsr[i])], P);
-      double Rn = d.R[i] + rate * dt;
+      const double rate = srs_ryr_rate(s_old[i], no, ncl, ca_c[k], ca_j[k], P);
+      double Rn = Rr[k] + rate * dt;
       int ns = s_old[i];
       if (Rn > 0) {
         uint64_t o[4];
@@ -589,13 +625,16 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
       d.ryr_state[i] = ns;
     }
     // LCCs, cpu/RunSimulation.cpp:229-400
-    for (int i = tid; i < d.n_lcc; i += GATE_THREADS) {
-      const int s = d.lcc_state[i], vs = d.lccv_state[i];
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) {
+      const int i = (GATE_THREADS - 1 - tid) + k * GATE_THREADS;
+      if (i >= d.n_lcc) continue;
+      const int s = ls[k], vs = lvs[k];
       double tr[3];
       int32_t tg[3];
-      const double lr = srs_lcc_rates(s, vs, l.vm, Ca[cq(d, d.lcc_ele[i])], tr, tg);
+      const double lr = srs_lcc_rates(s, vs, l.vm, ca_l[k], tr, tg);
       const int ic = nr + i;
-      double Rn = d.R[ic] + (tr[0] + tr[1] + tr[2] + lr) * dt;
+      double Rn = Rl[k] + (tr[0] + tr[1] + tr[2] + lr) * dt;
       if (Rn > 0) {
         uint64_t o[4];
         srs_philox4x64_10((uint64_t)step, (uint64_t)ic, d.stream, SRS_PURPOSE_GATING, d.key, SRS_PHILOX_KEY1, o);
@@ -1047,8 +1086,6 @@ struct HostMesh {
   std::vector<uint16_t> cls;
   std::vector<double> tab;
   int32_t n_class = 0;
-  std::vector<uint16_t> wsum;
-  uint16_t* d_wsum = nullptr;
   int32_t *d_nbr = nullptr;
   double *d_coef = nullptr, *d_V0 = nullptr, *d_tt = nullptr, *d_br = nullptr, *d_tab = nullptr;
   uint8_t* d_flags = nullptr;
@@ -1484,15 +1521,6 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
   int rc;
   c->compact_all = !c->meshes.empty();
   for (auto& hmp : c->meshes) c->compact_all &= hmp->compact;
-  for (auto& hmp : c->meshes) {  // per-wavefront flag summaries (flags are final now)
-    HostMesh& hm = *hmp;
-    hm.wsum.assign(hm.np / 64, 0);
-    for (int w = 0; w < hm.np / 64; ++w) {
-      uint8_t o = 0, a = 0xFF;
-      for (int k = 64 * w; k < std::min(64 * w + 64, hm.n); ++k) { o |= hm.flags[k]; a &= hm.flags[k]; }
-      hm.wsum[w] = (uint16_t)(o | (a << 8));
-    }
-  }
   for (auto& hmp : c->meshes) {
     HostMesh& hm = *hmp;
     if (!c->compact_all) {
@@ -1505,7 +1533,6 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
       if ((rc = dev_upload(c, &hm.d_tab, hm.tab))) return rc;
     }
     if ((rc = dev_upload(c, &hm.d_flags, hm.flags))) return rc;
-    if ((rc = dev_upload(c, &hm.d_wsum, hm.wsum))) return rc;
     if ((rc = dev_upload(c, &hm.d_V0, hm.V0))) return rc;
     if ((rc = dev_upload(c, &hm.d_tt, hm.tt))) return rc;
     if (!hm.br.empty() && (rc = dev_upload(c, &hm.d_br, hm.br))) return rc;
@@ -1551,7 +1578,7 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
     HostMesh& hm = *c->meshes[hr.mesh];
     RealDesc d{};
     d.nbr = hm.d_nbr; d.coef = hm.d_coef; d.flags = hm.d_flags; d.V0 = hm.d_V0; d.tt = hm.d_tt; d.br = hm.d_br;
-    d.off = hm.d_off; d.cls = hm.d_cls; d.dq = hm.d_dq; d.tab = hm.d_tab; d.wsum = hm.d_wsum;
+    d.off = hm.d_off; d.cls = hm.d_cls; d.dq = hm.d_dq; d.tab = hm.d_tab;
     d.n_class = hm.n_class;
     d.n = hm.n; d.np = hm.np;
     d.X[0] = gx0[real_group[r]]; d.X[1] = gx1[real_group[r]];
