@@ -405,7 +405,11 @@ int srs_grid_read(const char* base, int no_flux_boundary, srs_grid** out, int32_
   for (int i = 0; i < n_ele; ++i) {
     if (!er.next(line)) return srs::fail(SRS_E_IO, "%s: truncated at element %d", er.path.c_str(), i);
     if (!parse_ints(line, 5, iv)) return srs::fail(SRS_E_IO, "%s: malformed line %d", er.path.c_str(), er.line);
-    for (int q = 0; q < 4; ++q) en[4 * (size_t)i + q] = (int32_t)iv[1 + q] - 1;
+    for (int q = 0; q < 4; ++q) {
+      if (iv[1 + q] < 1 || iv[1 + q] > n_nodes)
+        return srs::fail(SRS_E_IO, "%s: element %d node %ld out of range", er.path.c_str(), i, iv[1 + q]);
+      en[4 * (size_t)i + q] = (int32_t)iv[1 + q] - 1;
+    }
   }
 
   // channels, :597-655
