@@ -1038,10 +1038,10 @@ __global__ __launch_bounds__(ADJ_BS) void k_scan_adj(const RealDesc* __restrict_
 // Writes each step's RyR states (the ones the next step's fluxes use) into a
 // schedule, for srs_record_schedule.
 __global__ void k_sched_out(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int32_t* out,
-                            int64_t n_rows) {
+                            int64_t base, int64_t n_rows) {
   const RealDesc& d = D[0];
-  const int64_t row = L[0].step;  // k_gate already advanced the step counter
-  if (row >= n_rows) return;
+  const int64_t row = L[0].step - base;  // k_gate already advanced the step counter
+  if (row < 1 || row >= n_rows) return;
   for (int i = threadIdx.x; i < d.n_ryr; i += blockDim.x) out[row * d.n_ryr + i] = d.ryr_state[i];
 }
 
@@ -2160,21 +2160,21 @@ int srs_record_schedule(srs_ctx* c, int32_t r, int64_t n_steps, int32_t* schedul
   HIPCHK(hipSetDevice(c->device));
   RealLive lv;
   if ((rc = live_rw(c, r, &lv, false))) return rc;
+  if (lv.done) return srs::fail(SRS_E_ARG, "record schedule: the realization has finished its protocol");
   int32_t* d_out = nullptr;
   HIPCHK(hipMalloc(&d_out, (size_t)n_steps * nr * sizeof(int32_t)));
   const RealDesc& d = c->reals[r].desc;
   hipError_t e = hipMemcpy(d_out, d.ryr_state, nr * sizeof(int32_t), hipMemcpyDeviceToDevice);
   for (int64_t k = 0; k < n_steps && e == hipSuccess; ++k) {
     launch_step(c, G_AFTER_SWEEP);
-    hipLaunchKernelGGL(k_sched_out, dim3(1), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_out + 0,
-                       (int64_t)n_steps);
+    hipLaunchKernelGGL(k_sched_out, dim3(1), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_out,
+                       (int64_t)lv.step, (int64_t)n_steps);
     if ((k & 1023) == 1023) e = hipStreamSynchronize(c->stream);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipMemcpy(schedule, d_out, (size_t)n_steps * nr * sizeof(int32_t), hipMemcpyDeviceToHost);
   (void)hipFree(d_out);
   if (e != hipSuccess) return srs::fail(SRS_E_HIP, "record schedule: %s", hipGetErrorString(e));
-  (void)lv;
   return SRS_OK;
 }
 

@@ -158,3 +158,34 @@ def test_total_calcium_conserved_on_device():
         X = ctx.state_get(0)
     assert abs(total(X) - T0) <= 1e-12 * T0
     assert np.abs(X - pr.X0).max() > 1.0  # Ca did move
+
+
+def test_schedule_recorded_mid_run_follows_the_gated_trajectory():
+    """Recording after earlier steps: row 0 holds the RyR states at the call,
+    row k the states after k more gated steps (the step counter of the
+    realization does not start at zero), and a realization that has finished
+    its protocol is refused."""
+    from tests.helpers import fast_gating, gpu_context, initial_fields
+
+    pr = W.prepare(W.tiny_box_spec(nx=6, ny=6, nz=4, tt=True, ryr=3, lcc=1), W.spark_params())
+    p = fast_gating(pr.params.copy())
+    F = initial_fields(pr, seed=4)
+    n0, n = 37, 120
+    a = gpu_context([pr], [F], params=[p], seeds=[9])
+    b = gpu_context([pr], [F], params=[p], seeds=[9])
+    a.explicit_step(n0, S.STEP_PLAIN)
+    b.explicit_step(n0, S.STEP_PLAIN)
+    sched = a.record_schedule(0, n)
+    rows = [b.channels_get(0)[0]]
+    for _ in range(n - 1):
+        b.explicit_step(1, S.STEP_PLAIN)
+        rows.append(b.channels_get(0)[0])
+    rows = np.array(rows)
+    assert np.array_equal(sched, rows)
+    assert len({r.tobytes() for r in rows}) > 1  # the RyRs did move
+    ls = a.loop_get(0)
+    a.loop_set(0, t=ls["t"], dt_last=ls["dt_last"], step=ls["step"], done=1)
+    with pytest.raises(S.SrsError, match="finished"):
+        a.record_schedule(0, 4)
+    a.close()
+    b.close()
