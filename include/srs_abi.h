@@ -431,16 +431,40 @@ int srs_params_set(srs_ctx* ctx, int32_t real, const double reals[SRS_N_REAL]);
  * at the call). */
 int srs_record_schedule(srs_ctx* ctx, int32_t real, int64_t n_steps, int32_t* schedule);
 
+/* The same for every realization of a context, stepped together: each
+ * realization's n_steps x n_ryr rows, concatenated in realization order. */
+int srs_record_schedules(srs_ctx* ctx, int64_t n_steps, int32_t* schedules);
+
 /* From the realization's current state, n_steps steps with the RyR states of
  * each step taken from `schedule` and gating off (LCC states held), sampling
  * scan line `scan_id` after sample_steps[k] steps (increasing, 1..n_steps).
  * loss = sum over samples and fluorescence points of (cafb/f0 - 1 - target)^2.
  * If grad is not NULL, the adjoint computes d loss / d theta (srs_adjoint_theta
- * order) with checkpoints every `ckpt` steps and recomputation.  The
- * realization's state, channel states and loop state are restored. */
+ * order) with checkpoints every `ckpt` steps (1..65536) and recomputation.
+ * Needs realization groups of size 1; other realizations of the context are
+ * held meanwhile.  The realization's state, channel states and loop state are
+ * restored. */
 int srs_linescan_loss_grad(srs_ctx* ctx, int32_t real, int32_t scan_id, int64_t n_steps, const int32_t* schedule,
                            int32_t n_samples, const int64_t* sample_steps, const double* target, int32_t ckpt,
                            double* loss, double* grad);
+
+/* srs_linescan_loss_grad for the realizations real_ids[0..n_real), each with
+ * its own schedule, target and parameters, stepped together in sub-batches:
+ *   schedules  the members' n_steps x n_ryr(member) schedules, concatenated in
+ *              real_ids order;
+ *   targets    [n_real][n_samples][n_f];
+ *   ckpt       checkpoint interval, 0 = ceil(sqrt(n_steps));
+ *   max_batch  most members stepped at once, 0 = as many as device memory
+ *              holds (checkpoints, one segment of states and the adjoint take
+ *              about 48 B x elements x (n_steps/ckpt + ckpt + 2) each);
+ *   losses     [n_real];  grads [n_real][SRS_N_THETA] or NULL;
+ *   scans      NULL or [n_real][n_samples][n_f]: the simulated dF/F0.
+ * Results for a member equal srs_linescan_loss_grad on that member alone, bit
+ * for bit.  All members must be on the scan's mesh. */
+int srs_linescan_loss_grad_batch(srs_ctx* ctx, int32_t n_real, const int32_t* real_ids, int32_t scan_id,
+                                 int64_t n_steps, const int32_t* schedules, int32_t n_samples,
+                                 const int64_t* sample_steps, const double* targets, int32_t ckpt, int32_t max_batch,
+                                 double* losses, double* grads, double* scans);
 
 #ifdef __cplusplus
 }

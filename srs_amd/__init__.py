@@ -503,6 +503,17 @@ class Context:
         check(lib().srs_record_schedule(self._h, r, int(n_steps), _i32(out)))
         return out
 
+    def record_schedules(self, n_steps: int) -> list:
+        """record_schedule for every realization of the context at once."""
+        nrs = [self.info(r)[1] for r in range(self.n_real)]
+        flat = np.zeros(n_steps * sum(nrs), np.int32)
+        check(lib().srs_record_schedules(self._h, int(n_steps), _i32(flat)))
+        out, o = [], 0
+        for nr in nrs:
+            out.append(flat[o:o + n_steps * nr].reshape(n_steps, nr))
+            o += n_steps * nr
+        return out
+
     def loss_grad(self, r: int, scan_id: int, schedule: np.ndarray, sample_steps, target: np.ndarray, ckpt: int = 256,
                   grad: bool = True):
         """Line-scan loss of a replayed run and (optionally) its adjoint
@@ -516,6 +527,33 @@ class Context:
                                            ss.ctypes.data_as(C.POINTER(C.c_int64)), _f64(tg), int(ckpt),
                                            C.byref(loss), _f64(g) if grad else None))
         return (loss.value, g) if grad else loss.value
+
+    def loss_grad_batch(self, reals, scan_id: int, schedules, sample_steps, targets, ckpt: int = 0,
+                        max_batch: int = 0, grad: bool = True, scans: bool = False):
+        """loss_grad for several realizations stepped together: schedules[m]
+        is realization reals[m]'s (n_steps, n_ryr) schedule, targets[m] its
+        (n_samples, n_f) target.  Returns losses (n,), gradients (n, 13) if
+        grad, and the simulated dF/F0 (n, n_samples, n_f) if scans."""
+        ids = np.ascontiguousarray(reals, np.int32)
+        n = len(ids)
+        n_steps = int(np.shape(schedules[0])[0])
+        for sc in schedules:
+            if np.shape(sc)[0] != n_steps:
+                raise ValueError("every schedule needs the same number of steps")
+        sch = np.ascontiguousarray(np.concatenate([np.asarray(sc, np.int32).ravel() for sc in schedules]), np.int32)
+        ss = np.ascontiguousarray(sample_steps, np.int64)
+        tg = np.ascontiguousarray(targets, np.float64)
+        if tg.shape[:2] != (n, len(ss)):
+            raise ValueError("targets must be (n_real, n_samples, n_f)")
+        losses = np.zeros(n)
+        g = np.zeros((n, N_THETA)) if grad else None
+        sc = np.zeros(tg.shape) if scans else None
+        check(lib().srs_linescan_loss_grad_batch(self._h, n, _i32(ids), scan_id, n_steps, _i32(sch), len(ss),
+                                                 ss.ctypes.data_as(C.POINTER(C.c_int64)), _f64(tg), int(ckpt),
+                                                 int(max_batch), _f64(losses), _f64(g) if grad else None,
+                                                 _f64(sc) if scans else None))
+        out = (losses,) + ((g,) if grad else ()) + ((sc,) if scans else ())
+        return out if len(out) > 1 else losses
 
     def time_sweep(self, reps: int):
         a, b = C.c_double(), C.c_double()

@@ -742,10 +742,17 @@ constexpr int SCAN_BS = 256;
 constexpr int SCAN_MAX_POINTS = 1024;
 constexpr int SCAN_CELLS_PER_THREAD = 4;
 
+// Realization of workgroup row blockIdx.y in a batched launch: rid[blockIdx.y],
+// or r when there is no table (a launch for one realization).
+__device__ __forceinline__ int batch_real(const int32_t* __restrict__ rid, int r) { return rid ? rid[blockIdx.y] : r; }
+
 __global__ __launch_bounds__(SCAN_BS) void k_scan_blur(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
-                                                      int r, const double* __restrict__ cen, ScanGeo g, int n_f,
+                                                      int r, const int32_t* __restrict__ rid,
+                                                      const double* __restrict__ cen, ScanGeo g, int n_f,
                                                       double* __restrict__ part) {
   __shared__ double red[SCAN_BS / 64][SCAN_MAX_POINTS];
+  r = batch_real(rid, r);
+  part += (size_t)blockIdx.y * gridDim.x * n_f;
   const RealDesc& d = D[r];
   const double* X = d.X[L[r].cur];
   const int n = d.n, np = d.np;
@@ -785,10 +792,14 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_blur(const RealDesc* __restric
   }
 }
 
-__global__ void k_scan_final(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int r, int n_blocks,
-                             int n_f, int n_ca, const double* __restrict__ part, const double* __restrict__ pad,
+__global__ void k_scan_final(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int r,
+                             const int32_t* __restrict__ rid, int n_blocks, int n_f, int n_ca,
+                             const double* __restrict__ part, const double* __restrict__ pad,
                              const int32_t* __restrict__ near_f, const int32_t* __restrict__ near_ca, double f0,
-                             double c0, double* __restrict__ out) {
+                             double c0, double* __restrict__ out, int out_stride) {
+  r = batch_real(rid, r);
+  part += (size_t)blockIdx.y * n_blocks * n_f;
+  out += (size_t)blockIdx.y * out_stride;
   const double* X = D[r].X[L[r].cur];
   for (int i = threadIdx.x; i < n_f; i += blockDim.x) {
     double s = 0.0;
@@ -810,7 +821,9 @@ __global__ void k_scan_final(const RealDesc* __restrict__ D, const RealLive* __r
 // and dL/dtheta += dt (dr/dtheta)^T lambda_{n+1}.  The diffusion operator
 // satisfies V_i A_ij = V_j A_ji (both equal the face coefficient), so
 // (A^T y)_j = V_j (A (y / V))_j: the transpose is another gather over the same
-// rows.  Single-realization layout (quads, then the TnC and CSQ planes).
+// rows.  Single-realization layout (quads, then the TnC and CSQ planes); a
+// launch covers the realizations of a table, one workgroup row (blockIdx.y)
+// each, with row y's adjoint, state and weight arrays at y times their size.
 constexpr int ADJ_NT = 13;   // D_CA, D_DYE, kon/koff of CaM, TnC, ATP, Fluo, V_RYR, A_P, K_D_SL_HI
 constexpr int ADJ_BS = 256;
 __constant__ const int kAdjTheta[ADJ_NT] = {SRS_R_D_CA,     SRS_R_D_DYE,     SRS_R_K_ON_CMDN, SRS_R_K_OFF_CMDN,
@@ -837,11 +850,15 @@ __device__ __forceinline__ SlBuf sl_buffer(const RealDesc& d, int j, double c, u
 }
 
 // Pass 1: w = (lambda_Ca / beta, lambda_ATP, lambda_CaM, lambda_Fluo) / V0.
-__global__ __launch_bounds__(ADJ_BS) void k_adj_w(const RealDesc* __restrict__ D, const double* __restrict__ X,
-                                                  const double* __restrict__ lam, d4* __restrict__ w) {
-  const RealDesc& d = D[0];
+__global__ __launch_bounds__(ADJ_BS) void k_adj_w(const RealDesc* __restrict__ D, const int32_t* __restrict__ rid,
+                                                  const double* __restrict__ X, const double* __restrict__ lam,
+                                                  d4* __restrict__ w) {
+  const RealDesc& d = D[rid[blockIdx.y]];
   const int j = blockIdx.x * ADJ_BS + threadIdx.x;
   if (j >= d.n) return;
+  X += blockIdx.y * 6 * (size_t)d.np;
+  lam += blockIdx.y * 6 * (size_t)d.np;
+  w += blockIdx.y * (size_t)d.np;
   const uint8_t f = d.flags[j];
   const d4 m = ((const d4*)lam)[j];
   const double c = X[4 * (size_t)j];
@@ -853,14 +870,23 @@ __global__ __launch_bounds__(ADJ_BS) void k_adj_w(const RealDesc* __restrict__ D
 // Pass 2: lambda_n and the theta-gradient partials of one step.
 template <bool COMPACT>
 __global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
-                                                     const double* __restrict__ X, const double* __restrict__ lam,
-                                                     const d4* __restrict__ w, const int32_t* __restrict__ ryr,
-                                                     double* __restrict__ lam_out, double* __restrict__ gpart) {
+                                                     const int32_t* __restrict__ rid, const double* __restrict__ X,
+                                                     const double* __restrict__ lam, const d4* __restrict__ w,
+                                                     int64_t step, double* __restrict__ lam_out,
+                                                     double* __restrict__ gpart) {
   __shared__ double red[ADJ_BS / 64][ADJ_NT];
-  const RealDesc& d = D[0];
-  const RealLive& l = L[0];
+  const int r = rid[blockIdx.y];
+  const RealDesc& d = D[r];
+  const RealLive& l = L[r];
   const int j = blockIdx.x * ADJ_BS + threadIdx.x;
   const size_t np = d.np;
+  X += blockIdx.y * 6 * np;
+  lam += blockIdx.y * 6 * np;
+  lam_out += blockIdx.y * 6 * np;
+  w += blockIdx.y * np;
+  gpart += (size_t)blockIdx.y * gridDim.x * ADJ_NT;
+  // RyR states during this step: row `step` of the frozen schedule
+  const int32_t* ryr = d.n_ryr ? d.sched + step * d.n_ryr : nullptr;
   double gr[ADJ_NT];
 #pragma unroll
   for (int k = 0; k < ADJ_NT; ++k) gr[k] = 0.0;
@@ -1015,13 +1041,16 @@ __global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict_
 
 // The loss gradient of one scan sample in the Fluo adjoint:
 // lambda_Fluo,j += V0_j sum_i gcoef_i w(c_j - p_i) over points within 1 um.
-__global__ __launch_bounds__(ADJ_BS) void k_scan_adj(const RealDesc* __restrict__ D, const double* __restrict__ cen,
-                                                     ScanGeo g, int n_f, const double* __restrict__ gcoef,
+__global__ __launch_bounds__(ADJ_BS) void k_scan_adj(const RealDesc* __restrict__ D, const int32_t* __restrict__ rid,
+                                                     const double* __restrict__ cen, ScanGeo g, int n_f,
+                                                     const double* __restrict__ gcoef, int gstride,
                                                      double* __restrict__ lam) {
-  const RealDesc& d = D[0];
+  const RealDesc& d = D[rid[blockIdx.y]];
   const int j = blockIdx.x * ADJ_BS + threadIdx.x;
   if (j >= d.n) return;
   const size_t np = d.np;
+  gcoef += (size_t)blockIdx.y * gstride;
+  lam += blockIdx.y * 6 * np;
   const double cx = cen[j], cy = cen[np + j], cz = cen[2 * np + j];
   double s = 0.0;
   for (int i = 0; i < n_f; ++i) {
@@ -1035,13 +1064,44 @@ __global__ __launch_bounds__(ADJ_BS) void k_scan_adj(const RealDesc* __restrict_
   lam[4 * (size_t)j + 3] += d.V0[j] * s;
 }
 
+// Copies the current state (X[cur]) of each realization of a table to slot y
+// of a contiguous buffer (to_buf) or back.
+__global__ void k_batch_state(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                              const int32_t* __restrict__ rid, double* __restrict__ buf, int to_buf) {
+  const int r = rid[blockIdx.y];
+  const RealDesc& d = D[r];
+  const size_t n6 = 6 * (size_t)d.np;
+  double* X = d.X[L[r].cur];
+  double* B = buf + blockIdx.y * n6;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n6; i += (size_t)gridDim.x * blockDim.x) {
+    if (to_buf) B[i] = X[i];
+    else X[i] = B[i];
+  }
+}
+
+// Starts a replayed segment at `step` for each realization of a table: the
+// step counter, schedule rows counted from zero, RyR states of row `step`.
+__global__ void k_seg_start(const RealDesc* __restrict__ D, RealLive* __restrict__ L, const int32_t* __restrict__ rid,
+                            int64_t step) {
+  const int r = rid[blockIdx.x];
+  const RealDesc& d = D[r];
+  for (int i = threadIdx.x; i < d.n_ryr; i += blockDim.x) d.ryr_state[i] = d.sched[step * d.n_ryr + i];
+  if (threadIdx.x == 0) {
+    L[r].step = step;
+    L[r].sched_base = 0;
+  }
+}
+
 // Writes each step's RyR states (the ones the next step's fluxes use) into a
 // schedule, for srs_record_schedule.
-__global__ void k_sched_out(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int32_t* out,
-                            int64_t base, int64_t n_rows) {
-  const RealDesc& d = D[0];
-  const int64_t row = L[0].step - base;  // k_gate already advanced the step counter
+__global__ void k_sched_out(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                            const int64_t* __restrict__ base, const size_t* __restrict__ off, int32_t* out,
+                            int64_t n_rows) {
+  const int r = blockIdx.x;
+  const RealDesc& d = D[r];
+  const int64_t row = L[r].step - base[r];  // k_gate already advanced the step counter
   if (row < 1 || row >= n_rows) return;
+  out += off[r];
   for (int i = threadIdx.x; i < d.n_ryr; i += blockDim.x) out[row * d.n_ryr + i] = d.ryr_state[i];
 }
 
@@ -1180,6 +1240,7 @@ struct HostReal {
   int32_t ints[SRS_N_INT];
   double reals[SRS_N_REAL];
   uint64_t key = 0, stream = 0;
+  int32_t group = 0;  // realization group (index into the sweep's group table)
   RealDesc desc{};
 };
 
@@ -1582,6 +1643,7 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
     d.n_class = hm.n_class;
     d.n = hm.n; d.np = hm.np;
     d.X[0] = gx0[real_group[r]]; d.X[1] = gx1[real_group[r]];
+    hr.group = real_group[r];
     d.kst = KG;
     d.slot = real_slot[r];
     d.n_ryr = (int32_t)hr.ryr_ele.size();
@@ -2100,10 +2162,10 @@ int srs_scan_sample(srs_ctx* c, int32_t r, int32_t scan_id, double* ca, double* 
   if (c->reals[r].mesh != S.mesh) return srs::fail(SRS_E_ARG, "realization %d is not on the scan's mesh", r);
   HIPCHK(hipSetDevice(c->device));
   if (S.n_f > 0)
-    hipLaunchKernelGGL(k_scan_blur, dim3(S.n_blocks), dim3(SCAN_BS), 0, c->stream, c->d_desc, c->d_live, r, S.d_cen,
-                       S.geo, S.n_f, S.d_part);
-  hipLaunchKernelGGL(k_scan_final, dim3(1), dim3(256), 0, c->stream, c->d_desc, c->d_live, r, S.n_blocks, S.n_f,
-                     S.n_ca, S.d_part, S.d_pad, S.d_near_f, S.d_near_ca, S.desc.f0, S.desc.c0, S.d_out);
+    hipLaunchKernelGGL(k_scan_blur, dim3(S.n_blocks), dim3(SCAN_BS), 0, c->stream, c->d_desc, c->d_live, r, nullptr,
+                       S.d_cen, S.geo, S.n_f, S.d_part);
+  hipLaunchKernelGGL(k_scan_final, dim3(1), dim3(256), 0, c->stream, c->d_desc, c->d_live, r, nullptr, S.n_blocks,
+                     S.n_f, S.n_ca, S.d_part, S.d_pad, S.d_near_f, S.d_near_ca, S.desc.f0, S.desc.c0, S.d_out, 0);
   HIPCHK(hipGetLastError());
   std::vector<double> out((size_t)S.n_ca + 2 * (size_t)S.n_f);
   HIPCHK(hipMemcpyAsync(out.data(), S.d_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -2150,198 +2212,352 @@ static int single_real(srs_ctx* c, int32_t r) {
   return SRS_OK;
 }
 
+// n_steps gated steps of the whole context, recording every realization's RyR
+// states per step: realization r's n_steps x n_ryr rows at out + off[r].
+static int record_run(srs_ctx* c, int64_t n_steps, int32_t* out_host) {
+  const int n_real = (int)c->reals.size();
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<RealLive> lv(n_real);
+  HIPCHK(hipMemcpy(lv.data(), c->d_live, n_real * sizeof(RealLive), hipMemcpyDeviceToHost));
+  std::vector<int64_t> base(n_real);
+  std::vector<size_t> off(n_real + 1, 0);
+  for (int r = 0; r < n_real; ++r) {
+    if (lv[r].done) return srs::fail(SRS_E_ARG, "record schedule: realization %d has finished its protocol", r);
+    base[r] = lv[r].step;
+    off[r + 1] = off[r] + (size_t)n_steps * c->reals[r].ryr_ele.size();
+  }
+  DevMem M;
+  int32_t* d_out = nullptr;
+  int64_t* d_base = nullptr;
+  size_t* d_off = nullptr;
+  HIPCHK(M.alloc(&d_out, off[n_real]));
+  HIPCHK(M.alloc(&d_base, (size_t)n_real));
+  HIPCHK(M.alloc(&d_off, (size_t)n_real));
+  HIPCHK(hipMemcpy(d_base, base.data(), n_real * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_off, off.data(), n_real * sizeof(size_t), hipMemcpyHostToDevice));
+  for (int r = 0; r < n_real; ++r)
+    if (off[r + 1] > off[r])
+      HIPCHK(hipMemcpy(d_out + off[r], c->reals[r].desc.ryr_state, c->reals[r].ryr_ele.size() * sizeof(int32_t),
+                       hipMemcpyDeviceToDevice));
+  hipError_t e = hipSuccess;
+  for (int64_t k = 0; k < n_steps && e == hipSuccess; ++k) {
+    launch_step(c, G_AFTER_SWEEP);
+    hipLaunchKernelGGL(k_sched_out, dim3(n_real), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_base, d_off,
+                       d_out, n_steps);
+    e = hipGetLastError();
+    if (e == hipSuccess && (k & 1023) == 1023) e = hipStreamSynchronize(c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(out_host, d_out, off[n_real] * sizeof(int32_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return srs::fail(SRS_E_HIP, "record schedule: %s", hipGetErrorString(e));
+  return SRS_OK;
+}
+
 int srs_record_schedule(srs_ctx* c, int32_t r, int64_t n_steps, int32_t* schedule) {
   int rc = single_real(c, r);
   if (rc) return rc;
   if (n_steps <= 0) return SRS_OK;
-  const int nr = (int)c->reals[r].ryr_ele.size();
-  if (nr == 0) return srs::fail(SRS_E_ARG, "realization has no RyRs");
+  if (c->reals[r].ryr_ele.empty()) return srs::fail(SRS_E_ARG, "realization has no RyRs");
   if (!schedule) return srs::fail(SRS_E_ARG, "null schedule");
-  HIPCHK(hipSetDevice(c->device));
-  RealLive lv;
-  if ((rc = live_rw(c, r, &lv, false))) return rc;
-  if (lv.done) return srs::fail(SRS_E_ARG, "record schedule: the realization has finished its protocol");
-  int32_t* d_out = nullptr;
-  HIPCHK(hipMalloc(&d_out, (size_t)n_steps * nr * sizeof(int32_t)));
-  const RealDesc& d = c->reals[r].desc;
-  hipError_t e = hipMemcpy(d_out, d.ryr_state, nr * sizeof(int32_t), hipMemcpyDeviceToDevice);
-  for (int64_t k = 0; k < n_steps && e == hipSuccess; ++k) {
-    launch_step(c, G_AFTER_SWEEP);
-    hipLaunchKernelGGL(k_sched_out, dim3(1), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_out,
-                       (int64_t)lv.step, (int64_t)n_steps);
-    if ((k & 1023) == 1023) e = hipStreamSynchronize(c->stream);
+  return record_run(c, n_steps, schedule);
+}
+
+int srs_record_schedules(srs_ctx* c, int64_t n_steps, int32_t* schedules) {
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  if (!c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
+  if (n_steps <= 0) return SRS_OK;
+  if (!schedules) return srs::fail(SRS_E_ARG, "null schedules");
+  return record_run(c, n_steps, schedules);
+}
+
+// Line-scan loss and adjoint gradient of the realizations `ids` (all on the
+// scan's mesh, group size 1; arguments already checked).  Members run in
+// sub-batches of at most `max_batch` that step together; every other
+// realization is held (done) meanwhile, and all are left as they were.
+// Schedules are each member's n_steps x n_ryr rows, concatenated in member
+// order; targets are [member][sample][point].
+static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t scan_id, int64_t n_steps,
+                         const int32_t* schedules, int32_t n_samples, const int64_t* sample_steps,
+                         const double* targets, int32_t ckpt, int32_t max_batch, double* losses, double* grads,
+                         double* scans) {
+  const HostScan& S = c->scans[scan_id];
+  const HostMesh& hm = *c->meshes[S.mesh];
+  const size_t np = hm.np, n6 = 6 * np;
+  const int nf = S.n_f, nca = S.n_ca, nout = nca + 2 * nf;
+  const int n_mem = (int)ids.size(), n_real = (int)c->reals.size();
+  const bool want_grad = grads != nullptr;
+  const int64_t n_ck = (n_steps + ckpt - 1) / ckpt;
+  const int64_t n_ck_kept = want_grad ? n_ck : 1;  // a loss alone keeps only the start state
+  const int nblk = (hm.n + ADJ_BS - 1) / ADJ_BS;
+  std::vector<size_t> soff(n_mem + 1, 0);
+  size_t max_sched = 0;
+  for (int m = 0; m < n_mem; ++m) {
+    const size_t k = (size_t)n_steps * c->reals[ids[m]].ryr_ele.size();
+    soff[m + 1] = soff[m] + k;
+    max_sched = std::max(max_sched, k);
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e == hipSuccess) e = hipMemcpy(schedule, d_out, (size_t)n_steps * nr * sizeof(int32_t), hipMemcpyDeviceToHost);
-  (void)hipFree(d_out);
-  if (e != hipSuccess) return srs::fail(SRS_E_HIP, "record schedule: %s", hipGetErrorString(e));
+  // device bytes per member of a sub-batch
+  const size_t per = 8 * n6 * (size_t)(n_ck_kept + (want_grad ? ckpt + 2 : 0)) + (want_grad ? 32 * np : 0) +
+                     8 * (size_t)n_samples * nout + 8 * (size_t)S.n_blocks * nf + 8 * (size_t)nblk * ADJ_NT +
+                     8 * (size_t)n_samples * nf + 4 * max_sched + 64;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  size_t mem_free = 0, mem_total = 0;
+  HIPCHK(hipMemGetInfo(&mem_free, &mem_total));
+  const size_t budget = mem_free / 10 * 9;
+  int bs = max_batch > 0 ? std::min(max_batch, n_mem) : n_mem;
+  bs = (int)std::min<size_t>((size_t)bs, budget / per);
+  if (bs < 1)
+    return srs::fail(SRS_E_ARG, "checkpoint plan needs %.3g GB per realization, %.3g GB free; raise or lower the "
+                     "checkpoint interval (about sqrt(n_steps) needs least)", per / 1e9, mem_free / 1e9);
+
+  // everything restored at the end
+  std::vector<RealLive> live0(n_real);
+  HIPCHK(hipMemcpy(live0.data(), c->d_live, n_real * sizeof(RealLive), hipMemcpyDeviceToHost));
+  std::vector<std::vector<int32_t>> ryr0(n_mem);
+  for (int m = 0; m < n_mem; ++m) {
+    const HostReal& hr = c->reals[ids[m]];
+    ryr0[m].resize(hr.ryr_ele.size());
+    if (!ryr0[m].empty())
+      HIPCHK(hipMemcpy(ryr0[m].data(), hr.desc.ryr_state, ryr0[m].size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  }
+
+  DevMem M;
+  double *ck = nullptr, *samp = nullptr, *part = nullptr, *dg = nullptr, *gpart = nullptr;
+  double *lam0 = nullptr, *lam1 = nullptr, *seg = nullptr;
+  d4* w = nullptr;
+  int32_t *d_sched = nullptr, *d_rid = nullptr;
+  int2* d_sub_tiles = nullptr;
+  const int nt_mesh = (hm.n + c->tile - 1) / c->tile;
+  HIPCHK(M.alloc(&ck, (size_t)n_ck_kept * bs * n6));
+  HIPCHK(M.alloc(&d_sub_tiles, (size_t)bs * nt_mesh));
+  HIPCHK(M.alloc(&samp, (size_t)n_samples * bs * nout));
+  HIPCHK(M.alloc(&part, (size_t)bs * S.n_blocks * nf));
+  HIPCHK(M.alloc(&d_sched, std::max<size_t>(1, (size_t)bs * max_sched)));
+  HIPCHK(M.alloc(&d_rid, (size_t)bs));
+  if (want_grad) {
+    HIPCHK(M.alloc(&lam0, (size_t)bs * n6));
+    HIPCHK(M.alloc(&lam1, (size_t)bs * n6));
+    HIPCHK(M.alloc(&w, (size_t)bs * np));
+    HIPCHK(M.alloc(&seg, (size_t)ckpt * bs * n6));
+    HIPCHK(M.alloc(&gpart, (size_t)bs * nblk * ADJ_NT));
+    HIPCHK(M.alloc(&dg, (size_t)bs * n_samples * nf));
+  }
+  const uint32_t gmode = G_AFTER_SWEEP | G_NO_GATING | G_REPLAY;
+  const unsigned cp_blocks = (unsigned)std::min<size_t>((n6 + 255) / 256, 2048);
+  std::vector<double> sh, gco, gp;
+  std::vector<RealLive> lv(n_real);
+  std::vector<int2> sub_tiles;
+  // the sweep visits only the members' tiles (a held realization's tiles would
+  // still cost a dispatch each); the context's own list is put back on every
+  // return, before the buffers above are freed
+  struct TileRestore {
+    srs_ctx* c;
+    int2* tiles;
+    int n_tiles, grid;
+    ~TileRestore() { c->d_tiles = tiles; c->n_tiles = n_tiles; c->grid_sweep = grid; }
+  } tile_restore{c, c->d_tiles, c->n_tiles, c->grid_sweep};
+  hipError_t e = hipSuccess;
+  const int rc = [&]() -> int {
+  for (int m0 = 0; m0 < n_mem; m0 += bs) {
+    const int nb = std::min(bs, n_mem - m0);
+    const dim3 g_adj(nblk, nb), g_cp(cp_blocks, nb);
+    if (n_real > 1) {
+      sub_tiles.clear();
+      for (int y = 0; y < nb; ++y)
+        for (int t = 0; t < nt_mesh; ++t) sub_tiles.push_back(make_int2(c->reals[ids[m0 + y]].group, t));
+      HIPCHK(hipMemcpy(d_sub_tiles, sub_tiles.data(), sub_tiles.size() * sizeof(int2), hipMemcpyHostToDevice));
+      c->d_tiles = d_sub_tiles;
+      c->n_tiles = (int)sub_tiles.size();
+      c->grid_sweep = (c->n_tiles + NXCD - 1) / NXCD * NXCD;
+    }
+    // members replay their schedules from row 0; the rest are held
+    for (int q = 0; q < n_real; ++q) {
+      lv[q] = live0[q];
+      lv[q].done = 1;
+    }
+    for (int y = 0; y < nb; ++y) {
+      const int r = ids[m0 + y];
+      lv[r].done = 0;
+      lv[r].ffwd = 0;
+      lv[r].step = 0;
+      lv[r].sched_base = 0;
+      HostReal& hr = c->reals[r];
+      RealDesc dsc = hr.desc;
+      dsc.sched = d_sched + (size_t)y * max_sched;
+      dsc.n_sched = n_steps;
+      HIPCHK(hipMemcpy(d_sched + (size_t)y * max_sched, schedules + soff[m0 + y],
+                       (soff[m0 + y + 1] - soff[m0 + y]) * sizeof(int32_t), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->d_desc + r, &dsc, sizeof(dsc), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(c->d_live, lv.data(), n_real * sizeof(RealLive), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_rid, ids.data() + m0, nb * sizeof(int32_t), hipMemcpyHostToDevice));
+    auto scan_into = [&](double* out) {
+      hipLaunchKernelGGL(k_scan_blur, dim3(S.n_blocks, nb), dim3(SCAN_BS), 0, c->stream, c->d_desc, c->d_live, 0,
+                         d_rid, S.d_cen, S.geo, nf, part);
+      hipLaunchKernelGGL(k_scan_final, dim3(1, nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, 0, d_rid,
+                         S.n_blocks, nf, nca, part, S.d_pad, S.d_near_f, S.d_near_ca, S.desc.f0, S.desc.c0, out, nout);
+    };
+    auto ck_at = [&](int64_t k) { return ck + (size_t)k * nb * n6; };
+
+    // forward, with checkpoints and scan samples
+    hipLaunchKernelGGL(k_seg_start, dim3(nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, (int64_t)0);
+    int ks = 0;
+    for (int64_t n = 0; n < n_steps && e == hipSuccess; ++n) {
+      if (n % ckpt == 0 && n / ckpt < n_ck_kept)
+        hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid,
+                           ck_at(n / ckpt), 1);
+      launch_step(c, gmode);
+      if (ks < n_samples && sample_steps[ks] == n + 1) scan_into(samp + (size_t)(ks++) * nb * nout);
+      e = hipGetLastError();
+      if (e == hipSuccess && (n & 4095) == 4095) e = hipStreamSynchronize(c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return srs::fail(SRS_E_HIP, "adjoint forward: %s", hipGetErrorString(e));
+    sh.resize((size_t)n_samples * nb * nout);
+    if ((e = hipMemcpy(sh.data(), samp, sh.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+      return srs::fail(SRS_E_HIP, "adjoint forward: %s", hipGetErrorString(e));
+    // losses and their gradients in the blurred fluorescence
+    gco.assign((size_t)nb * n_samples * nf, 0.0);
+    const double f0 = S.desc.f0;
+    for (int y = 0; y < nb; ++y) {
+      const double* tg = targets + (size_t)(m0 + y) * n_samples * nf;
+      double Ls = 0.0;
+      for (int k = 0; k < n_samples; ++k)
+        for (int i = 0; i < nf; ++i) {
+          const double dff = sh[((size_t)k * nb + y) * nout + nca + nf + i] / f0 - 1.0;
+          const double res = dff - tg[(size_t)k * nf + i];
+          Ls += res * res;
+          gco[((size_t)y * n_samples + k) * nf + i] = 2.0 * res / f0;
+          if (scans) scans[((size_t)(m0 + y) * n_samples + k) * nf + i] = dff;
+        }
+      losses[m0 + y] = Ls;
+    }
+
+    if (want_grad) {
+      e = hipMemcpy(dg, gco.data(), gco.size() * sizeof(double), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemsetAsync(lam0, 0, (size_t)nb * n6 * sizeof(double), c->stream);
+      if (e == hipSuccess) e = hipMemsetAsync(gpart, 0, (size_t)nb * nblk * ADJ_NT * sizeof(double), c->stream);
+      double* lam = lam0;   // lambda_{n+1}
+      double* lout = lam1;  // lambda_n
+      ks = n_samples - 1;
+      for (int64_t s0 = (n_ck - 1) * ckpt; s0 >= 0 && e == hipSuccess; s0 -= ckpt) {
+        const int64_t s1 = std::min<int64_t>(n_steps, s0 + ckpt);
+        // recompute the segment from its checkpoint, keeping every state
+        hipLaunchKernelGGL(k_seg_start, dim3(nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, s0);
+        hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid,
+                           ck_at(s0 / ckpt), 0);
+        e = hipMemcpyAsync(seg, ck_at(s0 / ckpt), (size_t)nb * n6 * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream);
+        for (int64_t n = s0; n < s1 - 1 && e == hipSuccess; ++n) {
+          launch_step(c, gmode);
+          hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid,
+                             seg + (size_t)(n + 1 - s0) * nb * n6, 1);
+          e = hipGetLastError();
+        }
+        // backward through the segment
+        for (int64_t n = s1 - 1; n >= s0 && e == hipSuccess; --n) {
+          if (ks >= 0 && sample_steps[ks] == n + 1) {
+            hipLaunchKernelGGL(k_scan_adj, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, d_rid, S.d_cen, S.geo, nf,
+                               dg + (size_t)ks * nf, n_samples * nf, lam);
+            --ks;
+          }
+          const double* Xn = seg + (size_t)(n - s0) * nb * n6;
+          hipLaunchKernelGGL(k_adj_w, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, d_rid, Xn, lam, w);
+          if (c->compact_all)
+            hipLaunchKernelGGL(k_adj_step<true>, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, d_rid, Xn,
+                               lam, w, n, lout, gpart);
+          else
+            hipLaunchKernelGGL(k_adj_step<false>, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, d_rid, Xn,
+                               lam, w, n, lout, gpart);
+          std::swap(lam, lout);
+          e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      }
+      if (e == hipSuccess) {
+        gp.resize((size_t)nb * nblk * ADJ_NT);
+        e = hipMemcpy(gp.data(), gpart, gp.size() * sizeof(double), hipMemcpyDeviceToHost);
+      }
+      if (e != hipSuccess) return srs::fail(SRS_E_HIP, "adjoint backward: %s", hipGetErrorString(e));
+      for (int y = 0; y < nb; ++y)
+        for (int k = 0; k < ADJ_NT; ++k) {
+          double sum = 0.0;
+          for (int b = 0; b < nblk; ++b) sum += gp[((size_t)y * nblk + b) * ADJ_NT + k];
+          grads[(size_t)(m0 + y) * ADJ_NT + k] = sum;
+        }
+    }
+    // the members' states as they were: live first (it names the current
+    // buffer), then the start state into that buffer
+    HIPCHK(hipMemcpy(c->d_live, live0.data(), n_real * sizeof(RealLive), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, ck_at(0), 0);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return SRS_OK;
+  }();
+  // restore descriptors, live state and RyR states (also after a failure)
+  for (int m = 0; m < n_mem; ++m) {
+    const HostReal& hr = c->reals[ids[m]];
+    (void)hipMemcpy(c->d_desc + ids[m], &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice);
+    if (!ryr0[m].empty())
+      (void)hipMemcpy(hr.desc.ryr_state, ryr0[m].data(), ryr0[m].size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  }
+  (void)hipMemcpy(c->d_live, live0.data(), n_real * sizeof(RealLive), hipMemcpyHostToDevice);
+  e = hipDeviceSynchronize();
+  if (rc == SRS_OK && e != hipSuccess) return srs::fail(SRS_E_HIP, "adjoint restore: %s", hipGetErrorString(e));
+  return rc;
+}
+
+static int loss_grad_check(srs_ctx* c, int32_t scan_id, int64_t n_steps, int32_t n_samples,
+                           const int64_t* sample_steps, const double* target, int32_t ckpt, const void* loss) {
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  if (!c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
+  if (c->group != 1) return srs::fail(SRS_E_ARG, "the adjoint needs realization groups of size 1");
+  if (scan_id < 0 || scan_id >= (int)c->scans.size()) return srs::fail(SRS_E_ARG, "bad scan id %d", scan_id);
+  if (c->scans[scan_id].n_f <= 0) return srs::fail(SRS_E_ARG, "the scan has no fluorescence points");
+  if (n_steps <= 0 || n_samples <= 0 || !sample_steps || !target || !loss) return srs::fail(SRS_E_ARG, "bad arguments");
+  if (ckpt <= 0 || ckpt > 65536) return srs::fail(SRS_E_ARG, "checkpoint interval must be 1..65536");
+  for (int k = 0; k < n_samples; ++k)
+    if (sample_steps[k] < 1 || sample_steps[k] > n_steps || (k && sample_steps[k] <= sample_steps[k - 1]))
+      return srs::fail(SRS_E_ARG, "sample steps must increase within 1..n_steps");
   return SRS_OK;
 }
 
 int srs_linescan_loss_grad(srs_ctx* c, int32_t r, int32_t scan_id, int64_t n_steps, const int32_t* schedule,
                            int32_t n_samples, const int64_t* sample_steps, const double* target, int32_t ckpt,
                            double* loss, double* grad) {
-  int rc = single_real(c, r);
+  int rc = check_real(c, r);
   if (rc) return rc;
-  if (scan_id < 0 || scan_id >= (int)c->scans.size()) return srs::fail(SRS_E_ARG, "bad scan id %d", scan_id);
-  const HostScan& S = c->scans[scan_id];
-  if (c->reals[r].mesh != S.mesh) return srs::fail(SRS_E_ARG, "realization is not on the scan's mesh");
-  if (n_steps <= 0 || n_samples <= 0 || !sample_steps || !target || !loss) return srs::fail(SRS_E_ARG, "bad arguments");
-  if (ckpt <= 0 || ckpt > 4096) return srs::fail(SRS_E_ARG, "checkpoint interval must be 1..4096");
-  for (int k = 0; k < n_samples; ++k)
-    if (sample_steps[k] < 1 || sample_steps[k] > n_steps || (k && sample_steps[k] <= sample_steps[k - 1]))
-      return srs::fail(SRS_E_ARG, "sample steps must increase within 1..n_steps");
-  HostReal& hr = c->reals[r];
-  const HostMesh& hm = *c->meshes[hr.mesh];
-  const int nr = (int)hr.ryr_ele.size();
-  if (nr > 0 && !schedule) return srs::fail(SRS_E_ARG, "null schedule");
-  const size_t np = hm.np, n6 = 6 * np;
-  const int nf = S.n_f, nca = S.n_ca, nout = nca + 2 * nf;
-  if (nf <= 0) return srs::fail(SRS_E_ARG, "the scan has no fluorescence points");
-  HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if ((rc = loss_grad_check(c, scan_id, n_steps, n_samples, sample_steps, target, ckpt, loss))) return rc;
+  if (c->reals[r].mesh != c->scans[scan_id].mesh) return srs::fail(SRS_E_ARG, "realization is not on the scan's mesh");
+  if (!c->reals[r].ryr_ele.empty() && !schedule) return srs::fail(SRS_E_ARG, "null schedule");
+  return loss_grad_run(c, {r}, scan_id, n_steps, schedule, n_samples, sample_steps, target, ckpt, 1, loss, grad,
+                       nullptr);
+}
 
-  // saved state, restored at the end
-  RealLive lv0;
-  if ((rc = live_rw(c, r, &lv0, false))) return rc;
-  std::vector<int32_t> ryr0(nr);
-  if (nr) HIPCHK(hipMemcpy(ryr0.data(), hr.desc.ryr_state, nr * sizeof(int32_t), hipMemcpyDeviceToHost));
-
-  DevMem M;
-  const int64_t n_ck = (n_steps + ckpt - 1) / ckpt;
-  double *ck = nullptr, *samp = nullptr;
-  int32_t* d_sched = nullptr;
-  HIPCHK(M.alloc(&ck, (size_t)n_ck * n6));
-  HIPCHK(M.alloc(&samp, (size_t)n_samples * nout));
-  if (nr) {
-    HIPCHK(M.alloc(&d_sched, (size_t)n_steps * nr));
-    HIPCHK(hipMemcpy(d_sched, schedule, (size_t)n_steps * nr * sizeof(int32_t), hipMemcpyHostToDevice));
+int srs_linescan_loss_grad_batch(srs_ctx* c, int32_t n_real, const int32_t* real_ids, int32_t scan_id,
+                                 int64_t n_steps, const int32_t* schedules, int32_t n_samples,
+                                 const int64_t* sample_steps, const double* targets, int32_t ckpt, int32_t max_batch,
+                                 double* losses, double* grads, double* scans) {
+  int rc = c ? SRS_OK : srs::fail(SRS_E_ARG, "null context");
+  if (rc) return rc;
+  if (n_real <= 0 || !real_ids) return srs::fail(SRS_E_ARG, "no realizations");
+  if (ckpt == 0) ckpt = (int32_t)std::max<double>(1.0, std::ceil(std::sqrt((double)std::max<int64_t>(n_steps, 1))));
+  if ((rc = loss_grad_check(c, scan_id, n_steps, n_samples, sample_steps, targets, ckpt, losses))) return rc;
+  std::vector<int32_t> ids(real_ids, real_ids + n_real);
+  std::vector<char> seen(c->reals.size(), 0);
+  bool any_ryr = false;
+  for (int32_t r : ids) {
+    if ((rc = check_real(c, r))) return rc;
+    if (seen[r]++) return srs::fail(SRS_E_ARG, "realization %d listed twice", r);
+    if (c->reals[r].mesh != c->scans[scan_id].mesh)
+      return srs::fail(SRS_E_ARG, "realization %d is not on the scan's mesh", r);
+    any_ryr |= !c->reals[r].ryr_ele.empty();
   }
-  RealDesc dsc = hr.desc;
-  dsc.sched = d_sched;
-  dsc.n_sched = n_steps;
-  HIPCHK(hipMemcpy(c->d_desc + r, &dsc, sizeof(dsc), hipMemcpyHostToDevice));
-  const uint32_t gmode = G_AFTER_SWEEP | G_NO_GATING | G_REPLAY;
-  auto set_live = [&](int64_t step, int cur) -> int {
-    RealLive lv = lv0;
-    lv.step = step;
-    lv.sched_base = 0;  // schedule row = step index
-    lv.cur = cur;
-    lv.done = 0;
-    lv.ffwd = 0;
-    HIPCHK(hipMemcpy(c->d_live + r, &lv, sizeof(lv), hipMemcpyHostToDevice));
-    if (nr) HIPCHK(hipMemcpy(hr.desc.ryr_state, d_sched + (size_t)step * nr, nr * sizeof(int32_t),
-                             hipMemcpyDeviceToDevice));
-    return SRS_OK;
-  };
-  auto scan_into = [&](double* out) {
-    hipLaunchKernelGGL(k_scan_blur, dim3(S.n_blocks), dim3(SCAN_BS), 0, c->stream, c->d_desc, c->d_live, r, S.d_cen,
-                       S.geo, nf, S.d_part);
-    hipLaunchKernelGGL(k_scan_final, dim3(1), dim3(256), 0, c->stream, c->d_desc, c->d_live, r, S.n_blocks, nf, nca,
-                       S.d_part, S.d_pad, S.d_near_f, S.d_near_ca, S.desc.f0, S.desc.c0, out);
-  };
-
-  // forward, with checkpoints and scan samples
-  const int cur0 = lv0.cur;
-  if ((rc = set_live(0, cur0))) return rc;
-  hipError_t e = hipSuccess;
-  int ks = 0;
-  for (int64_t n = 0; n < n_steps && e == hipSuccess; ++n) {
-    const int cur = cur0 ^ (int)(n & 1);
-    if (n % ckpt == 0)
-      e = hipMemcpyAsync(ck + (size_t)(n / ckpt) * n6, hr.desc.X[cur], n6 * sizeof(double), hipMemcpyDeviceToDevice,
-                         c->stream);
-    launch_step(c, gmode);
-    if (ks < n_samples && sample_steps[ks] == n + 1) scan_into(samp + (size_t)(ks++) * nout);
-    if (e == hipSuccess) e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) return srs::fail(SRS_E_HIP, "adjoint forward: %s", hipGetErrorString(e));
-  std::vector<double> sh((size_t)n_samples * nout);
-  HIPCHK(hipMemcpy(sh.data(), samp, sh.size() * sizeof(double), hipMemcpyDeviceToHost));
-  // loss and its gradient in the blurred fluorescence
-  std::vector<double> gco((size_t)n_samples * nf);
-  double Ls = 0.0;
-  const double f0 = S.desc.f0;
-  for (int k = 0; k < n_samples; ++k)
-    for (int i = 0; i < nf; ++i) {
-      const double dff = sh[(size_t)k * nout + nca + nf + i] / f0 - 1.0;
-      const double res = dff - target[(size_t)k * nf + i];
-      Ls += res * res;
-      gco[(size_t)k * nf + i] = 2.0 * res / f0;
-    }
-  *loss = Ls;
-
-  if (grad) {
-    double *lam0, *lam1, *seg, *gpart, *dg;
-    d4* w;
-    const int nblk = (hm.n + ADJ_BS - 1) / ADJ_BS;
-    HIPCHK(M.alloc(&lam0, n6));
-    HIPCHK(M.alloc(&lam1, n6));
-    HIPCHK(M.alloc(&w, np));
-    HIPCHK(M.alloc(&seg, (size_t)(ckpt + 1) * n6));
-    HIPCHK(M.alloc(&gpart, (size_t)nblk * ADJ_NT));
-    HIPCHK(M.alloc(&dg, gco.size()));
-    HIPCHK(hipMemcpy(dg, gco.data(), gco.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(lam0, 0, n6 * sizeof(double)));
-    HIPCHK(hipMemset(lam1, 0, n6 * sizeof(double)));
-    HIPCHK(hipMemset(gpart, 0, (size_t)nblk * ADJ_NT * sizeof(double)));
-    double* lam = lam0;   // lambda_{n+1}
-    double* lout = lam1;  // lambda_n
-    ks = n_samples - 1;
-    for (int64_t s0 = (n_ck - 1) * ckpt; s0 >= 0 && e == hipSuccess; s0 -= ckpt) {
-      const int64_t s1 = std::min<int64_t>(n_steps, s0 + ckpt);
-      // recompute the segment from its checkpoint, keeping every state
-      const int curs = cur0 ^ (int)(s0 & 1);
-      if ((rc = set_live(s0, curs))) return rc;
-      e = hipMemcpyAsync(hr.desc.X[curs], ck + (size_t)(s0 / ckpt) * n6, n6 * sizeof(double), hipMemcpyDeviceToDevice,
-                         c->stream);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(seg, ck + (size_t)(s0 / ckpt) * n6, n6 * sizeof(double), hipMemcpyDeviceToDevice,
-                           c->stream);
-      for (int64_t n = s0; n < s1 - 1 && e == hipSuccess; ++n) {
-        launch_step(c, gmode);
-        const int curn = cur0 ^ (int)((n + 1) & 1);
-        e = hipMemcpyAsync(seg + (size_t)(n + 1 - s0) * n6, hr.desc.X[curn], n6 * sizeof(double),
-                           hipMemcpyDeviceToDevice, c->stream);
-      }
-      // backward through the segment
-      for (int64_t n = s1 - 1; n >= s0 && e == hipSuccess; --n) {
-        if (ks >= 0 && sample_steps[ks] == n + 1) {
-          hipLaunchKernelGGL(k_scan_adj, dim3(nblk), dim3(ADJ_BS), 0, c->stream, c->d_desc, S.d_cen, S.geo, nf,
-                             dg + (size_t)ks * nf, lam);
-          --ks;
-        }
-        const double* Xn = seg + (size_t)(n - s0) * n6;
-        const int32_t* ry = nr ? d_sched + (size_t)n * nr : nullptr;
-        hipLaunchKernelGGL(k_adj_w, dim3(nblk), dim3(ADJ_BS), 0, c->stream, c->d_desc, Xn, lam, w);
-        if (c->compact_all)
-          hipLaunchKernelGGL(k_adj_step<true>, dim3(nblk), dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, Xn, lam,
-                             w, ry, lout, gpart);
-        else
-          hipLaunchKernelGGL(k_adj_step<false>, dim3(nblk), dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, Xn,
-                             lam, w, ry, lout, gpart);
-        std::swap(lam, lout);
-        e = hipGetLastError();
-      }
-      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    }
-    if (e != hipSuccess) return srs::fail(SRS_E_HIP, "adjoint backward: %s", hipGetErrorString(e));
-    std::vector<double> gp((size_t)nblk * ADJ_NT);
-    HIPCHK(hipMemcpy(gp.data(), gpart, gp.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int k = 0; k < ADJ_NT; ++k) {
-      double s = 0.0;
-      for (int b = 0; b < nblk; ++b) s += gp[(size_t)b * ADJ_NT + k];
-      grad[k] = s;
-    }
-  }
-  // restore the realization as it was
-  HIPCHK(hipMemcpy(hr.desc.X[cur0], ck, n6 * sizeof(double), hipMemcpyDeviceToDevice));
-  HIPCHK(hipMemcpy(c->d_desc + r, &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_live + r, &lv0, sizeof(lv0), hipMemcpyHostToDevice));
-  if (nr) HIPCHK(hipMemcpy(hr.desc.ryr_state, ryr0.data(), nr * sizeof(int32_t), hipMemcpyHostToDevice));
-  HIPCHK(hipDeviceSynchronize());
-  return SRS_OK;
+  if (any_ryr && !schedules) return srs::fail(SRS_E_ARG, "null schedules");
+  return loss_grad_run(c, ids, scan_id, n_steps, schedules, n_samples, sample_steps, targets, ckpt, max_batch, losses,
+                       grads, scans);
 }
 
 }  // extern "C"

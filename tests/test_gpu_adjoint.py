@@ -189,3 +189,226 @@ def test_schedule_recorded_mid_run_follows_the_gated_trajectory():
         a.record_schedule(0, 4)
     a.close()
     b.close()
+
+
+# ---------------------------------------------------------------- batched adjoint
+
+B_STEPS, B_SAMPLES = 500, 10
+B_SUBSETS = ([0, 1, 2, 3, 4, 5, 6, 7, 8], [0, 4, 8], [1, 3, 4, 5, 7], [4], [2, 4, 6, 8])
+
+
+def _subset_channels(g, idx):
+    """RyRs idx of the grid's lattice with the neighbour table renumbered."""
+    idx = np.asarray(idx)
+    new = -np.ones(g.n_ryr, np.int64)
+    new[idx] = np.arange(len(idx))
+    nb = g.ryr_nb.reshape(-1, 4)[idx]
+    nb = np.where(nb >= 0, new[np.maximum(nb, 0)], -1).astype(np.int32)
+    return g.ryr_ele[idx], g.ryr_jsr[idx], nb.ravel()
+
+
+def _scan_desc(g):
+    cen = g.centroids()
+    lo, hi = cen.min(axis=0), cen.max(axis=0)
+    mid = (lo + hi) / 2
+    d = S.read_scan_parameters(None)
+    d.axis = 0
+    d.x_min, d.x_max, d.res_x_f, d.res_x_ca = float(lo[0] - mid[0]), float(hi[0] - mid[0]), 0.03, 0.01
+    rc = cen[g.ryr_ele].mean(axis=0) - mid
+    d.offset1, d.offset2 = float(rc[1]), float(rc[2])
+    for k in range(3):
+        d.domain[k] = float(hi[k] - lo[k]) + 0.08
+        d.var[k] = (0.001, 0.003, 0.001)[k]
+    d.pad_min, d.pad_max, d.pad_res = -0.6, 0.6, 0.05
+    return cen - mid, d
+
+
+def _member(pr, m):
+    """Parameters, channels, schedule and initial state of member m (seeded)."""
+    g = pr.grid
+    rng = np.random.default_rng(100 + m)
+    p = pr.params.copy()
+    p.setr(V_RYR=20 * p.reals[S.R["V_RYR"]], DEFAULT_CA_NSR=p.reals[S.R["DEFAULT_CA_NSR"]] * (0.8 + 0.1 * m))
+    sub = B_SUBSETS[m]
+    # a frozen trajectory: each RyR opens for one random window
+    sched = np.zeros((B_STEPS, len(sub)), np.int32)
+    for i in range(len(sub)):
+        a = int(rng.integers(0, B_STEPS // 2))
+        sched[a:a + int(rng.integers(50, 250)), i] = 1
+    sched[0, 0] = 1
+    R = np.log(rng.random(len(sub) + g.n_lcc))
+    return p, sub, sched, R
+
+
+def _batch_ctx(pr, members, group=1):
+    g = pr.grid
+    ctx = S.Context(0)
+    ctx.tune(group=group)
+    mid = ctx.add_mesh_grid(g, pr.aij, pr.nij, perm=W.kuhn_block_order(g))
+    for p, sub, _, _ in members:
+        ctx.add_realization(mid, *_subset_channels(g, sub), g.lcc_ele, p, seed=5, stream=0)
+    ctx.finalize()
+    for k, (p, sub, sched, R) in enumerate(members):
+        ctx.state_set(k, pr.X0)
+        ctx.channels_set(k, sched[0], np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R)
+        ctx.globals_set(k, 0.0, p.reals[S.R["DEFAULT_CA_NSR"]])
+    cen, d = _scan_desc(g)
+    sid, nf, _ = ctx.add_scan(mid, cen, d)
+    return ctx, sid, nf, d
+
+
+def _targets(pr, members, samples, perturb=0.1):
+    """Each member's dF/F0 at its theta moved by +-perturb, from a context of
+    its own through replay and scan sampling (a different code path from the
+    batched loss)."""
+    idx = S.adjoint_theta()
+    out, P_true = [], []
+    for m, mem in enumerate(members):
+        p, sub, sched, R = mem
+        ctx, sid, nf, d = _batch_ctx(pr, [mem])
+        sgn = np.where(np.random.default_rng(200 + m).random(len(idx)) < 0.5, -1.0, 1.0)
+        P = p.reals.copy()
+        P[idx] *= 1 + perturb * sgn
+        out.append(_scan_run(ctx, sid, sched, samples, P, pr.X0, d.f0))
+        P_true.append(P[idx])
+        ctx.close()
+    return np.array(out), np.array(P_true)
+
+
+def _batch_setup():
+    pr = W.prepare(W.tiny_box_spec(nx=8, ny=8, nz=5, tt=True, ryr=3, lcc=1), W.spark_params())
+    members = [_member(pr, m) for m in range(len(B_SUBSETS))]
+    samples = np.linspace(B_STEPS // B_SAMPLES, B_STEPS, B_SAMPLES).astype(np.int64)
+    return pr, members, samples
+
+
+def _set_theta(ctx, members, theta):
+    idx = S.adjoint_theta()
+    for k, (p, _, _, _) in enumerate(members):
+        P = p.reals.copy()
+        P[idx] = theta[k] if np.ndim(theta) == 2 else theta
+        ctx.params_set(k, P)
+
+
+def test_batched_adjoint_equals_single_realization_runs():
+    """Per-member losses and gradients of the batched call equal the
+    one-realization call on each member bit for bit, for any sub-batch size;
+    the losses vanish at each member's generating parameters; every
+    realization, member or not, is left as it was."""
+    pr, members, samples = _batch_setup()
+    tg, P_true = _targets(pr, members, samples)
+    ctx, sid, nf, _ = _batch_ctx(pr, members)
+    n = len(members)
+    scheds = [m[2] for m in members]
+    assert tg.shape == (n, B_SAMPLES, nf)
+    _set_theta(ctx, members, P_true)
+    L_true, sc = ctx.loss_grad_batch(range(n), sid, scheds, samples, tg, ckpt=64, grad=False, scans=True)
+    assert np.all(L_true == 0.0) and np.array_equal(sc, tg)
+    _set_theta(ctx, members, np.array([m[0].reals[S.adjoint_theta()] for m in members]))
+    before = [(ctx.state_get(k), ctx.channels_get(k), ctx.loop_get(k)) for k in range(n)]
+    L_all, g_all = ctx.loss_grad_batch(range(n), sid, scheds, samples, tg, ckpt=64)
+    assert np.all(L_all > 0) and np.all(np.isfinite(g_all))
+    for mb in (1, 2, 3):
+        L2, g2 = ctx.loss_grad_batch(range(n), sid, scheds, samples, tg, ckpt=64, max_batch=mb)
+        assert np.array_equal(L2, L_all) and np.array_equal(g2, g_all)
+    for k in range(n):
+        L1, g1 = ctx.loss_grad(k, sid, scheds[k], samples, tg[k], 64)
+        assert L1 == L_all[k] and np.array_equal(g1, g_all[k])
+    # a subset of the members, in another order
+    sub = [3, 0, 2]
+    L3, g3 = ctx.loss_grad_batch(sub, sid, [scheds[k] for k in sub], samples, tg[sub], ckpt=64)
+    assert np.array_equal(L3, L_all[sub]) and np.array_equal(g3, g_all[sub])
+    for k in range(n):
+        X, ch, lp = before[k]
+        assert np.array_equal(ctx.state_get(k), X)
+        assert all(np.array_equal(a, b) for a, b in zip(ctx.channels_get(k), ch))
+        assert ctx.loop_get(k) == lp
+    ctx.close()
+
+
+def test_batched_adjoint_gradients_match_central_differences():
+    """Four sampled members: |theta g - theta g_fd| <= 1e-5 relative, the FD
+    losses of all members coming from batched loss-only calls."""
+    pr, members, samples = _batch_setup()
+    tg, _ = _targets(pr, members, samples)
+    ctx, sid, nf, _ = _batch_ctx(pr, members)
+    n = len(members)
+    scheds = [m[2] for m in members]
+    th = np.array([m[0].reals[S.adjoint_theta()] for m in members])
+    _set_theta(ctx, members, th)
+    L0, g = ctx.loss_grad_batch(range(n), sid, scheds, samples, tg)  # automatic checkpoint interval
+    fd = np.zeros_like(g)
+    eps = 1e-5
+    for k in range(th.shape[1]):
+        tp, tm = th.copy(), th.copy()
+        tp[:, k] *= 1 + eps
+        tm[:, k] *= 1 - eps
+        _set_theta(ctx, members, tp)
+        Lp = ctx.loss_grad_batch(range(n), sid, scheds, samples, tg, grad=False)
+        _set_theta(ctx, members, tm)
+        Lm = ctx.loss_grad_batch(range(n), sid, scheds, samples, tg, grad=False)
+        fd[:, k] = (Lp - Lm) / (2 * eps * th[:, k])
+    for m in (0, 1, 2, 4):
+        sens, sens_fd = th[m] * g[m], th[m] * fd[m]
+        scale = np.abs(sens_fd).max()
+        assert scale > 0
+        err = np.abs(sens - sens_fd) / np.maximum(np.abs(sens_fd), 1e-3 * scale)
+        assert err.max() <= 1e-5, (m, err, sens, sens_fd)
+    ctx.close()
+
+
+def test_batched_adjoint_argument_errors():
+    pr, members, samples = _batch_setup()
+    ctx, sid, nf, _ = _batch_ctx(pr, members[:2])
+    scheds = [m[2] for m in members[:2]]
+    tg = np.zeros((2, B_SAMPLES, nf))
+    with pytest.raises(S.SrsError, match="twice"):
+        ctx.loss_grad_batch([0, 0], sid, [scheds[0], scheds[0]], samples, tg)
+    with pytest.raises(S.SrsError, match="out of range"):
+        ctx.loss_grad_batch([0, 5], sid, scheds, samples, tg)
+    with pytest.raises(S.SrsError, match="sample steps"):
+        ctx.loss_grad_batch([0, 1], sid, scheds, samples[::-1].copy(), tg)
+    ctx.close()
+    ctx, sid, nf, _ = _batch_ctx(pr, members[:2], group=2)
+    with pytest.raises(S.SrsError, match="groups of size 1"):
+        ctx.loss_grad_batch([0, 1], sid, scheds, samples, tg)
+    ctx.close()
+
+
+def test_schedules_recorded_together_equal_separate_recordings():
+    """srs_record_schedules over a context of three realizations (own seeds
+    and RyR subsets) gives each the schedule it records alone."""
+    pr = W.prepare(W.tiny_box_spec(nx=6, ny=6, nz=4, tt=True, ryr=3, lcc=1), W.spark_params())
+    g = pr.grid
+    p = pr.params.copy()
+    dt = p.reals[S.R["T_STEP"]]
+    p.setr(RYR_K_MINUS=1 / (40 * dt), RYR_K_PLUS=1 / (40 * dt * 0.2 ** 2.1))
+    subs = (B_SUBSETS[0], B_SUBSETS[2], B_SUBSETS[1])
+    n = 300
+
+    def build(ks):
+        ctx = S.Context(0)
+        mid = ctx.add_mesh_grid(g, pr.aij, pr.nij)
+        for k in ks:
+            ctx.add_realization(mid, *_subset_channels(g, subs[k]), g.lcc_ele, p, seed=31 + k, stream=k)
+        ctx.finalize()
+        for q, k in enumerate(ks):
+            rs = np.zeros(len(subs[k]), np.int32)
+            rs[0] = 1
+            ctx.state_set(q, pr.X0)
+            ctx.channels_set(q, rs, np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32),
+                             np.log(np.random.default_rng(k).random(len(subs[k]) + g.n_lcc)))
+            ctx.globals_set(q, 0.0, p.reals[S.R["DEFAULT_CA_NSR"]])
+        return ctx
+
+    ctx = build([0, 1, 2])
+    ctx.explicit_step(11, S.STEP_PLAIN)
+    both = ctx.record_schedules(n)
+    ctx.close()
+    for k in range(3):
+        one = build([k])
+        one.explicit_step(11, S.STEP_PLAIN)
+        sk = one.record_schedule(0, n)
+        one.close()
+        assert both[k].shape == (n, len(subs[k])) and np.array_equal(both[k], sk)
+        assert len({r.tobytes() for r in sk}) > 1
