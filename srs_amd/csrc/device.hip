@@ -1274,9 +1274,16 @@ struct srs_ctx {
   std::vector<void*> allocs;
   RealDesc* d_desc = nullptr;
   RealLive* d_live = nullptr;
-  int2* d_tiles = nullptr;   // {group, tile}
+  int2* d_tiles = nullptr;   // {group, tile} of the sweep
   int n_tiles = 0, grid_sweep = 0;
+  // Every group's tiles; when the groups whose members have all finished hold
+  // enough of them, the sweep runs over a list without them (d_tiles_live).
+  std::vector<int2> tiles_all;
+  int2* d_tiles_all = nullptr;
+  int2* d_tiles_live = nullptr;
+  bool tiles_compacted = false;
   int32_t* d_groups = nullptr;  // [n_groups][group] realization ids, -1 pads
+  std::vector<int32_t> groups;
   double* stage = nullptr;       // staging buffer for state transfers of grouped realizations
   size_t stage_len = 0;
   double last_total_ms = 0, last_sweep_ms = 0;
@@ -1345,6 +1352,7 @@ SweepFn pick_sweep(const srs_ctx* c) {
 }
 
 void launch_sweep(srs_ctx* c) {
+  if (c->grid_sweep == 0) return;  // every realization has finished
   // one realization: tile b is tile b of realization 0, no table lookups
   const int single = c->reals.size() == 1 ? 1 : 0;
   hipLaunchKernelGGL(pick_sweep(c), dim3(c->grid_sweep), dim3(c->tile), 0, c->stream, c->d_desc, c->d_live,
@@ -1355,6 +1363,64 @@ int launch_step(srs_ctx* c, uint32_t gmode) {
   launch_sweep(c);
   hipLaunchKernelGGL(k_gate, dim3((unsigned)c->reals.size()), dim3(GATE_THREADS), 0, c->stream, c->d_desc,
                      c->d_live, gmode);
+  return SRS_OK;
+}
+
+// Captured step blocks hold the sweep's tile list and grid; they are dropped
+// when either changes.
+int drop_graphs(srs_ctx* c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  return SRS_OK;
+}
+
+void set_tiles(srs_ctx* c, int2* d, int n) {
+  c->d_tiles = d;
+  c->n_tiles = n;
+  c->grid_sweep = (n + NXCD - 1) / NXCD * NXCD;
+}
+
+// The full tile list again, before the host changes live state (a finished
+// realization may be restarted).
+int tiles_full(srs_ctx* c) {
+  if (!c->tiles_compacted) return SRS_OK;
+  int rc = drop_graphs(c);
+  if (rc) return rc;
+  set_tiles(c, c->d_tiles_all, (int)c->tiles_all.size());
+  c->tiles_compacted = false;
+  return SRS_OK;
+}
+
+// A tile whose realizations have all finished still costs the sweep about
+// what a live one does (its workgroup is dispatched, reads the tile table and
+// the done flags, and exits), so once such tiles make up an eighth of the
+// list the sweep gets a list without them.  A finished realization stays
+// finished until the host changes its live state, which puts the full list
+// back (tiles_full).
+int tiles_compact(srs_ctx* c) {
+  const int n_real = (int)c->reals.size();
+  if (n_real < 2) return SRS_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<RealLive> lv(n_real);
+  HIPCHK(hipMemcpy(lv.data(), c->d_live, n_real * sizeof(RealLive), hipMemcpyDeviceToHost));
+  const int KG = c->group;
+  const int n_groups = (int)c->groups.size() / KG;
+  std::vector<char> live(n_groups, 0);
+  for (int g = 0; g < n_groups; ++g)
+    for (int s = 0; s < KG; ++s) {
+      const int r = c->groups[(size_t)KG * g + s];
+      if (r >= 0 && !lv[r].done) live[g] = 1;
+    }
+  std::vector<int2> t;
+  for (const int2& x : c->tiles_all)
+    if (live[x.x]) t.push_back(x);
+  if ((int64_t)t.size() == c->n_tiles || (int64_t)t.size() * 8 > (int64_t)c->n_tiles * 7) return SRS_OK;
+  int rc = drop_graphs(c);
+  if (rc) return rc;
+  if (!t.empty()) HIPCHK(hipMemcpy(c->d_tiles_live, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice));
+  set_tiles(c, c->d_tiles_live, (int)t.size());
+  c->tiles_compacted = true;
   return SRS_OK;
 }
 
@@ -1691,6 +1757,10 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
   if ((rc = dev_upload(c, &c->d_live, lives))) return rc;
   if ((rc = dev_upload(c, &c->d_tiles, tiles))) return rc;
   if ((rc = dev_upload(c, &c->d_groups, groups))) return rc;
+  c->tiles_all = tiles;
+  c->d_tiles_all = c->d_tiles;
+  c->groups = groups;
+  if (c->reals.size() > 1 && (rc = dev_alloc(c, &c->d_tiles_live, tiles.size()))) return rc;
   // meshes' channel flags were updated by add_realization; refresh them
   for (auto& hmp : c->meshes)
     HIPCHK(hipMemcpy(hmp->d_flags, hmp->flags.data(), hmp->flags.size(), hipMemcpyHostToDevice));
@@ -1797,6 +1867,10 @@ int srs_channels_get(srs_ctx* c, int32_t r, int32_t* ryr_state, int32_t* lcc_sta
 static int live_rw(srs_ctx* c, int32_t r, RealLive* lv, bool write) {
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (write) {
+    int rc = tiles_full(c);
+    if (rc) return rc;
+  }
   if (write) HIPCHK(hipMemcpy(c->d_live + r, lv, sizeof(*lv), hipMemcpyHostToDevice));
   else HIPCHK(hipMemcpy(lv, c->d_live + r, sizeof(*lv), hipMemcpyDeviceToHost));
   return SRS_OK;
@@ -1854,6 +1928,8 @@ int srs_explicit_step(srs_ctx* c, int64_t n_steps, uint32_t flags) {
   if (n_steps < 0) return srs::fail(SRS_E_ARG, "negative step count");
   HIPCHK(hipSetDevice(c->device));
   const uint32_t gmode = G_AFTER_SWEEP | ((flags & SRS_STEP_PROTOCOL) ? G_PROTOCOL : 0u);
+  int rc = tiles_compact(c);
+  if (rc) return rc;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   int64_t done = 0;
   if ((flags & SRS_STEP_GRAPH) && n_steps >= 16) {

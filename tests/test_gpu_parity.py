@@ -390,6 +390,75 @@ def test_no_channels_and_done_realizations():
     assert rel_err(X0, Fo.X) <= TOL
 
 
+@pytest.mark.parametrize("group", [1, 2])
+def test_finished_realizations_leave_the_sweep(group):
+    """Protocol runs that end at different times, stepped in chunks: the
+    sweep drops the tiles of finished groups, and every realization's records
+    and final state equal its own one-realization run; a finished realization
+    restarted by the host (longer T_FINAL, done cleared) is stepped again."""
+    pr = _tiny(tt=True)
+    dt = pr.params.reals[1]
+    n_real, chunk = 6, 40
+    params, fields = [], []
+    for k in range(n_real):
+        p = fast_gating(pr.params)
+        p.setr(T_FINAL=(60 + 55 * k) * dt, OUTPUT_STATES_INTERVAL=9 * dt, INIT_RYR_HOLD=20 * dt)
+        p.seti(INIT_RYR=k % pr.grid.n_ryr, FLAG_RYR_OUT=1)
+        params.append(p)
+        fields.append(initial_fields(pr, ryr_open=(k % pr.grid.n_ryr,), seed=50 + k))
+
+    def build(ks, grp):
+        ctx = S.Context(0)
+        ctx.tune(group=grp)
+        mid = ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij)
+        g = pr.grid
+        for k in ks:
+            ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, params[k], seed=70 + k, stream=k)
+        ctx.finalize(256)
+        for q, k in enumerate(ks):
+            F = fields[k]
+            ctx.state_set(q, F.X)
+            ctx.channels_set(q, F.ryr_state, F.lcc_state, F.lccv_state, F.R)
+            ctx.globals_set(q, F.gv[0], F.gv[1])
+        return ctx
+
+    def run(ctx, n):
+        for _ in range(200):
+            if all(ctx.loop_get(q)["done"] for q in range(n)):
+                return
+            ctx.explicit_step(chunk, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        raise AssertionError("runs did not finish")
+
+    ctx = build(range(n_real), group)
+    run(ctx, n_real)
+    got = [(ctx.state_get(k), ctx.records(k), ctx.loop_get(k)) for k in range(n_real)]
+    # restart realization 0 for 100 more steps of protocol
+    P0 = params[0].reals.copy()
+    P0[S.R["T_FINAL"]] += 100 * dt
+    ctx.params_set(0, P0)
+    ls = got[0][2]
+    ctx.loop_set(0, t=ls["t"], dt_last=ls["dt_last"], step=ls["step"], done=0, n_open=ls["n_ryr_open_last"],
+                 ca_max=ls["ca_ss_max_last"])
+    ctx.explicit_step(chunk, S.STEP_PROTOCOL | S.STEP_GRAPH)
+    restarted = ctx.state_get(0)
+    ctx.close()
+    assert len({g[2]["step"] for g in got}) > 1  # the runs ended at different steps
+    for k in range(n_real):
+        one = build([k], 1)
+        run(one, 1)
+        assert np.array_equal(one.state_get(0), got[k][0]), k
+        np.testing.assert_array_equal(one.records(0), got[k][1])
+        assert one.loop_get(0) == got[k][2], k
+        if k == 0:
+            one.params_set(0, P0)
+            one.loop_set(0, t=ls["t"], dt_last=ls["dt_last"], step=ls["step"], done=0,
+                         n_open=ls["n_ryr_open_last"], ca_max=ls["ca_ss_max_last"])
+            one.explicit_step(chunk, S.STEP_PROTOCOL | S.STEP_GRAPH)
+            assert np.array_equal(one.state_get(0), restarted)
+            assert not np.array_equal(restarted, got[0][0])
+        one.close()
+
+
 @pytest.mark.parametrize("cfg", [dict(tile=128, occupancy=8), dict(tile=128, nt_geometry=1), dict(tile=512, nt_state=1),
                                  dict(tile=256, plane_pad=448, nt_geometry=1, nt_state=1), dict(compact=0),
                                  dict(group=2), dict(group=4, compact=0, occupancy=8), dict(lds=1),
