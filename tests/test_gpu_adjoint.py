@@ -412,3 +412,29 @@ def test_schedules_recorded_together_equal_separate_recordings():
         one.close()
         assert both[k].shape == (n, len(subs[k])) and np.array_equal(both[k], sk)
         assert len({r.tobytes() for r in sk}) > 1
+
+
+def test_batch_fit_reduces_the_mean_loss():
+    """The batch-mean problem through the torch.autograd wrapper: its
+    gradient is the mean of the members' adjoint gradients, and a few Adam
+    steps on log theta lower the mean loss."""
+    import torch
+
+    from srs_amd.fit import BatchLinescanProblem, fit, linescan_loss_fn
+
+    pr, members, samples = _batch_setup()
+    tg, _ = _targets(pr, members, samples)
+    ctx, sid, nf, _ = _batch_ctx(pr, members)
+    n = len(members)
+    prob = BatchLinescanProblem(ctx, sid, range(n), [m[2] for m in members], samples, tg, [m[0] for m in members],
+                                [pr.X0] * n, ckpt=32)
+    th0 = prob.theta()
+    L, g = prob.loss_grad(th0)
+    Ls, gs = ctx.loss_grad_batch(range(n), sid, [m[2] for m in members], samples, tg, ckpt=32)
+    assert L == float(np.mean(Ls)) and np.array_equal(g, gs.mean(axis=0))
+    t = torch.tensor(th0, dtype=torch.float64, requires_grad=True)
+    linescan_loss_fn().apply(t, prob).backward()
+    np.testing.assert_array_equal(t.grad.numpy(), g)
+    th = fit(prob, steps=6, lr=0.02)
+    assert prob.loss(th) < L
+    ctx.close()
