@@ -525,6 +525,10 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
   const double* P = d.P;
   const double dt = P[SRS_R_T_STEP];
   const int64_t step = l.step;
+  // read by every thread before the first barrier: thread 0 rewrites the
+  // live state at the end, and the time decides below whether the whole
+  // workgroup takes part in the loop-body counts
+  const double t = l.t + dt;  // cpu/RunSimulation.cpp:572-573 (ffwd: :477-478)
   // ExplicitStep honours flag_nogating; the ffwd loop calls
   // UpdateChannelGating directly (cpu/RunSimulation.cpp:476, 650-652).
   const bool gating = !(mode & G_NO_GATING) && (ff || !d.I[SRS_I_FLAG_NO_GATING]);
@@ -656,24 +660,64 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
       for (int i = tid; i < nr; i += GATE_THREADS) d.ryr_state[i] = d.sched[k * nr + i];
   }
 
+  const double iv = P[SRS_R_OUTPUT_STATES_INTERVAL];
+  const bool out_due = floor(t / iv) > floor((t - dt) / iv);
+  // The loop body's counts (open RyRs, open LCCs, the largest cleft Ca over
+  // the RyRs) are needed in the ffwd phase and at states outputs.  The
+  // workgroup reduces them with wavefront ballots and a cross-lane max
+  // (exact: integer counts and a maximum, so any order gives the sequential
+  // result).  Each thread reads the channel entries it wrote itself above.
+  int n_open_all = 0, n_lcc_all = 0;
+  double cmax_all = 0.0;
+  if ((mode & G_PROTOCOL) && (ff || out_due)) {
+    __shared__ int s_cnt[2][GATE_THREADS / 64];
+    __shared__ double s_max[GATE_THREADS / 64];
+    int cr = 0, cl = 0;
+    double m = 0.0;
+    for (int b = 0; b < nr; b += GATE_THREADS) {
+      const int i = b + tid;
+      const bool open = i < nr && d.ryr_state[i] != 0;
+      cr += __builtin_popcountll(__builtin_amdgcn_ballot_w64(open));
+      if (i < nr) {
+        const double c = Ca[cq(d, d.ryr_ele[i])];
+        if (c > m) m = c;
+      }
+    }
+    for (int b = 0; b < d.n_lcc; b += GATE_THREADS) {
+      const int i = b + (GATE_THREADS - 1 - tid);
+      const bool open = i < d.n_lcc && d.lccv_state[i] == 2 && (d.lcc_state[i] == 6 || d.lcc_state[i] == 12);
+      cl += __builtin_popcountll(__builtin_amdgcn_ballot_w64(open));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double v = __shfl_xor(m, o);
+      if (v > m) m = v;
+    }
+    const int wave = tid >> 6;
+    if ((tid & 63) == 0) {
+      s_cnt[0][wave] = cr;
+      s_cnt[1][wave] = cl;
+      s_max[wave] = m;
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int w = 0; w < GATE_THREADS / 64; ++w) {
+        n_open_all += s_cnt[0][w];
+        n_lcc_all += s_cnt[1][w];
+        if (s_max[w] > cmax_all) cmax_all = s_max[w];
+      }
+  }
+
   if (tid != 0) return;
   if (swept) l.cur = cur ^ 1;
   l.step = step + 1;
-  const double t = l.t + dt;  // cpu/RunSimulation.cpp:572-573 (ffwd: :477-478)
   l.t = t;
   l.dt_last = dt;
   if (!(mode & G_PROTOCOL)) return;
 
   if (ff) {  // the rest of the ffwd loop body, cpu/RunSimulation.cpp:480-518
-    int n_open = 0, n_open_lcc = 0;
-    for (int i = 0; i < nr; ++i) n_open += d.ryr_state[i];
-    for (int i = 0; i < d.n_lcc; ++i)
-      if (d.lccv_state[i] == 2 && (d.lcc_state[i] == 6 || d.lcc_state[i] == 12)) n_open_lcc++;
+    const int n_open = n_open_all, n_open_lcc = n_lcc_all;
     const bool stop = n_open > 0 || n_open_lcc > 0 || t > P[SRS_R_T_FINAL];
-    if (!stop) {
-      const double iv = P[SRS_R_OUTPUT_STATES_INTERVAL];
-      if (floor(t / iv) > floor((t - dt) / iv)) write_record(d, l, Ca, t);
-    }
+    if (!stop && out_due) write_record(d, l, Ca, t);
     if (d.I[SRS_I_FLAG_V_CLAMP]) l.vm = (t <= P[SRS_R_T_CLAMP]) ? P[SRS_R_V_CLAMP] : P[SRS_R_DEFAULT_VM];
     if (!stop) return;
     // The main loop starts at this t with N_RyR_Open from the last ffwd
@@ -698,18 +742,13 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
   }
   if (d.I[SRS_I_FLAG_V_CLAMP]) l.vm = (t <= P[SRS_R_T_CLAMP]) ? P[SRS_R_V_CLAMP] : P[SRS_R_DEFAULT_VM];  // :531-537
   const int ir = d.I[SRS_I_INIT_RYR];
-  if (ir != -1 && t < P[SRS_R_INIT_RYR_HOLD] && !d.ryr_state[ir]) d.ryr_state[ir] = 1;  // :540-542
-  const double iv = P[SRS_R_OUTPUT_STATES_INTERVAL];
-  if (floor(t / iv) > floor((t - dt) / iv)) {  // :552-566
-    int n_open = 0;
-    double cmax = 0;
-    for (int i = 0; i < nr; ++i) {
-      n_open += d.ryr_state[i];
-      const double c = Ca[cq(d, d.ryr_ele[i])];
-      if (c > cmax) cmax = c;
-    }
-    l.n_open_last = n_open;
-    l.ca_max_last = cmax;
+  if (ir != -1 && t < P[SRS_R_INIT_RYR_HOLD] && !d.ryr_state[ir]) {  // :540-542
+    d.ryr_state[ir] = 1;
+    ++n_open_all;  // counted closed above
+  }
+  if (out_due) {  // :552-566
+    l.n_open_last = n_open_all;
+    l.ca_max_last = cmax_all;
     write_record(d, l, Ca, t);
   }
 }
