@@ -888,7 +888,33 @@ __device__ __forceinline__ SlBuf sl_buffer(const RealDesc& d, int j, double c, u
   return s;
 }
 
-// Pass 1: w = (lambda_Ca / beta, lambda_ATP, lambda_CaM, lambda_Fluo) / V0.
+// The loss source of one scan sample at element j: sum_i gcoef_i w(c_j - p_i)
+// over points within 1 um (the blur's PSF, linescan/parameters.h:60-62).
+__device__ __forceinline__ double scan_adj_sum(const double* __restrict__ cen, const ScanGeo& g, int n_f,
+                                               const double* __restrict__ gcoef, size_t np, int j) {
+  const double cx = cen[j], cy = cen[np + j], cz = cen[2 * np + j];
+  double s = 0.0;
+  for (int i = 0; i < n_f; ++i) {
+    double p[3] = {g.base[0], g.base[1], g.base[2]};
+    p[g.axis] = g.x_min + i * g.res;
+    const double rx = cx - p[0], ry = cy - p[1], rz = cz - p[2];
+    if (rx * rx + ry * ry + rz * rz < 1.0)
+      s += gcoef[i] * (g.alpha * exp(-0.5 * ((rx * rx / (g.sx * g.sx)) + (ry * ry / (g.sy * g.sy)) +
+                                             (rz * rz / (g.sz * g.sz)))) / (g.sx * g.sy * g.sz));
+  }
+  return s;
+}
+
+// w = (lambda_Ca / beta, lambda_ATP, lambda_CaM, lambda_Fluo) / V0 of element j.
+__device__ __forceinline__ d4 adj_weight(const RealDesc& d, int j, double c, uint8_t f, double la_c, double la_a,
+                                         double la_m, double la_d) {
+  const SlBuf sb = sl_buffer(d, j, c, f);
+  const double iv = 1.0 / d.V0[j];
+  return d4{la_c / sb.beta * iv, la_a * iv, la_m * iv, la_d * iv};
+}
+
+// Pass 1 (the first step of a recomputed segment; later steps get w from the
+// step kernel): w = (lambda_Ca / beta, lambda_ATP, lambda_CaM, lambda_Fluo) / V0.
 __global__ __launch_bounds__(ADJ_BS) void k_adj_w(const RealDesc* __restrict__ D, const int32_t* __restrict__ rid,
                                                   const double* __restrict__ X, const double* __restrict__ lam,
                                                   d4* __restrict__ w) {
@@ -898,21 +924,29 @@ __global__ __launch_bounds__(ADJ_BS) void k_adj_w(const RealDesc* __restrict__ D
   X += blockIdx.y * 6 * (size_t)d.np;
   lam += blockIdx.y * 6 * (size_t)d.np;
   w += blockIdx.y * (size_t)d.np;
-  const uint8_t f = d.flags[j];
   const d4 m = ((const d4*)lam)[j];
-  const double c = X[4 * (size_t)j];
-  const SlBuf sb = sl_buffer(d, j, c, f);
-  const double iv = 1.0 / d.V0[j];
-  w[j] = d4{m.x / sb.beta * iv, m.y * iv, m.z * iv, m.w * iv};
+  w[j] = adj_weight(d, j, X[4 * (size_t)j], d.flags[j], m.x, m.y, m.z, m.w);
 }
 
+// What the step kernel does after lambda_n: add the loss source of a scan
+// sample taken after n steps (SCAN), and form w for step n - 1 from lambda_n
+// and that step's state (Xprev, when it is in the recomputed segment).
+struct AdjNext {
+  const double* Xprev;  // state of step n - 1, or null
+  d4* wout;             // its w
+  const double* cen;    // scan cell centres [3][np]
+  const double* gcoef;  // the sample's loss coefficients per point
+  ScanGeo g;
+  int n_f, gstride;
+};
+
 // Pass 2: lambda_n and the theta-gradient partials of one step.
-template <bool COMPACT>
+template <bool COMPACT, bool SCAN>
 __global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
                                                      const int32_t* __restrict__ rid, const double* __restrict__ X,
                                                      const double* __restrict__ lam, const d4* __restrict__ w,
                                                      int64_t step, double* __restrict__ lam_out,
-                                                     double* __restrict__ gpart) {
+                                                     double* __restrict__ gpart, AdjNext nx) {
   __shared__ double red[ADJ_BS / 64][ADJ_NT];
   const int r = rid[blockIdx.y];
   const RealDesc& d = D[r];
@@ -924,6 +958,8 @@ __global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict_
   lam_out += blockIdx.y * 6 * np;
   w += blockIdx.y * np;
   gpart += (size_t)blockIdx.y * gridDim.x * ADJ_NT;
+  // the block's running partials, loaded now so the final update waits on no load
+  const double gp_old = threadIdx.x < ADJ_NT ? gpart[(size_t)blockIdx.x * ADJ_NT + threadIdx.x] : 0.0;
   // RyR states during this step: row `step` of the frozen schedule
   const int32_t* ryr = d.n_ryr ? d.sched + step * d.n_ryr : nullptr;
   double gr[ADJ_NT];
@@ -1058,9 +1094,14 @@ __global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict_
       lc += dt * mu.x * (-rca * sb.dbeta_dc / b2);
       gr[12] += dt * mu.x * (-rca * sb.dbeta_dkh / b2);
     }
+    if (SCAN) ld += V0 * scan_adj_sum(nx.cen, nx.g, nx.n_f, nx.gcoef + (size_t)blockIdx.y * nx.gstride, np, j);
     ((d4*)lam_out)[j] = d4{lc, la, lm, ld};
     lam_out[4 * np + j] = lT;
     lam_out[5 * np + j] = lC;
+    if (nx.Xprev) {
+      const double* Xp = nx.Xprev + blockIdx.y * 6 * np;
+      nx.wout[blockIdx.y * np + j] = adj_weight(d, j, Xp[4 * (size_t)j], f, lc, la, lm, ld);
+    }
   }
   // block reduction of the gradient partials, accumulated per block
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1074,7 +1115,7 @@ __global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict_
   if (threadIdx.x < ADJ_NT) {
     double s = 0.0;
     for (int q = 0; q < ADJ_BS / 64; ++q) s += red[q][threadIdx.x];
-    gpart[(size_t)blockIdx.x * ADJ_NT + threadIdx.x] += s;
+    gpart[(size_t)blockIdx.x * ADJ_NT + threadIdx.x] = gp_old + s;
   }
 }
 
@@ -1088,19 +1129,8 @@ __global__ __launch_bounds__(ADJ_BS) void k_scan_adj(const RealDesc* __restrict_
   const int j = blockIdx.x * ADJ_BS + threadIdx.x;
   if (j >= d.n) return;
   const size_t np = d.np;
-  gcoef += (size_t)blockIdx.y * gstride;
   lam += blockIdx.y * 6 * np;
-  const double cx = cen[j], cy = cen[np + j], cz = cen[2 * np + j];
-  double s = 0.0;
-  for (int i = 0; i < n_f; ++i) {
-    double p[3] = {g.base[0], g.base[1], g.base[2]};
-    p[g.axis] = g.x_min + i * g.res;
-    const double rx = cx - p[0], ry = cy - p[1], rz = cz - p[2];
-    if (rx * rx + ry * ry + rz * rz < 1.0)
-      s += gcoef[i] * (g.alpha * exp(-0.5 * ((rx * rx / (g.sx * g.sx)) + (ry * ry / (g.sy * g.sy)) +
-                                             (rz * rz / (g.sz * g.sz)))) / (g.sx * g.sy * g.sz));
-  }
-  lam[4 * (size_t)j + 3] += d.V0[j] * s;
+  lam[4 * (size_t)j + 3] += d.V0[j] * scan_adj_sum(cen, g, n_f, gcoef + (size_t)blockIdx.y * gstride, np, j);
 }
 
 // Copies the current state (X[cur]) of each realization of a table to slot y
@@ -1116,6 +1146,22 @@ __global__ void k_batch_state(const RealDesc* __restrict__ D, const RealLive* __
     if (to_buf) B[i] = X[i];
     else X[i] = B[i];
   }
+}
+
+// Points the ping-pong buffers of each realization y of a table at slots of a
+// recomputed segment (slot i of y at seg + (i * nb + y) * 6 np): the current
+// state at slot step - s0, the next one at the slot after, so the sweep writes
+// the segment in place.
+__global__ void k_seg_ptrs(RealDesc* __restrict__ D, const RealLive* __restrict__ L, const int32_t* __restrict__ rid,
+                           double* seg, int64_t s0, int nb) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= nb) return;
+  const int r = rid[y];
+  const int cur = L[r].cur;
+  const size_t n6 = 6 * (size_t)D[r].np;
+  const size_t i = (size_t)(L[r].step - s0);
+  D[r].X[cur] = seg + (i * nb + y) * n6;
+  D[r].X[cur ^ 1] = seg + ((i + 1) * nb + y) * n6;
 }
 
 // Starts a replayed segment at `step` for each realization of a table: the
@@ -2413,7 +2459,7 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
     max_sched = std::max(max_sched, k);
   }
   // device bytes per member of a sub-batch
-  const size_t per = 8 * n6 * (size_t)(n_ck_kept + (want_grad ? ckpt + 2 : 0)) + (want_grad ? 32 * np : 0) +
+  const size_t per = 8 * n6 * (size_t)(n_ck_kept + (want_grad ? ckpt + 2 : 0)) + (want_grad ? 64 * np : 0) +
                      8 * (size_t)n_samples * nout + 8 * (size_t)S.n_blocks * nf + 8 * (size_t)nblk * ADJ_NT +
                      8 * (size_t)n_samples * nf + 4 * max_sched + 64;
   HIPCHK(hipSetDevice(c->device));
@@ -2441,7 +2487,7 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
   DevMem M;
   double *ck = nullptr, *samp = nullptr, *part = nullptr, *dg = nullptr, *gpart = nullptr;
   double *lam0 = nullptr, *lam1 = nullptr, *seg = nullptr;
-  d4* w = nullptr;
+  d4 *w0 = nullptr, *w1 = nullptr;
   int32_t *d_sched = nullptr, *d_rid = nullptr;
   int2* d_sub_tiles = nullptr;
   const int nt_mesh = (hm.n + c->tile - 1) / c->tile;
@@ -2454,7 +2500,8 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
   if (want_grad) {
     HIPCHK(M.alloc(&lam0, (size_t)bs * n6));
     HIPCHK(M.alloc(&lam1, (size_t)bs * n6));
-    HIPCHK(M.alloc(&w, (size_t)bs * np));
+    HIPCHK(M.alloc(&w0, (size_t)bs * np));
+    HIPCHK(M.alloc(&w1, (size_t)bs * np));
     HIPCHK(M.alloc(&seg, (size_t)ckpt * bs * n6));
     HIPCHK(M.alloc(&gpart, (size_t)bs * nblk * ADJ_NT));
     HIPCHK(M.alloc(&dg, (size_t)bs * n_samples * nf));
@@ -2556,37 +2603,44 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
       if (e == hipSuccess) e = hipMemsetAsync(gpart, 0, (size_t)nb * nblk * ADJ_NT * sizeof(double), c->stream);
       double* lam = lam0;   // lambda_{n+1}
       double* lout = lam1;  // lambda_n
+      d4* wcur = w0;        // w of step n
+      d4* wnext = w1;       // w of step n - 1
       ks = n_samples - 1;
+      // the loss source of a sample after the last step; the step kernel
+      // adds the others to the lambda it produces
+      if (sample_steps[ks] == n_steps) {
+        hipLaunchKernelGGL(k_scan_adj, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, d_rid, S.d_cen, S.geo, nf,
+                           dg + (size_t)ks * nf, n_samples * nf, lam);
+        --ks;
+      }
       for (int64_t s0 = (n_ck - 1) * ckpt; s0 >= 0 && e == hipSuccess; s0 -= ckpt) {
         const int64_t s1 = std::min<int64_t>(n_steps, s0 + ckpt);
-        // recompute the segment from its checkpoint, keeping every state
+        // recompute the segment from its checkpoint, the sweep writing each
+        // state into its slot of the segment
         hipLaunchKernelGGL(k_seg_start, dim3(nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, s0);
-        hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid,
-                           ck_at(s0 / ckpt), 0);
         e = hipMemcpyAsync(seg, ck_at(s0 / ckpt), (size_t)nb * n6 * sizeof(double), hipMemcpyDeviceToDevice,
                            c->stream);
         for (int64_t n = s0; n < s1 - 1 && e == hipSuccess; ++n) {
+          hipLaunchKernelGGL(k_seg_ptrs, dim3((nb + 63) / 64), dim3(64), 0, c->stream, c->d_desc, c->d_live, d_rid,
+                             seg, s0, nb);
           launch_step(c, gmode);
-          hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid,
-                             seg + (size_t)(n + 1 - s0) * nb * n6, 1);
           e = hipGetLastError();
         }
         // backward through the segment
         for (int64_t n = s1 - 1; n >= s0 && e == hipSuccess; --n) {
-          if (ks >= 0 && sample_steps[ks] == n + 1) {
-            hipLaunchKernelGGL(k_scan_adj, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, d_rid, S.d_cen, S.geo, nf,
-                               dg + (size_t)ks * nf, n_samples * nf, lam);
-            --ks;
-          }
           const double* Xn = seg + (size_t)(n - s0) * nb * n6;
-          hipLaunchKernelGGL(k_adj_w, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, d_rid, Xn, lam, w);
-          if (c->compact_all)
-            hipLaunchKernelGGL(k_adj_step<true>, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, d_rid, Xn,
-                               lam, w, n, lout, gpart);
-          else
-            hipLaunchKernelGGL(k_adj_step<false>, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, d_rid, Xn,
-                               lam, w, n, lout, gpart);
+          if (n == s1 - 1)
+            hipLaunchKernelGGL(k_adj_w, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, d_rid, Xn, lam, wcur);
+          const bool scan = ks >= 0 && sample_steps[ks] == n;
+          AdjNext nx{n > s0 ? Xn - nb * n6 : nullptr, wnext, S.d_cen, scan ? dg + (size_t)ks * nf : nullptr,
+                     S.geo, nf, n_samples * nf};
+          if (scan) --ks;
+          auto kern = c->compact_all ? (scan ? k_adj_step<true, true> : k_adj_step<true, false>)
+                                     : (scan ? k_adj_step<false, true> : k_adj_step<false, false>);
+          hipLaunchKernelGGL(kern, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, d_rid, Xn, lam, wcur, n,
+                             lout, gpart, nx);
           std::swap(lam, lout);
+          std::swap(wcur, wnext);
           e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -2603,8 +2657,13 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
           grads[(size_t)(m0 + y) * ADJ_NT + k] = sum;
         }
     }
-    // the members' states as they were: live first (it names the current
-    // buffer), then the start state into that buffer
+    // the members' states as they were: descriptors (the recomputation
+    // pointed their buffers into the segment) and live state (it names the
+    // current buffer) first, then the start state into that buffer
+    for (int y = 0; y < nb; ++y) {
+      const HostReal& hr = c->reals[ids[m0 + y]];
+      HIPCHK(hipMemcpy(c->d_desc + ids[m0 + y], &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(c->d_live, live0.data(), n_real * sizeof(RealLive), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, ck_at(0), 0);
     HIPCHK(hipStreamSynchronize(c->stream));
