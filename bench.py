@@ -324,6 +324,71 @@ def bench_adjoint_batch(n_real=256, n_steps=256, n_points=64, n_samples=8, max_b
                 mean_grad_norm=float(np.linalg.norm(grad.mean(axis=0) * p.reals[idx])))
 
 
+def bench_fidelity(n_real=512, t_final=20.0, chunk=2000, group=4):
+    """Spark-fidelity ensemble with the reference's Fidelity protocol
+    (param/walker2014/Fidelity.xml: flag_fidelity, a run ends once 4 RyRs are
+    open or once no channel is open and the cleft Ca is below c_min = 1 uM;
+    luminal RyR regulation n = 4; states every 0.01 ms), T_FINAL capped at
+    t_final: n_real realizations on the h = 40 nm spark box, each with one
+    RyR (drawn per realization) open at t = 0 and its own Philox stream,
+    stepped in chunks of `chunk` steps until every run has ended.  Timed with
+    finished groups left out of the sweep (the default) and with every tile
+    swept, on the same seeds."""
+    import srs_amd as S
+    from srs_amd import workloads as W
+
+    pr = W.prepare_spark()
+    g = pr.grid
+    p = pr.params.copy()
+    p.setr(T_FINAL=t_final, C_MIN=1.0, RYR_K_PLUS=1.107e-4, RYR_PHI_M=4.0, OUTPUT_STATES_INTERVAL=0.01,
+           INIT_RYR_HOLD=0.0)
+    p.seti(FLAG_FIDELITY_SIM=1, N_RYR_FIDELITY=4, INIT_RYR=-1, FLAG_V_CLAMP=0)
+    dt = float(p.reals[S.R["T_STEP"]])
+    first = np.random.default_rng(11).integers(0, g.n_ryr, n_real)
+    out = {}
+    for drop in (1, 0):
+        ctx = S.Context(0)
+        ctx.tune(group=group)
+        mid = ctx.add_mesh_grid(g, pr.aij, pr.nij, perm=W.kuhn_block_order(g))
+        for r in range(n_real):
+            ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=38376427, stream=r)
+        ctx.finalize()
+        ctx.tune(drop_finished=drop)
+        for r in range(n_real):
+            rs, ls, lvs, R = W.initial_channels(pr, open_ryr=int(first[r]), seed=r)
+            ctx.state_set(r, pr.X0)
+            ctx.channels_set(r, rs, ls, lvs, R)
+            ctx.globals_set(r, p.reals[S.R["DEFAULT_VM"]], p.reals[S.R["DEFAULT_CA_NSR"]])
+            ctx.loop_set(r, n_open=1)
+        ctx.explicit_step(128, S.STEP_PROTOCOL | S.STEP_GRAPH)  # graph capture outside the timed region
+        ctx.sync()
+        t0 = time.perf_counter()
+        steps = 128
+        while True:
+            ls = [ctx.loop_get(r) for r in range(n_real)]
+            n_done = sum(x["done"] for x in ls)
+            if n_done == n_real:
+                break
+            if (steps // chunk) % 20 == 0:
+                log(f"  fidelity: {steps} steps, {n_done} of {n_real} runs ended")
+            ctx.explicit_step(chunk, S.STEP_PROTOCOL | S.STEP_GRAPH)
+            steps += chunk
+        wall = time.perf_counter() - t0
+        ctx.close()
+        t_end = np.array([x["t"] for x in ls])
+        sparks = np.array([x["n_ryr_open_last"] >= 4 for x in ls])
+        out["drop" if drop else "all"] = dict(wall_s=wall, steps=steps)
+        log(f"fidelity ({'finished groups dropped' if drop else 'every tile swept'}): {n_real} runs in {wall:.2f} s")
+    return dict(workload=f"spark_box h=40 nm ({g.n_ele} elements), {n_real} realizations, Fidelity protocol "
+                         f"(4 open RyRs or no open channel and cleft Ca < 1 uM ends a run), T_FINAL {t_final} ms, "
+                         f"groups of {group}, chunks of {chunk} steps",
+                n_real=n_real, dt_ms=dt, fidelity=float(sparks.mean()), mean_run_ms=float(t_end.mean()),
+                max_run_ms=float(t_end.max()), sim_ms_total=float(t_end.sum()),
+                wall_s=out["drop"]["wall_s"], wall_s_every_tile=out["all"]["wall_s"],
+                sparks_per_s=n_real / out["drop"]["wall_s"], sparks_per_s_every_tile=n_real / out["all"]["wall_s"],
+                spark_ms_per_s=float(t_end.sum()) / out["drop"]["wall_s"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -338,6 +403,8 @@ def main():
     ap.add_argument("--adjoint-batch", type=int, default=256,
                     help="realizations in the batched adjoint leg on the h=30 nm dyad geometry (0: skip)")
     ap.add_argument("--adjoint-batch-steps", type=int, default=256)
+    ap.add_argument("--fidelity", type=int, default=0,
+                    help="realizations in a spark-fidelity ensemble run to completion (0: skip)")
     ap.add_argument("--order", choices=("block", "rcm", "given"), default="block",
                     help="device element order: Kuhn cube blocks, reverse Cuthill-McKee, or the reference BFS order")
     ap.add_argument("--tune", default="auto",
@@ -433,6 +500,8 @@ def main():
         log(f"adjoint: {a['n_steps']} steps, forward {a['forward_s']:.2f} s, loss+gradient {a['loss_and_gradient_s']:.2f} s")
     if args.adjoint_batch > 0:
         out["adjoint_batch"] = bench_adjoint_batch(args.adjoint_batch, args.adjoint_batch_steps)
+    if args.fidelity > 0:
+        out["fidelity"] = bench_fidelity(args.fidelity)
     if trials:
         out["autotune"] = [{**cfg, "sweep_us": round(ms * 1e3, 2)} for cfg, ms in trials]
     print(json.dumps(out))

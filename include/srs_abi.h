@@ -260,6 +260,8 @@ int srs_ctx_create(int device, srs_ctx** out);
                                     of them (set before srs_ctx_finalize) */
 #define SRS_TUNE_LDS 8           /* 1: stage each sweep tile's diffusing species in LDS and read neighbours
                                     inside the tile from there (groups of 1) */
+#define SRS_TUNE_DROP_FINISHED 9 /* 1 (default): srs_explicit_step leaves the tiles of groups whose
+                                    realizations have all finished out of the sweep; 0: sweep them all */
 int srs_ctx_tune(srs_ctx* ctx, int32_t key, int32_t value);
 
 /* Replaces FreeDomainStruct / FreeGlobalStruct (gpu/RunSimulation.cu:874-901). */

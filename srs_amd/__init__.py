@@ -63,6 +63,7 @@ TUNE_COMPACT = 5
 TUNE_OCCUPANCY = 6
 TUNE_GROUP = 7
 TUNE_LDS = 8
+TUNE_DROP_FINISHED = 9
 
 
 def _p(a, t):
@@ -328,7 +329,8 @@ class Context:
     def tune(self, **kw):
         """tile=, nt_geometry=, nt_state=, plane_pad= (see srs_ctx_tune)."""
         keys = dict(tile=TUNE_TILE, nt_geometry=TUNE_NT_GEOMETRY, nt_state=TUNE_NT_STATE, plane_pad=TUNE_PLANE_PAD,
-                    compact=TUNE_COMPACT, occupancy=TUNE_OCCUPANCY, group=TUNE_GROUP, lds=TUNE_LDS)
+                    compact=TUNE_COMPACT, occupancy=TUNE_OCCUPANCY, group=TUNE_GROUP, lds=TUNE_LDS,
+                    drop_finished=TUNE_DROP_FINISHED)
         for k, v in kw.items():
             check(lib().srs_ctx_tune(self._h, keys[k], int(v)))
         return self

@@ -375,7 +375,8 @@ __device__ __forceinline__ void sweep_group(const RealDesc* __restrict__ D, cons
     const Loc c = load_loc<FK>(D[mem[0]], L[mem[0]], j, g);
     step_elem<NTS, FK>(D[mem[0]], L[mem[0]], j, g, c);
   } else {
-    // members one after another: the register budget of a single realization
+    // members one after another: the register budget of a single realization;
+    // -1 marks an empty slot or a member that does not step
     for (int s = 0; s < KG; ++s)
       if (mem[s] >= 0) {
         const Loc c = load_loc<FK>(D[mem[s]], L[mem[s]], j, g);
@@ -425,15 +426,18 @@ __device__ __forceinline__ void sweep_tile(const RealDesc* __restrict__ D, const
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   if (b >= n_tiles) return;
   const int2 tl = single ? make_int2(0, b) : tiles[b];  // {group, tile}
+  // members that step: a finished member (done) or one in the gating-only
+  // ffwd phase keeps its fields, so its part of the tile is skipped
   int mem[KG];
-  bool any = false;
+  int first = -1;
 #pragma unroll
   for (int s = 0; s < KG; ++s) {
-    mem[s] = single ? (s == 0 ? 0 : -1) : groups[KG * tl.x + s];
-    if (mem[s] >= 0) any |= !(L[mem[s]].done | L[mem[s]].ffwd);
+    const int r = single ? (s == 0 ? 0 : -1) : groups[KG * tl.x + s];
+    mem[s] = (r >= 0 && !(L[r].done | L[r].ffwd)) ? r : -1;
+    if (first < 0) first = mem[s];
   }
-  if (!any) return;
-  const RealDesc& d = D[mem[0]];
+  if (first < 0) return;
+  const RealDesc& d = D[first];
   const int j = tl.y * BS + threadIdx.x;
   if (j >= d.n) return;
   const Geo g = load_geo<NTG, COMPACT, -1>(d, j);
@@ -1349,6 +1353,7 @@ struct srs_ctx {
   int occ8 = 0;          // sweep compiled for eight wavefronts per SIMD
   int group = 1;         // realizations per realization group (1, 2 or 4)
   int lds = 0;           // stage each tile's quads in LDS for the neighbour gathers (groups of 1)
+  int drop_finished = 1; // sweep without the tiles of groups that have all finished
   bool compact_all = false;
   bool layout_fixed = false;
   hipStream_t stream = nullptr;
@@ -1485,7 +1490,7 @@ int tiles_full(srs_ctx* c) {
 // back (tiles_full).
 int tiles_compact(srs_ctx* c) {
   const int n_real = (int)c->reals.size();
-  if (n_real < 2) return SRS_OK;
+  if (n_real < 2 || !c->drop_finished) return SRS_OK;
   HIPCHK(hipStreamSynchronize(c->stream));
   std::vector<RealLive> lv(n_real);
   HIPCHK(hipMemcpy(lv.data(), c->d_live, n_real * sizeof(RealLive), hipMemcpyDeviceToHost));
@@ -1572,6 +1577,9 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
     case SRS_TUNE_OCCUPANCY: c->occ8 = value == 8; return SRS_OK;
     case SRS_TUNE_NT_STATE: c->nt_state = value != 0; return SRS_OK;
     case SRS_TUNE_LDS: c->lds = value != 0; return SRS_OK;
+    case SRS_TUNE_DROP_FINISHED:
+      c->drop_finished = value != 0;
+      return c->finalized ? tiles_full(c) : SRS_OK;
     case SRS_TUNE_GROUP:
       if (value != 1 && value != 2 && value != 4) return srs::fail(SRS_E_ARG, "group must be 1, 2 or 4");
       if (c->finalized) return srs::fail(SRS_E_STATE, "group must be set before srs_ctx_finalize");
