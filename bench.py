@@ -135,10 +135,21 @@ def bench_cpu(pr, budget_s=15.0):
     return n, wall, kind
 
 
-def bench_batched(n_real, steps, warmup, order, tune):
+def bench_batched(n_real, steps, warmup, order, tune=None, n_tune=512):
+    """The batched leg and its plain-sequential reference.  Unless `tune` is
+    given, the batched sweep is autotuned (tile, realization group, cache
+    hints, LDS staging) on the first `n_tune` members of the same ensemble,
+    and the plain path on each of the three meshes on its own."""
     import srs_amd as S
     from srs_amd import workloads as W
+    from srs_amd.autotune import autotune
 
+    trials = None
+    if tune is None:
+        sample = list(range(min(n_real, n_tune)))
+        tune, trials = autotune(lambda cfg: W.ensemble_context(n_real, order=order, tune=cfg, only=sample)[0],
+                                tiles=(128, 256), compact=(1,), occupancy=(0,), groups=(1, 2, 4))
+        log(f"batched autotune: {tune}")
     ctx, dts, n_ele, nbytes = W.ensemble_context(n_real, order=order, tune=tune, log=log)
     log(f"batched: {n_real} realizations, {n_ele} elements, group {tune.get('group', 1)}")
     ctx.explicit_step(warmup, S.STEP_GRAPH)
@@ -152,10 +163,13 @@ def bench_batched(n_real, steps, warmup, order, tune):
     # The plain path, one realization at a time: one member on each of the
     # three meshes, then the batch's mix of meshes run back to back.
     members = W.ensemble_members(n_real)
-    rate = {}
+    rate, ptunes = {}, {}
     for m in range(3):
         r = next(k for k, mem in enumerate(members) if mem["mesh"] == m)
-        pctx, pdts, _, _ = W.ensemble_context(n_real, order=order, tune={**tune, "group": 1}, only=[r])
+        ptune, _ = autotune(lambda cfg: W.ensemble_context(n_real, order=order, tune=cfg, only=[r])[0],
+                            tiles=(128, 256), compact=(1,), occupancy=(0,))
+        ptunes[m] = ptune
+        pctx, pdts, _, _ = W.ensemble_context(n_real, order=order, tune=ptune, only=[r])
         pctx.explicit_step(200, S.STEP_GRAPH)
         pctx.sync()
         t0 = time.perf_counter()
@@ -165,7 +179,8 @@ def bench_batched(n_real, steps, warmup, order, tune):
         pctx.close()
     seq = n_real / sum(1.0 / rate[mem["mesh"]] for mem in members)
     return dict(wall=wall, sim_ms=float(np.sum(dts)) * steps, sweep_ms=sweep_ms, gate_ms=gate_ms, n_ele=n_ele,
-                nbytes=nbytes, plain_rate=seq, plain_per_mesh=[rate[m] for m in range(3)])
+                nbytes=nbytes, plain_rate=seq, plain_per_mesh=[rate[m] for m in range(3)], tune=tune,
+                plain_tunes=[ptunes[m] for m in range(3)], trials=trials)
 
 
 def bench_adjoint(t_sim=5.0, n_points=64, n_samples=50, ckpt=256):
@@ -396,8 +411,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--batch", type=int, default=2048, help="realizations in the batched leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=20)
-    ap.add_argument("--batch-group", type=int, default=4, choices=(1, 2, 4),
-                    help="realizations per realization group in the batched leg")
+    ap.add_argument("--batch-group", type=int, default=0, choices=(0, 1, 2, 4),
+                    help="realizations per realization group in the batched leg (0: autotune the batched sweep)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work (0: skip)")
     ap.add_argument("--adjoint", type=int, default=1, help="1: time the adjoint workload (loss + gradient)")
     ap.add_argument("--adjoint-batch", type=int, default=256,
@@ -478,18 +493,21 @@ def main():
                                "sample": f"{n} ExplicitStep calls with gating on the same {N}-element mesh"}
         log(f"cpu {kind}: {n} steps in {cwall:.2f} s")
     if args.batch > 0:
-        b = bench_batched(args.batch, args.batch_steps, 2, args.order, {**tune, "group": args.batch_group})
+        b = bench_batched(args.batch, args.batch_steps, 2, args.order,
+                          {**tune, "group": args.batch_group} if args.batch_group else None)
         brate = b["sim_ms"] / b["wall"]
         bach = b["nbytes"] / (b["sweep_ms"] * 1e-3) / 1e9
         out["batched"] = {
             "workload": f"{args.batch} realizations on h=40/30/25 nm spark boxes (56,250/130,680/230,400 elements), "
                         "9-100 RyR each, SR load 0.5-1.25 mM, FP64",
             "value": brate, "unit": "spark-ms simulated/s", "sparks_per_s": brate / 50.0,
-            "steps": args.batch_steps, "ms_per_step": 1e3 * b["wall"] / args.batch_steps, "group": args.batch_group,
+            "steps": args.batch_steps, "ms_per_step": 1e3 * b["wall"] / args.batch_steps,
+            "group": b["tune"].get("group", 1), "tuning": b["tune"],
             "sweep_ms": b["sweep_ms"], "sweep_gbs": bach, "sweep_frac": bach / HBM_PEAK_GBS,
             "plain_sequential_value": b["plain_rate"], "speedup_vs_plain_sequential": brate / b["plain_rate"],
             "plain_per_mesh": {"h40": b["plain_per_mesh"][0], "h30": b["plain_per_mesh"][1],
                                "h25": b["plain_per_mesh"][2]},
+            "plain_tuning": {"h40": b["plain_tunes"][0], "h30": b["plain_tunes"][1], "h25": b["plain_tunes"][2]},
             "plain_sequential": "the plain path, one realization per context, over the batch's mix of meshes "
                                 "(spark-ms/s = n / sum of 1/rate of each member's mesh)",
         }

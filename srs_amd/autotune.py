@@ -1,8 +1,10 @@
 """Autotuner for the sweep kernel's launch shape and cache hints.
 
-Candidates: elements per workgroup (tile), non-temporal loads of the
-read-once operator rows (nt_geometry), non-temporal stores of the new state
-(nt_state), and SoA plane padding (plane_pad).  Every candidate computes the
+Candidates: elements per workgroup (tile), realizations per realization
+group (group, for batches), non-temporal loads of the read-once operator rows
+(nt_geometry), non-temporal stores of the new state (nt_state), SoA plane
+padding (plane_pad), the compact row format, the register budget and LDS
+staging.  Every candidate computes the
 same bits; only speed differs.  Each trial runs on a scratch context built
 by the caller's factory, so the context that is finally used has taken no
 steps.  Timing is the mean sweep-kernel duration from HIP events.
@@ -15,27 +17,29 @@ from typing import Callable
 
 from . import Context
 
-LAYOUT_KEYS = ("tile", "plane_pad", "compact")
+LAYOUT_KEYS = ("tile", "plane_pad", "compact", "group")
 KERNEL_KEYS = ("nt_geometry", "nt_state", "occupancy", "lds")
 
 
 def autotune(make: Callable[[dict], Context], tiles=(128, 256, 512), pads=(0,), compact=(1, 0), nt_geometry=(0, 1),
-             nt_state=(0, 1), occupancy=(0, 8), lds=(0, 1), reps: int = 20, warm: int = 3,
+             nt_state=(0, 1), occupancy=(0, 8), lds=(0, 1), groups=(1,), reps: int = 20, warm: int = 3,
              log=None) -> tuple[dict, list]:
     """Returns (best configuration, all trials as (config, sweep_ms))."""
     trials = []
-    for tile, pad, cmp in itertools.product(tiles, pads, compact):
-        ctx = make({"tile": tile, "plane_pad": pad, "compact": cmp})
+    for tile, pad, cmp, grp in itertools.product(tiles, pads, compact, groups):
+        layout = {"tile": tile, "plane_pad": pad, "compact": cmp}
+        if len(groups) > 1 or grp != 1:
+            layout["group"] = grp
+        ctx = make(layout)
         try:
             for g, s, o, ld in itertools.product(nt_geometry, nt_state, occupancy, lds):
-                if ld and o:
-                    continue  # the LDS-staged sweep has one register budget
+                if ld and (o or grp != 1):
+                    continue  # the LDS-staged sweep: one register budget, groups of 1
                 ctx.tune(nt_geometry=g, nt_state=s, occupancy=o, lds=ld)
                 ctx.explicit_step(warm)
                 t0 = time.perf_counter()
                 ms, _ = ctx.time_sweep(reps)
-                cfg = {"tile": tile, "plane_pad": pad, "compact": cmp, "nt_geometry": g, "nt_state": s, "occupancy": o,
-                       "lds": ld}
+                cfg = {**layout, "nt_geometry": g, "nt_state": s, "occupancy": o, "lds": ld}
                 trials.append((cfg, ms))
                 if log:
                     log(f"autotune {cfg}: sweep {ms * 1e3:.2f} us ({time.perf_counter() - t0:.2f} s)")
