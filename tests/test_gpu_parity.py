@@ -483,3 +483,35 @@ def test_tunables_do_not_change_results(cfg):
         outs.append((ctx.state_get(0), ctx.channels_get(0)[0]))
         ctx.close()
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_ryr_open_probability_law():
+    """At fixed Ca (no channel flux, SERCA and refill off, buffers at
+    equilibrium) an uncoupled RyR's open and closed dwell times are geometric
+    on the step grid (hazard integration with ln U residuals), so its long-run
+    open fraction is m_o / (m_o + m_c) with m = 1 / (1 - exp(-rate dt)).
+    576 independent channels (64 realizations x 9 RyRs, own Philox streams),
+    20,000 recorded steps each; the mean of the per-channel open fractions is
+    held to 4 standard errors."""
+    pr = _tiny(nx=6, ny=6, nz=4, tt=False, ryr=3, lcc=0)
+    g = pr.grid
+    p = pr.params.copy()
+    dt = p.reals[1]
+    c = float(pr.X0[g.ryr_ele[0]])
+    km = 1.0 / (40 * dt)
+    kp = 0.6 * km / c ** p.reals[S.R["RYR_ETA"]]  # closed -> open rate 0.6 km at this Ca
+    p.setr(RYR_K_MINUS=km, RYR_K_PLUS=kp, V_RYR=0.0, A_P=0.0, V_REFILL=0.0, V_CLAMP=-80.0, DEFAULT_VM=-80.0)
+    n_real, n_steps = 64, 20000
+    fields = [initial_fields(pr, ryr_open=(), vm=-80.0, seed=300 + k) for k in range(n_real)]
+    assert np.all(pr.X0[g.ryr_ele] == c)
+    with gpu_context([pr] * n_real, fields, params=[p] * n_real, seeds=[91] * n_real, share_mesh=True) as ctx:
+        sched = ctx.record_schedules(n_steps)
+        Xe = ctx.state_get(0)
+    assert np.max(np.abs(Xe[g.ryr_ele] / c - 1)) < 1e-9  # Ca stayed put (round-off of the equilibrium only)
+    frac = np.concatenate([s[1000:].mean(axis=0) for s in sched])  # drop the approach from all-closed
+    m_o = 1.0 / (1.0 - np.exp(-km * dt))
+    m_c = 1.0 / (1.0 - np.exp(-kp * c ** p.reals[S.R["RYR_ETA"]] * dt))
+    po = m_o / (m_o + m_c)
+    se = frac.std(ddof=1) / np.sqrt(frac.size)
+    assert abs(frac.mean() - po) < 4 * se, (frac.mean(), po, se)
+    assert 0.3 < po < 0.45 and se < 0.01
