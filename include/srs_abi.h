@@ -336,6 +336,18 @@ typedef struct srs_loop_state {
 int srs_loop_set(srs_ctx* ctx, int32_t real, const srs_loop_state* ls);
 int srs_loop_get(srs_ctx* ctx, int32_t real, srs_loop_state* ls);
 
+/* Checkpoint and resume of one realization (the reference restarts from a
+ * states file only through the unreachable INITIALIZE_FROM_FILE path,
+ * cpu/InitFuncs.cpp:815-888; these calls replace it): fields, channel states,
+ * residuals and loop state in one versioned buffer.  Gating draws are counted
+ * by step, so a realization loaded from a checkpoint continues bit for bit as
+ * the saved run would have; loading needs the same mesh size, channel counts,
+ * seed and stream.  Records not yet read with srs_records_read are not saved
+ * (the record buffer restarts empty). */
+int srs_checkpoint_size(srs_ctx* ctx, int32_t real, size_t* bytes);
+int srs_checkpoint_save(srs_ctx* ctx, int32_t real, void* buf, size_t bytes);
+int srs_checkpoint_load(srs_ctx* ctx, int32_t real, const void* buf, size_t bytes);
+
 /* Step flags */
 #define SRS_STEP_PLAIN 0u      /* exactly ExplicitStep: sweep, then gating unless flag_nogating */
 #define SRS_STEP_PROTOCOL 1u   /* also run the RunSimulation loop body on the device: time

@@ -445,6 +445,19 @@ class Context:
         check(lib().srs_loop_get(self._h, r, C.byref(s)))
         return {k: getattr(s, k) for k, _ in _abi.SrsLoopState._fields_}
 
+    def checkpoint(self, r) -> bytes:
+        """The realization's fields, channels and loop state as one buffer."""
+        n = C.c_size_t()
+        check(lib().srs_checkpoint_size(self._h, r, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(lib().srs_checkpoint_save(self._h, r, buf, n.value))
+        return buf.raw
+
+    def restore(self, r, data: bytes):
+        """Loads a buffer from checkpoint() into realization r."""
+        buf = C.create_string_buffer(bytes(data), len(data))
+        check(lib().srs_checkpoint_load(self._h, r, buf, len(data)))
+
     def explicit_step(self, n_steps=1, flags=STEP_PLAIN):
         check(lib().srs_explicit_step(self._h, int(n_steps), flags))
 

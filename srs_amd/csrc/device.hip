@@ -2014,6 +2014,115 @@ int srs_loop_get(srs_ctx* c, int32_t r, srs_loop_state* ls) {
   return SRS_OK;
 }
 
+// ---------------------------------------------------------- checkpoints
+// Layout of a realization's checkpoint: header, fields in the reference
+// numbering ([6][n_ele] FP64, as srs_state_get), RyR states, LCC states, LCC
+// voltage-gate states (int32, padded to 8 bytes), residuals, then the loop
+// state.
+namespace {
+struct CkHeader {
+  char magic[8];
+  int32_t version, n_ele, n_ryr, n_lcc;
+  uint64_t key, stream;
+};
+struct CkLive {
+  double t, dt_last, vm, ca_nsr, t_lcc_close, ca_max_last;
+  int64_t step;
+  int32_t done, lcc_open, n_open_last, ffwd;
+};
+constexpr char kCkMagic[8] = {'S', 'R', 'S', 'C', 'K', 'P', 'T', '1'};
+
+size_t ck_int_bytes(size_t nr, size_t nl) { return (sizeof(int32_t) * (nr + 2 * nl) + 7) / 8 * 8; }
+
+size_t ck_bytes(const HostReal& hr, int32_t n_ele) {
+  const size_t nr = hr.ryr_ele.size(), nl = hr.lcc_ele.size();
+  return sizeof(CkHeader) + 6 * sizeof(double) * (size_t)n_ele + ck_int_bytes(nr, nl) + sizeof(double) * (nr + nl) +
+         sizeof(CkLive);
+}
+}  // namespace
+
+int srs_checkpoint_size(srs_ctx* c, int32_t r, size_t* bytes) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  if (!bytes) return srs::fail(SRS_E_ARG, "null size");
+  *bytes = ck_bytes(c->reals[r], c->meshes[c->reals[r].mesh]->n);
+  return SRS_OK;
+}
+
+int srs_checkpoint_save(srs_ctx* c, int32_t r, void* buf, size_t bytes) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  const HostReal& hr = c->reals[r];
+  const int32_t n = c->meshes[hr.mesh]->n;
+  if (!buf || bytes < ck_bytes(hr, n)) return srs::fail(SRS_E_ARG, "checkpoint buffer smaller than srs_checkpoint_size");
+  const size_t nr = hr.ryr_ele.size(), nl = hr.lcc_ele.size();
+  char* p = (char*)buf;
+  CkHeader h{};
+  memcpy(h.magic, kCkMagic, 8);
+  h.version = 1;
+  h.n_ele = n;
+  h.n_ryr = (int32_t)nr;
+  h.n_lcc = (int32_t)nl;
+  h.key = hr.key;
+  h.stream = hr.stream;
+  memcpy(p, &h, sizeof h);
+  p += sizeof h;
+  if ((rc = srs_state_get(c, r, (double*)p))) return rc;
+  p += 6 * sizeof(double) * (size_t)n;
+  int32_t* rs = (int32_t*)p;
+  int32_t* ls = rs + nr;
+  int32_t* lv = ls + nl;
+  memset(lv + nl, 0, ck_int_bytes(nr, nl) - sizeof(int32_t) * (nr + 2 * nl));
+  double* R = (double*)(p + ck_int_bytes(nr, nl));
+  if ((rc = srs_channels_get(c, r, rs, ls, lv, R))) return rc;
+  p = (char*)(R + nr + nl);
+  RealLive L;
+  if ((rc = live_rw(c, r, &L, false))) return rc;
+  const CkLive k{L.t, L.dt_last, L.vm, L.ca_nsr, L.t_lcc_close, L.ca_max_last, L.step,
+                 L.done, L.lcc_open, L.n_open_last, L.ffwd};
+  memcpy(p, &k, sizeof k);
+  return SRS_OK;
+}
+
+int srs_checkpoint_load(srs_ctx* c, int32_t r, const void* buf, size_t bytes) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  const HostReal& hr = c->reals[r];
+  const int32_t n = c->meshes[hr.mesh]->n;
+  const size_t need = ck_bytes(hr, n);
+  if (!buf || bytes < sizeof(CkHeader)) return srs::fail(SRS_E_ARG, "checkpoint buffer too small");
+  const char* p = (const char*)buf;
+  CkHeader h;
+  memcpy(&h, p, sizeof h);
+  if (memcmp(h.magic, kCkMagic, 8) || h.version != 1) return srs::fail(SRS_E_IO, "not an srs checkpoint (version 1)");
+  if (h.n_ele != n || h.n_ryr != (int32_t)hr.ryr_ele.size() || h.n_lcc != (int32_t)hr.lcc_ele.size())
+    return srs::fail(SRS_E_ARG, "checkpoint of %d elements, %d RyR, %d LCC does not fit this realization", h.n_ele,
+                     h.n_ryr, h.n_lcc);
+  if (h.key != hr.key || h.stream != hr.stream)
+    return srs::fail(SRS_E_ARG, "checkpoint of another seed or stream: its gating would not continue the same run");
+  if (bytes < need) return srs::fail(SRS_E_ARG, "checkpoint buffer truncated");
+  p += sizeof h;
+  const size_t nr = h.n_ryr, nl = h.n_lcc;
+  const double* X = (const double*)p;
+  const int32_t* rs = (const int32_t*)(X + 6 * (size_t)n);
+  const int32_t* ls = rs + nr;
+  const int32_t* lv = ls + nl;
+  const double* R = (const double*)((const char*)rs + ck_int_bytes(nr, nl));
+  CkLive k;
+  memcpy(&k, R + nr + nl, sizeof k);
+  if ((rc = srs_state_set(c, r, X))) return rc;
+  if ((rc = srs_channels_set(c, r, rs, ls, lv, R))) return rc;
+  RealLive L;
+  if ((rc = live_rw(c, r, &L, false))) return rc;
+  L.t = k.t; L.dt_last = k.dt_last; L.vm = k.vm; L.ca_nsr = k.ca_nsr; L.t_lcc_close = k.t_lcc_close;
+  L.ca_max_last = k.ca_max_last; L.step = k.step; L.sched_base = k.step;
+  L.done = k.done; L.lcc_open = k.lcc_open; L.n_open_last = k.n_open_last; L.ffwd = k.ffwd;
+  L.rec_count = 0;
+  L.rec_overflow = 0;
+  serca_terms(L.ca_nsr, hr.reals, L.serca);
+  return live_rw(c, r, &L, true);
+}
+
 int srs_explicit_step(srs_ctx* c, int64_t n_steps, uint32_t flags) {
   srs::clear_error();
   if (!c) return srs::fail(SRS_E_ARG, "null context");

@@ -515,3 +515,48 @@ def test_ryr_open_probability_law():
     se = frac.std(ddof=1) / np.sqrt(frac.size)
     assert abs(frac.mean() - po) < 4 * se, (frac.mean(), po, se)
     assert 0.3 < po < 0.45 and se < 0.01
+
+
+@pytest.mark.parametrize("group", [1, 2])
+def test_checkpoint_resume_continues_bit_for_bit(group):
+    """Protocol run with gating and states records: 300 steps straight, or
+    150 steps, a checkpoint, a fresh context restored from it and 150 more
+    steps -- the same fields, channel states, loop state and records."""
+    pr = _tiny(tt=True)
+    dt = pr.params.reals[1]
+    p = fast_gating(pr.params)
+    p.setr(T_FINAL=1e3 * dt, OUTPUT_STATES_INTERVAL=7 * dt)
+    p.seti(FLAG_RYR_OUT=1)
+    Fs = [initial_fields(pr, seed=60 + k) for k in range(2)]
+    n_real = 2
+
+    def build():
+        return gpu_context([pr] * n_real, [F.copy() for F in Fs], params=[p] * n_real, seeds=[5, 6],
+                           record_capacity=256, share_mesh=True, group=group)
+
+    with build() as a:
+        a.explicit_step(300, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        want = [(a.state_get(k), a.channels_get(k), a.loop_get(k), a.records(k)) for k in range(n_real)]
+    with build() as b:
+        b.explicit_step(150, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        cks = [b.checkpoint(k) for k in range(n_real)]
+        recs_b = [b.records(k) for k in range(n_real)]
+    with build() as c:
+        for k in range(n_real):
+            c.restore(k, cks[k])
+        c.explicit_step(150, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        got = [(c.state_get(k), c.channels_get(k), c.loop_get(k), c.records(k)) for k in range(n_real)]
+        # a checkpoint belongs to its realization's stream; a cut buffer is refused
+        with pytest.raises(S.SrsError, match="seed or stream"):
+            c.restore(1, cks[0])
+        with pytest.raises(S.SrsError, match="truncated"):
+            c.restore(0, cks[0][:-8])
+        with pytest.raises(S.SrsError, match="not an srs checkpoint"):
+            c.restore(0, b"XXXXXXXX" + cks[0][8:])
+    for k in range(n_real):
+        X, ch, lp, rec = want[k]
+        assert np.array_equal(got[k][0], X)
+        assert all(np.array_equal(u, v) for u, v in zip(got[k][1], ch))
+        assert got[k][2] == lp
+        np.testing.assert_array_equal(np.vstack([recs_b[k], got[k][3]]), rec)
+    assert lp["step"] == 300 and len(rec) > 10
