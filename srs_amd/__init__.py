@@ -327,7 +327,9 @@ class Context:
         self._scans = {}
 
     def tune(self, **kw):
-        """tile=, nt_geometry=, nt_state=, plane_pad= (see srs_ctx_tune)."""
+        """Sweep tunables (srs_ctx_tune): tile=, plane_pad=, compact=, group=
+        before the meshes / srs_ctx_finalize; nt_geometry=, nt_state=,
+        occupancy=, lds=, drop_finished= at any time.  None changes results."""
         keys = dict(tile=TUNE_TILE, nt_geometry=TUNE_NT_GEOMETRY, nt_state=TUNE_NT_STATE, plane_pad=TUNE_PLANE_PAD,
                     compact=TUNE_COMPACT, occupancy=TUNE_OCCUPANCY, group=TUNE_GROUP, lds=TUNE_LDS,
                     drop_finished=TUNE_DROP_FINISHED)
