@@ -135,6 +135,55 @@ def bench_cpu(pr, budget_s=15.0):
     return n, wall, kind
 
 
+def _cpu_ensemble_worker(args):
+    """One host process of the CPU ensemble: the reference's own step (or the
+    oracle when it is not built) with gating on one ensemble mesh for about
+    `budget` seconds; returns (simulated ms, wall s, kind)."""
+    m, budget = args
+    from oracle.oracle import Fields, Oracle, RefStep, grid_dict
+    from srs_amd import workloads as W
+    import srs_amd as S
+
+    g = S.box_grid(W.ensemble_mesh_spec(m))
+    aij, nij = S.assemble_operator(g)
+    p = W.spark_params()
+    dt = W.stable_dt(aij, 0.22)
+    p.setr(T_STEP=dt, INIT_RYR_HOLD=dt)
+    X0, _ = S.equilibrium_state(g, p.reals)
+    rs = np.zeros(g.n_ryr, np.int32)
+    rs[g.n_ryr // 2] = 1
+    R = np.log(np.random.default_rng(m).random(g.n_ryr + g.n_lcc) * (1 - 2**-53) + 2**-54)
+    F = Fields(X0, rs, np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R, [0.0, p.reals[16]])
+    G = grid_dict(g, aij, nij)
+    if RefStep.available():
+        runner, kind = RefStep(), "reference"
+        run = lambda n: runner.steps(G, p.ints, p.reals, F, n, dt)  # noqa: E731
+    else:
+        runner, kind = Oracle(), "port"
+        run = lambda n: runner.steps(G, p.reals, F, n, dt, gating=True)  # noqa: E731
+    t0 = time.perf_counter()
+    run(2)
+    n = max(3, int(budget / ((time.perf_counter() - t0) / 2)))
+    t0 = time.perf_counter()
+    run(n)
+    return n * dt, time.perf_counter() - t0, kind
+
+
+def bench_cpu_ensemble(procs=16, budget_s=8.0):
+    """The batched workload's CPU reference: `procs` host processes at once
+    (one realization each, the three ensemble meshes in turn, as an MPI run
+    with one rank per core would), each timing the reference's own step with
+    gating.  Aggregate simulated spark-ms per wall second."""
+    import multiprocessing as mp
+
+    procs = max(1, min(procs, os.cpu_count() or 1))
+    with mp.get_context("spawn").Pool(procs) as pool:  # fresh interpreters: no GPU state is copied
+        res = pool.map(_cpu_ensemble_worker, [(w % 3, budget_s) for w in range(procs)])
+    return dict(value=sum(ms / wall for ms, wall, _ in res), unit="spark-ms simulated/s", cores=procs,
+                kind=res[0][2], sample=f"{procs} processes, one ensemble realization each (h=40/30/25 nm meshes in "
+                                      f"turn), about {budget_s:.0f} s of ExplicitStep + gating each")
+
+
 def bench_batched(n_real, steps, warmup, order, tune=None, n_tune=512):
     """The batched leg and its plain-sequential reference.  Unless `tune` is
     given, the batched sweep is autotuned (tile, realization group, cache
@@ -512,6 +561,11 @@ def main():
                                 "(spark-ms/s = n / sum of 1/rate of each member's mesh)",
         }
         log(f"batched: {brate:.3f} spark-ms/s vs plain {b['plain_rate']:.4f} ({brate / b['plain_rate']:.1f}x)")
+        if args.cpu_budget > 0:
+            ce = bench_cpu_ensemble(16, min(args.cpu_budget, 8.0))
+            out["batched"]["cpu_ensemble"] = ce
+            out["batched"]["speedup_vs_cpu_ensemble"] = brate / ce["value"]
+            log(f"cpu ensemble ({ce['kind']}, {ce['cores']} processes): {ce['value']:.4f} spark-ms/s")
     if args.adjoint:
         a = bench_adjoint()
         out["adjoint"] = a
