@@ -49,3 +49,43 @@ def test_context_without_device_fails_loudly():
         pytest.skip("a GPU is present")
     with pytest.raises(S.SrsError):
         S.Context(0)
+
+
+def test_header_is_plain_c(tmp_path):
+    """include/srs_abi.h compiles as C99 and links from C: a caller in the
+    reference's own language family (its cpu/ build is C-style C++) needs no
+    C++ types.  Host-only calls run; the device call reports no device (or,
+    with a GPU present, creates and destroys a context)."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "bridge.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <string.h>
+#include "srs_abi.h"
+int main(void) {
+  int32_t ints[SRS_N_INT]; double reals[SRS_N_REAL]; char title[256];
+  if (srs_read_parameters(NULL, ints, reals, title) != SRS_OK) return 1;
+  if (reals[SRS_R_D_CA] <= 0.0) return 2;
+  srs_box_spec spec; memset(&spec, 0, sizeof spec);
+  if (srs_read_parameters("/nonexistent.xml", ints, reals, title) == SRS_OK) return 3;
+  if (strstr(srs_last_error(), "cannot open") == NULL) return 4;
+  srs_ctx* ctx = NULL;
+  int rc = srs_ctx_create(0, &ctx);
+  if (rc == SRS_OK) srs_ctx_destroy(ctx);
+  else if (rc != SRS_E_NODEV && rc != SRS_E_HIP) return 5;
+  printf("%s\n", srs_version());
+  return 0;
+}
+''')
+    lib_dir = os.path.dirname(_abi.LIB_PATH)
+    exe = tmp_path / "bridge"
+    subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                    "-L", lib_dir, "-lsrs_amd", f"-Wl,-rpath,{lib_dir}"], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, (out.returncode, out.stderr)
+    assert "gfx950" in out.stdout
