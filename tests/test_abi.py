@@ -89,3 +89,36 @@ int main(void) {
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, (out.returncode, out.stderr)
     assert "gfx950" in out.stdout
+
+
+REF_CPU = "/root/reference/cpu"
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_CPU, "parameters.h")), reason="reference sources not present")
+def test_integration_snippets_compile(tmp_path):
+    """The C bridge in INTEGRATION.md compiles against the reference's own
+    cpu/parameters.h (its SimData/GridStruct/ParamStruct field names) and our
+    header, and the fitting snippets compile as C99."""
+    import shutil
+    import subprocess
+
+    if shutil.which("g++") is None or shutil.which("gcc") is None:
+        pytest.skip("no host compiler")
+    blocks = re.findall(r"```c\n(.*?)```", open(os.path.join(ROOT, "INTEGRATION.md")).read(), re.S)
+    assert len(blocks) >= 4
+    bridge = tmp_path / "srs_bridge.cpp"
+    bridge.write_text("#include <cstdio>\n#include <cstdlib>\n#include <cstring>\n" + blocks[0] +
+                      "\nvoid SrsDeviceLoop(FILE* states_file) {\n" + blocks[1] + "\n}\n")
+    fit = tmp_path / "fit.c"
+    fit.write_text("#include <stdlib.h>\n#include \"srs_abi.h\"\n"
+                   "void fit_one(srs_ctx* ctx, int64_t n_steps, int32_t n_ryr, int32_t scan_id, int32_t n_samples,\n"
+                   "             const int64_t* sample_steps, const double* target, const double* new_reals) {\n"
+                   + blocks[2] + "}\n"
+                   "void fit_batch(srs_ctx* ctx, int64_t n_steps, int32_t scan_id, int32_t n_samples,\n"
+                   "               const int64_t* sample_steps, const double* targets, int32_t* schedules,\n"
+                   "               const int32_t* ids) {\n  enum { N = 4 };\n" + blocks[3] + "}\n")
+    inc = os.path.join(ROOT, "include")
+    for cmd in (["g++", "-fsyntax-only", "-Wall", "-Werror", "-I", REF_CPU, "-I", inc, str(bridge)],
+                ["gcc", "-std=c99", "-fsyntax-only", "-Wall", "-Werror", "-I", inc, str(fit)]):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
