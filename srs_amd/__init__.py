@@ -463,6 +463,10 @@ class Context:
     def explicit_step(self, n_steps=1, flags=STEP_PLAIN):
         check(lib().srs_explicit_step(self._h, int(n_steps), flags))
 
+    def prepare(self, n_steps, flags=STEP_GRAPH):
+        """Captures the graphs explicit_step(n_steps, flags) will replay."""
+        check(lib().srs_explicit_step_prepare(self._h, int(n_steps), flags))
+
     def channel_gating(self):
         check(lib().srs_channel_gating(self._h))
 

@@ -2123,6 +2123,42 @@ int srs_checkpoint_load(srs_ctx* c, int32_t r, const void* buf, size_t bytes) {
   return live_rw(c, r, &L, true);
 }
 
+// The captured block of `block` steps in gate mode `gmode` (made on first use
+// and cached; tiles_compact and tiles_full drop the cache).
+static int graph_for(srs_ctx* c, int64_t block, uint32_t gmode, hipGraphExec_t* out) {
+  auto key = std::make_pair(block, gmode);
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int64_t k = 0; k < block; ++k) launch_step(c, gmode);
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t ge;
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    it = c->graphs.emplace(key, ge).first;
+  }
+  *out = it->second;
+  return SRS_OK;
+}
+
+constexpr int64_t kGraphBlock = 128, kGraphMin = 16;
+
+int srs_explicit_step_prepare(srs_ctx* c, int64_t n_steps, uint32_t flags) {
+  srs::clear_error();
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  if (!c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
+  if (n_steps < 0) return srs::fail(SRS_E_ARG, "negative step count");
+  HIPCHK(hipSetDevice(c->device));
+  const uint32_t gmode = G_AFTER_SWEEP | ((flags & SRS_STEP_PROTOCOL) ? G_PROTOCOL : 0u);
+  int rc = tiles_compact(c);
+  if (rc) return rc;
+  hipGraphExec_t ge;
+  if (n_steps >= kGraphBlock && (rc = graph_for(c, kGraphBlock, gmode, &ge))) return rc;
+  if (n_steps % kGraphBlock >= kGraphMin && (rc = graph_for(c, n_steps % kGraphBlock, gmode, &ge))) return rc;
+  return SRS_OK;
+}
+
 int srs_explicit_step(srs_ctx* c, int64_t n_steps, uint32_t flags) {
   srs::clear_error();
   if (!c) return srs::fail(SRS_E_ARG, "null context");
@@ -2134,23 +2170,17 @@ int srs_explicit_step(srs_ctx* c, int64_t n_steps, uint32_t flags) {
   if (rc) return rc;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   int64_t done = 0;
-  if ((flags & SRS_STEP_GRAPH) && n_steps >= 16) {
-    const int64_t block = std::min<int64_t>(n_steps, 128);
-    auto key = std::make_pair(block, gmode);
-    auto it = c->graphs.find(key);
-    if (it == c->graphs.end()) {
-      hipGraph_t g;
-      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      for (int64_t k = 0; k < block; ++k) launch_step(c, gmode);
-      HIPCHK(hipStreamEndCapture(c->stream, &g));
-      hipGraphExec_t ge;
-      HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-      HIPCHK(hipGraphDestroy(g));
-      it = c->graphs.emplace(key, ge).first;
+  if (flags & SRS_STEP_GRAPH) {
+    // 128-step blocks, then the rest as one block of its own when it is long enough
+    hipGraphExec_t ge;
+    if (n_steps >= kGraphBlock) {
+      if ((rc = graph_for(c, kGraphBlock, gmode, &ge))) return rc;
+      for (; n_steps - done >= kGraphBlock; done += kGraphBlock) HIPCHK(hipGraphLaunch(ge, c->stream));
     }
-    while (n_steps - done >= block) {
-      HIPCHK(hipGraphLaunch(it->second, c->stream));
-      done += block;
+    if (n_steps - done >= kGraphMin) {
+      if ((rc = graph_for(c, n_steps - done, gmode, &ge))) return rc;
+      HIPCHK(hipGraphLaunch(ge, c->stream));
+      done = n_steps;
     }
   }
   for (; done < n_steps; ++done) launch_step(c, gmode);

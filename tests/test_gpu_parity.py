@@ -78,15 +78,23 @@ def test_coupled_ryr_gating_is_sequential():
 
 
 def test_graph_replay_equals_launches():
+    """300 steps as 2 captured 128-step blocks and a 44-step block (also
+    after capturing them ahead with prepare) equal 300 single launches; calls
+    of 7 steps (no graph) and 20 steps (one block of 20) too."""
     pr = _tiny(tt=True)
     p = fast_gating(pr.params)
     F = initial_fields(pr, seed=2)
     out = []
-    for flags in (S.STEP_PLAIN, S.STEP_GRAPH):
+    for flags, pre, split in ((S.STEP_PLAIN, False, (300,)), (S.STEP_GRAPH, False, (300,)),
+                              (S.STEP_GRAPH, True, (300,)), (S.STEP_GRAPH, False, (7, 20, 273))):
         with gpu_context([pr], [F], params=[p], seeds=[3]) as ctx:
-            ctx.explicit_step(300, flags)
+            if pre:
+                ctx.prepare(300, S.STEP_GRAPH)
+            for n in split:
+                ctx.explicit_step(n, flags)
             out.append((ctx.state_get(0), ctx.channels_get(0)[0]))
-    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    for X, rs in out[1:]:
+        assert np.array_equal(out[0][0], X) and np.array_equal(out[0][1], rs)
 
 
 @pytest.mark.parametrize("lds", [0, 1])
