@@ -442,3 +442,30 @@ def test_open_random_ryr_selection_law(oracle):
     np.testing.assert_allclose(freq(P), np.array([10, 40, 20, 30]) / 100, atol=1e-4)
     P[S.R["RYR_ETA"]], P[S.R["RYR_PHI_B"]], P[S.R["RYR_PHI_M"]] = 1.0, 5.0, 0.0
     np.testing.assert_allclose(freq(P), np.array([1, 2, 3, 4]) / 10, atol=1e-4)
+
+
+def test_host_code_under_sanitizers(tmp_path):
+    """The host C++ (parameter and scan readers, box mesher, mesh reader on
+    whole and truncated files, assembly, BFS) built with AddressSanitizer and
+    UndefinedBehaviorSanitizer runs clean (tests/host_sanitize_driver.cpp)."""
+    import shutil
+    import subprocess
+
+    if shutil.which("g++") is None:
+        pytest.skip("no host compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    csrc = os.path.join(root, "srs_amd", "csrc")
+    exe = tmp_path / "driver"
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+           "-fno-omit-frame-pointer", "-I", os.path.join(root, "include"), "-I", csrc,
+           os.path.join(root, "tests", "host_sanitize_driver.cpp")] + \
+          [os.path.join(csrc, f) for f in ("host_params.cpp", "host_mesh.cpp", "host_common.cpp")] + ["-o", str(exe)]
+    subprocess.run(cmd, check=True, capture_output=True)
+    args = [str(exe), str(tmp_path)]
+    if os.path.isdir(REF_PARAM):
+        args.append(REF_PARAM)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert "ERROR: AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-4000:]
+    assert "truncation sweep done" in r.stdout
