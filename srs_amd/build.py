@@ -21,7 +21,7 @@ EXE = os.path.join(HERE, "cru3d_amd")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HOST_SRCS = ["host_common.cpp", "host_params.cpp", "host_mesh.cpp", "host_run.cpp"]
-HIP_SRCS = ["device.hip"]
+HIP_SRCS = ["device.hip", "linescan.hip", "adjoint.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-I", INCLUDE, "-I", CSRC]
 HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

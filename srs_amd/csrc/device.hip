@@ -16,184 +16,11 @@
 // Arithmetic keeps the reference's operation order and this file is compiled
 // without FP contraction, so the field update is bit-identical to the
 // reference CPU path for the same row order of each element's neighbours.
-#include <hip/hip_runtime.h>
+// The line-scan operator is in linescan.hip, recording and the adjoint in
+// adjoint.hip; the shared types are in dev_common.h.
+#include "dev_common.h"
 
-#include <algorithm>
-#include <array>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <map>
-#include <memory>
-#include <numeric>
-#include <vector>
-
-#include "kinetics.h"
-#include "philox.h"
-#include "srs_internal.h"
-
-#pragma clang fp contract(off)
-
-namespace {
-
-constexpr int MAX_TILE = 512;  // largest k_sweep workgroup (elements per workgroup)
-constexpr int GATE_THREADS = 256;
-constexpr int MAX_RYR = 1024;  // the reference GPU path launches <<<1, N_RyR>>>
-constexpr int MAX_LCC = 1024;
-constexpr int NXCD = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
-
-enum : uint8_t { F_JSR = 1, F_TROPC = 2, F_SL = 4, F_BR = 8, F_CHAN = 16 };
-
-// State layout of one realization (6*np doubles): the four diffusing species
-// interleaved per element, (Ca, ATP, CaM, Fluo) in 32 bytes, so that one
-// neighbour gather fetches all four with one contiguous access; then the TnC
-// and CSQ planes, which are never gathered.
-constexpr int MOB_SLOT[6] = {0, 1, 2, -1, -1, 3};  // species -> slot in the interleaved quad
-// A realization group holds K realizations of one mesh in blocks of 64
-// elements: block b holds the 64 quads of slot 0, then slot 1, ..., so a
-// wavefront's loads of one realization stay contiguous.  Element e of slot
-// `slot` is entry ((e/64)*K + slot)*64 + e%64 of the quad array, and of the
-// TnC and CSQ planes (K*np entries each) that follow the 4*K*np quad doubles.
-__host__ __device__ inline size_t gidx(size_t e, int K, int slot) {
-  return ((e >> 6) * (size_t)K + (size_t)slot) * 64 + (e & 63);
-}
-__host__ __device__ inline size_t xoff(int s, size_t e, size_t np, int K = 1, int slot = 0) {
-  const size_t q = gidx(e, K, slot);
-  return s == srs::TRPN ? 4 * (size_t)K * np + q : s == srs::CSQN ? 5 * (size_t)K * np + q : 4 * q + MOB_SLOT[s];
-}
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-// Immutable per-realization descriptor (read with scalar loads).
-struct RealDesc {
-  const int32_t* nbr;     // [4][np]
-  const double* coef;     // [5][np] (plain layout)
-  const int16_t* off;     // compact layout: [4][np] neighbour offsets j' - j
-  const uint16_t* cls;    // compact layout: [np] operator-row class
-  const int16_t* dq;      // compact layout: [5][np] coefficient deltas in ulps of the class row
-  const double* tab;      // compact layout: [n_class][10] class row (5 values) and its ulps (5 values)
-  int32_t n_class;
-  const uint8_t* flags;   // [np]
-  const double* V0;       // [np]
-  const double* tt;       // [np]
-  const double* br;       // [4][np] Ca, Fluo, ATP, CaM boundary rates, or null
-  double* X[2];           // [6][np] ping-pong state
-  int32_t n, np;
-  int32_t kst, slot;      // realization group: K realizations interleaved, this one in slot `slot`
-  int32_t n_ryr, n_lcc;
-  const int32_t* ryr_ele;
-  const int32_t* ryr_jsr;
-  const int32_t* ryr_nb;
-  int32_t* ryr_state;
-  const int32_t* lcc_ele;
-  int32_t* lcc_state;
-  int32_t* lccv_state;
-  double* R;
-  int32_t n_celem;
-  const int32_t* celem;   // sorted element indices that carry channel flux
-  const int32_t* cstart;  // [n_celem+1]
-  const int32_t* centry;  // >=0: 2*ryr + side(0 cleft, 1 jSR); <0: -(lcc+1)
-  const int32_t* sched;   // replay schedule [n_sched][n_ryr] or null
-  int64_t n_sched;
-  float* rec;
-  int32_t rec_cap, rec_len, ryr_out, gpu_layout;
-  uint64_t key, stream;
-  double P[SRS_N_REAL];
-  int32_t I[SRS_N_INT];
-};
-
-// Offset of element e's Ca in a realization's state buffer (its quad's first
-// double); Fluo is at +3.
-__device__ __forceinline__ size_t cq(const RealDesc& d, int e) { return 4 * gidx((size_t)e, d.kst, d.slot); }
-
-// Mutable per-realization loop state.
-struct RealLive {
-  double t, dt_last, vm, ca_nsr, t_lcc_close, ca_max_last;
-  // CaNSR-dependent SERCA subexpressions of cpu/RunSimulation.cpp:85-88, the
-  // same for every element: 0.104217 + 17.923 K_sr, 1.75583e6 + K_sr 7.61673e6,
-  // 6.08463e11 + K_sr 4.50544e11, 9.17846e6 - 11478.2 K_sr, 0.329904 K_sr
-  double serca[5];
-  int64_t step;
-  int64_t sched_base;
-  int32_t done, lcc_open, n_open_last, cur, rec_count, rec_overflow;
-  int32_t ffwd;  // gating-only phase of flag_ffwd: the sweep skips this realization
-};
-
-enum : uint32_t {
-  G_AFTER_SWEEP = 1,  // new state is X[1-cur]; flip cur afterwards
-  G_PROTOCOL = 2,     // run the loop body for the next time
-  G_NO_GATING = 4,    // skip gating regardless of the parameter flag
-  G_REPLAY = 8,       // load RyR states for the next step from the schedule
-};
-
-__device__ __forceinline__ int xcd_tile(int bid, int grid) {
-  // Consecutive tiles of a realization land on one XCD, so neighbour gathers
-  // that cross a tile boundary hit that XCD's L2.
-  return (bid % NXCD) * (grid / NXCD) + bid / NXCD;
-}
-
-// Loads of data read once per step (operator rows, flags) may bypass cache
-// allocation so that the state planes stay resident between steps.  Pointers
-// read from the descriptor are generic; casting them to the global address
-// space lets the compiler use global (not flat) memory instructions.
-#define SRS_GLOBAL __attribute__((address_space(1)))
-template <class T>
-__device__ __forceinline__ const SRS_GLOBAL T* gp(const T* p) { return (const SRS_GLOBAL T*)p; }
-template <class T>
-__device__ __forceinline__ SRS_GLOBAL T* gpw(T* p) { return (SRS_GLOBAL T*)p; }
-
-template <bool NT, class T>
-__device__ __forceinline__ T ldg(const SRS_GLOBAL T* p) {
-  if constexpr (NT) return __builtin_nontemporal_load(p);
-  else return *p;
-}
-template <bool NT, class T>
-__device__ __forceinline__ void stg(SRS_GLOBAL T* p, T v) {
-  if constexpr (NT) __builtin_nontemporal_store(v, p);
-  else *p = v;
-}
-
-// Operator row of one element: coefficients (diagonal first), neighbour
-// indices and flags.  Mesh-level data, shared by every realization of a
-// realization group.
-struct Geo {
-  double a0, a1, a2, a3, a4;
-  int n1, n2, n3, n4;
-  uint8_t f;
-};
-
-// FK >= 0: the caller knows the element's flags (a wavefront whose lanes are
-// all alike), so none are loaded.
-template <bool NTG, bool COMPACT, int FK>
-__device__ __forceinline__ Geo load_geo(const RealDesc& d, const int j) {
-  Geo g;
-  const int np = d.np;
-  g.f = FK >= 0 ? (uint8_t)FK : ldg<NTG>(gp(d.flags) + j);
-  if constexpr (COMPACT) {
-    // Operator row = class row + small integer multiples of its ulps, which
-    // reproduces the assembled FP64 values exactly (every row was checked
-    // when the mesh was added).
-    const SRS_GLOBAL int16_t* off = gp(d.off);
-    g.n1 = j + ldg<NTG>(off + j);
-    g.n2 = j + ldg<NTG>(off + np + j);
-    g.n3 = j + ldg<NTG>(off + 2 * np + j);
-    g.n4 = j + ldg<NTG>(off + 3 * np + j);
-    const SRS_GLOBAL double* t = gp(d.tab) + 10 * (int)ldg<NTG>(gp(d.cls) + j);
-    const SRS_GLOBAL int16_t* dq = gp(d.dq);
-    g.a0 = t[0] + (double)ldg<NTG>(dq + j) * t[5];
-    g.a1 = t[1] + (double)ldg<NTG>(dq + np + j) * t[6];
-    g.a2 = t[2] + (double)ldg<NTG>(dq + 2 * np + j) * t[7];
-    g.a3 = t[3] + (double)ldg<NTG>(dq + 3 * np + j) * t[8];
-    g.a4 = t[4] + (double)ldg<NTG>(dq + 4 * np + j) * t[9];
-  } else {
-    const SRS_GLOBAL double* cf = gp(d.coef);
-    const SRS_GLOBAL int32_t* nb = gp(d.nbr);
-    g.a0 = ldg<NTG>(cf + j); g.a1 = ldg<NTG>(cf + np + j); g.a2 = ldg<NTG>(cf + 2 * np + j);
-    g.a3 = ldg<NTG>(cf + 3 * np + j); g.a4 = ldg<NTG>(cf + 4 * np + j);
-    g.n1 = ldg<NTG>(nb + j); g.n2 = ldg<NTG>(nb + np + j); g.n3 = ldg<NTG>(nb + 2 * np + j);
-    g.n4 = ldg<NTG>(nb + 3 * np + j);
-  }
-  return g;
-}
+namespace srs_dev {
 
 // The state of one element of one realization: its own and its neighbours'
 // diffusing quads, TnC and CSQ where they are live.
@@ -768,431 +595,6 @@ __global__ void k_channel_ca(const RealDesc* __restrict__ D, const RealLive* __r
   for (int i = threadIdx.x; i < d.n_lcc; i += blockDim.x) out[2 * d.n_ryr + i] = Ca[cq(d, d.lcc_ele[i])];
 }
 
-// ------------------------------------------------------------------ linescan
-// linescan/main.cpp:195-249.  The blurred fluorescence of scan point i is
-// sum_j V0_j CaF_j w(c_j - p_i) over cells with |c_j - p_i|^2 < 1, with the
-// Gaussian PSF w of linescan/parameters.h:60-62.  Mesh cells are summed on the
-// device, one partial sum per workgroup and point, then the partials in
-// workgroup order; the padding cells' part is a per-point constant.
-struct ScanGeo {
-  double base[3];   // scan line point with the axis coordinate zero
-  int axis;
-  double x_min, res;
-  double sx, sy, sz, alpha;
-};
-
-constexpr int SCAN_BS = 256;
-constexpr int SCAN_MAX_POINTS = 1024;
-constexpr int SCAN_CELLS_PER_THREAD = 4;
-
-// Realization of workgroup row blockIdx.y in a batched launch: rid[blockIdx.y],
-// or r when there is no table (a launch for one realization).
-__device__ __forceinline__ int batch_real(const int32_t* __restrict__ rid, int r) { return rid ? rid[blockIdx.y] : r; }
-
-__global__ __launch_bounds__(SCAN_BS) void k_scan_blur(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
-                                                      int r, const int32_t* __restrict__ rid,
-                                                      const double* __restrict__ cen, ScanGeo g, int n_f,
-                                                      double* __restrict__ part) {
-  __shared__ double red[SCAN_BS / 64][SCAN_MAX_POINTS];
-  r = batch_real(rid, r);
-  part += (size_t)blockIdx.y * gridDim.x * n_f;
-  const RealDesc& d = D[r];
-  const double* X = d.X[L[r].cur];
-  const int n = d.n, np = d.np;
-  double cx[SCAN_CELLS_PER_THREAD], cy[SCAN_CELLS_PER_THREAD], cz[SCAN_CELLS_PER_THREAD], vf[SCAN_CELLS_PER_THREAD];
-  for (int q = 0; q < SCAN_CELLS_PER_THREAD; ++q) {
-    const int j = (blockIdx.x * SCAN_CELLS_PER_THREAD + q) * SCAN_BS + threadIdx.x;
-    if (j < n) {
-      cx[q] = cen[j]; cy[q] = cen[np + j]; cz[q] = cen[2 * np + j];
-      vf[q] = d.V0[j] * X[cq(d, j) + 3];  // V0 * Ca-bound Fluo
-    } else {
-      cx[q] = cy[q] = cz[q] = 1e30;  // beyond the 1 um cut-off of every point
-      vf[q] = 0.0;
-    }
-  }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = 0; i < n_f; ++i) {
-    double p[3] = {g.base[0], g.base[1], g.base[2]};
-    p[g.axis] = g.x_min + i * g.res;
-    double acc = 0.0;
-    for (int q = 0; q < SCAN_CELLS_PER_THREAD; ++q) {
-      const double rx = cx[q] - p[0], ry = cy[q] - p[1], rz = cz[q] - p[2];
-      const double dist = rx * rx + ry * ry + rz * rz;
-      if (dist < 1.0) {
-        const double w = g.alpha * exp(-0.5 * ((rx * rx / (g.sx * g.sx)) + (ry * ry / (g.sy * g.sy)) +
-                                               (rz * rz / (g.sz * g.sz)))) / (g.sx * g.sy * g.sz);
-        acc += vf[q] * w;
-      }
-    }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane == 0) red[wave][i] = acc;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < n_f; i += SCAN_BS) {
-    double s = 0.0;
-    for (int w = 0; w < SCAN_BS / 64; ++w) s += red[w][i];
-    part[(size_t)blockIdx.x * n_f + i] = s;
-  }
-}
-
-__global__ void k_scan_final(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int r,
-                             const int32_t* __restrict__ rid, int n_blocks, int n_f, int n_ca,
-                             const double* __restrict__ part, const double* __restrict__ pad,
-                             const int32_t* __restrict__ near_f, const int32_t* __restrict__ near_ca, double f0,
-                             double c0, double* __restrict__ out, int out_stride) {
-  r = batch_real(rid, r);
-  part += (size_t)blockIdx.y * n_blocks * n_f;
-  out += (size_t)blockIdx.y * out_stride;
-  const double* X = D[r].X[L[r].cur];
-  for (int i = threadIdx.x; i < n_f; i += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < n_blocks; ++b) s += part[(size_t)b * n_f + i];
-    out[n_ca + n_f + i] = s + pad[i];
-    const int e = near_f[i];
-    out[n_ca + i] = e >= 0 ? X[cq(D[r], e) + 3] : f0;
-  }
-  for (int i = threadIdx.x; i < n_ca; i += blockDim.x) {
-    const int e = near_ca[i];
-    out[i] = e >= 0 ? X[cq(D[r], e)] : c0;
-  }
-}
-
-// ------------------------------------------------------------------- adjoint
-// Reverse mode of the explicit step with the channel trajectory frozen
-// (RyR states from a recorded schedule, LCC states held): for
-// X_{n+1} = X_n + dt r(X_n; theta), lambda_n = lambda_{n+1} + dt J^T lambda_{n+1}
-// and dL/dtheta += dt (dr/dtheta)^T lambda_{n+1}.  The diffusion operator
-// satisfies V_i A_ij = V_j A_ji (both equal the face coefficient), so
-// (A^T y)_j = V_j (A (y / V))_j: the transpose is another gather over the same
-// rows.  Single-realization layout (quads, then the TnC and CSQ planes); a
-// launch covers the realizations of a table, one workgroup row (blockIdx.y)
-// each, with row y's adjoint, state and weight arrays at y times their size.
-constexpr int ADJ_NT = 13;   // D_CA, D_DYE, kon/koff of CaM, TnC, ATP, Fluo, V_RYR, A_P, K_D_SL_HI
-constexpr int ADJ_BS = 256;
-__constant__ const int kAdjTheta[ADJ_NT] = {SRS_R_D_CA,     SRS_R_D_DYE,     SRS_R_K_ON_CMDN, SRS_R_K_OFF_CMDN,
-                                            SRS_R_K_ON_TRPN, SRS_R_K_OFF_TRPN, SRS_R_K_ON_ATP, SRS_R_K_OFF_ATP,
-                                            SRS_R_K_ON_DYE, SRS_R_K_OFF_DYE, SRS_R_V_RYR,     SRS_R_A_P,
-                                            SRS_R_K_D_SL_HI};
-
-// 1 / beta of the rapid SL buffer at element j and its derivatives in c and
-// in K_D_SL_HI (cpu/RunSimulation.cpp:15-22).
-struct SlBuf { double beta, dbeta_dc, dbeta_dkh; };
-__device__ __forceinline__ SlBuf sl_buffer(const RealDesc& d, int j, double c, uint8_t f) {
-  SlBuf s{1.0, 0.0, 0.0};
-  if (!(f & F_SL)) return s;
-  const double* P = d.P;
-  const double ttj = d.tt[j], v0j = d.V0[j];
-  const double bhi = ttj * P[SRS_R_B_TOT_SL_HI] / (v0j * (1e-15));
-  const double blo = ttj * P[SRS_R_B_TOT_SL_LO] / (v0j * (1e-15));
-  const double kh = P[SRS_R_K_D_SL_HI], kl = P[SRS_R_K_D_SL_LO];
-  const double eh = kh + c, el = kl + c;
-  s.beta = 1 + (bhi * kh / (eh * eh)) + (blo * kl / (el * el));
-  s.dbeta_dc = -2 * bhi * kh / (eh * eh * eh) - 2 * blo * kl / (el * el * el);
-  s.dbeta_dkh = bhi * (c - kh) / (eh * eh * eh);
-  return s;
-}
-
-// The loss source of one scan sample at element j: sum_i gcoef_i w(c_j - p_i)
-// over points within 1 um (the blur's PSF, linescan/parameters.h:60-62).
-__device__ __forceinline__ double scan_adj_sum(const double* __restrict__ cen, const ScanGeo& g, int n_f,
-                                               const double* __restrict__ gcoef, size_t np, int j) {
-  const double cx = cen[j], cy = cen[np + j], cz = cen[2 * np + j];
-  double s = 0.0;
-  for (int i = 0; i < n_f; ++i) {
-    double p[3] = {g.base[0], g.base[1], g.base[2]};
-    p[g.axis] = g.x_min + i * g.res;
-    const double rx = cx - p[0], ry = cy - p[1], rz = cz - p[2];
-    if (rx * rx + ry * ry + rz * rz < 1.0)
-      s += gcoef[i] * (g.alpha * exp(-0.5 * ((rx * rx / (g.sx * g.sx)) + (ry * ry / (g.sy * g.sy)) +
-                                             (rz * rz / (g.sz * g.sz)))) / (g.sx * g.sy * g.sz));
-  }
-  return s;
-}
-
-// w = (lambda_Ca / beta, lambda_ATP, lambda_CaM, lambda_Fluo) / V0 of element j.
-__device__ __forceinline__ d4 adj_weight(const RealDesc& d, int j, double c, uint8_t f, double la_c, double la_a,
-                                         double la_m, double la_d) {
-  const SlBuf sb = sl_buffer(d, j, c, f);
-  const double iv = 1.0 / d.V0[j];
-  return d4{la_c / sb.beta * iv, la_a * iv, la_m * iv, la_d * iv};
-}
-
-// Pass 1 (the first step of a recomputed segment; later steps get w from the
-// step kernel): w = (lambda_Ca / beta, lambda_ATP, lambda_CaM, lambda_Fluo) / V0.
-__global__ __launch_bounds__(ADJ_BS) void k_adj_w(const RealDesc* __restrict__ D, const int32_t* __restrict__ rid,
-                                                  const double* __restrict__ X, const double* __restrict__ lam,
-                                                  d4* __restrict__ w) {
-  const RealDesc& d = D[rid[blockIdx.y]];
-  const int j = blockIdx.x * ADJ_BS + threadIdx.x;
-  if (j >= d.n) return;
-  X += blockIdx.y * 6 * (size_t)d.np;
-  lam += blockIdx.y * 6 * (size_t)d.np;
-  w += blockIdx.y * (size_t)d.np;
-  const d4 m = ((const d4*)lam)[j];
-  w[j] = adj_weight(d, j, X[4 * (size_t)j], d.flags[j], m.x, m.y, m.z, m.w);
-}
-
-// What the step kernel does after lambda_n: add the loss source of a scan
-// sample taken after n steps (SCAN), and form w for step n - 1 from lambda_n
-// and that step's state (Xprev, when it is in the recomputed segment).
-struct AdjNext {
-  const double* Xprev;  // state of step n - 1, or null
-  d4* wout;             // its w
-  const double* cen;    // scan cell centres [3][np]
-  const double* gcoef;  // the sample's loss coefficients per point
-  ScanGeo g;
-  int n_f, gstride;
-};
-
-// Pass 2: lambda_n and the theta-gradient partials of one step.
-template <bool COMPACT, bool SCAN>
-__global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
-                                                     const int32_t* __restrict__ rid, const double* __restrict__ X,
-                                                     const double* __restrict__ lam, const d4* __restrict__ w,
-                                                     int64_t step, double* __restrict__ lam_out,
-                                                     double* __restrict__ gpart, AdjNext nx) {
-  __shared__ double red[ADJ_BS / 64][ADJ_NT];
-  const int r = rid[blockIdx.y];
-  const RealDesc& d = D[r];
-  const RealLive& l = L[r];
-  const int j = blockIdx.x * ADJ_BS + threadIdx.x;
-  const size_t np = d.np;
-  X += blockIdx.y * 6 * np;
-  lam += blockIdx.y * 6 * np;
-  lam_out += blockIdx.y * 6 * np;
-  w += blockIdx.y * np;
-  gpart += (size_t)blockIdx.y * gridDim.x * ADJ_NT;
-  // the block's running partials, loaded now so the final update waits on no load
-  const double gp_old = threadIdx.x < ADJ_NT ? gpart[(size_t)blockIdx.x * ADJ_NT + threadIdx.x] : 0.0;
-  // RyR states during this step: row `step` of the frozen schedule
-  const int32_t* ryr = d.n_ryr ? d.sched + step * d.n_ryr : nullptr;
-  double gr[ADJ_NT];
-#pragma unroll
-  for (int k = 0; k < ADJ_NT; ++k) gr[k] = 0.0;
-  if (j < d.n) {
-    const double* P = d.P;
-    const double dt = P[SRS_R_T_STEP];
-    const Geo g = load_geo<false, COMPACT, -1>(d, j);
-    const uint8_t f = g.f;
-    const d4* Q = (const d4*)X;
-    const d4 m0 = Q[j], m1 = Q[g.n1], m2 = Q[g.n2], m3 = Q[g.n3], m4 = Q[g.n4];
-    const d4 w0 = w[j], w1 = w[g.n1], w2 = w[g.n2], w3 = w[g.n3], w4 = w[g.n4];
-    const d4 mu = ((const d4*)lam)[j];
-    const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
-    const bool live_cs = (f & F_JSR) != 0;
-    const double bT = live_tr ? X[4 * np + j] : 0.0, bC = live_cs ? X[5 * np + j] : 0.0;
-    const double muT = lam[4 * np + j], muC = lam[5 * np + j];
-    const double c = m0.x, bA = m0.y, bM = m0.z, bD = m0.w;
-    const double V0 = d.V0[j];
-    const SlBuf sb = sl_buffer(d, j, c, f);
-    const double u = mu.x / sb.beta;
-
-    // transposed diffusion: dt D_s V0 (A w_s)_j
-    const double aw_c = g.a0 * w0.x + g.a1 * w1.x + g.a2 * w2.x + g.a3 * w3.x + g.a4 * w4.x;
-    const double aw_a = g.a0 * w0.y + g.a1 * w1.y + g.a2 * w2.y + g.a3 * w3.y + g.a4 * w4.y;
-    const double aw_m = g.a0 * w0.z + g.a1 * w1.z + g.a2 * w2.z + g.a3 * w3.z + g.a4 * w4.z;
-    const double aw_d = g.a0 * w0.w + g.a1 * w1.w + g.a2 * w2.w + g.a3 * w3.w + g.a4 * w4.w;
-    double lc = mu.x + dt * P[SRS_R_D_CA] * V0 * aw_c;
-    double la = mu.y + dt * P[SRS_R_D_ATP] * V0 * aw_a;
-    double lm = mu.z + dt * P[SRS_R_D_CMDN] * V0 * aw_m;
-    double ld = mu.w + dt * P[SRS_R_D_DYE] * V0 * aw_d;
-    double lT = muT, lC = muC;
-
-    // forward Ca rate before the SL division (for the SL terms) and the
-    // diffusion sums (for the D gradients)
-    const double ac = c * g.a0 + m1.x * g.a1 + m2.x * g.a2 + m3.x * g.a3 + m4.x * g.a4;
-    const double ad = bD * g.a0 + m1.w * g.a1 + m2.w * g.a2 + m3.w * g.a3 + m4.w * g.a4;
-    double rca = P[SRS_R_D_CA] * ac + ((f & F_BR) ? d.br[j] : 0.0);
-    gr[0] += dt * u * ac;          // D_CA
-    gr[1] += dt * mu.w * ad;       // D_DYE
-    if (!(f & F_JSR)) {
-      if (f & F_TROPC) {
-        const double kon = P[SRS_R_K_ON_TRPN], koff = P[SRS_R_K_OFF_TRPN], B = P[SRS_R_B_TOT_TRPN];
-        const double r = kon * (B - bT) * c - koff * bT;
-        rca -= r;
-        lc += dt * (muT - u) * kon * (B - bT);
-        lT += dt * (muT - u) * (-kon * c - koff);
-        gr[4] += dt * (muT - u) * (B - bT) * c;
-        gr[5] += dt * (muT - u) * (-bT);
-        // SERCA, cpu/RunSimulation.cpp:82-90
-        const double kdi = P[SRS_R_K_D_I];
-        double K = c / kdi;
-        K = K * K;
-        const double dK = 2.0 * (c / kdi) / kdi;
-        const double Nn = K * K * 3.24873e12 + K * l.serca[3] - l.serca[4];
-        const double Dn = l.serca[0] + K * l.serca[1] + K * K * l.serca[2];
-        const double v = Nn / Dn;
-        const double dv = ((2 * K * 3.24873e12 + l.serca[3]) * Dn - Nn * (l.serca[1] + 2 * K * l.serca[2])) / (Dn * Dn);
-        rca -= (2e-3) * v * P[SRS_R_A_P];
-        lc += dt * (-u) * (2e-3) * P[SRS_R_A_P] * dv * dK;
-        gr[11] += dt * u * (-(2e-3) * v);
-      }
-      const double konD = P[SRS_R_K_ON_DYE], koffD = P[SRS_R_K_OFF_DYE], BD = P[SRS_R_B_TOT_DYE];
-      const double rD = konD * (BD - bD) * c - koffD * bD;
-      rca -= rD;
-      lc += dt * (mu.w - u) * konD * (BD - bD);
-      ld += dt * (mu.w - u) * (-konD * c - koffD);
-      gr[8] += dt * (mu.w - u) * (BD - bD) * c;
-      gr[9] += dt * (mu.w - u) * (-bD);
-      const double konA = P[SRS_R_K_ON_ATP], koffA = P[SRS_R_K_OFF_ATP], BA = P[SRS_R_B_TOT_ATP];
-      const double rA = konA * (BA - bA) * c - koffA * bA;
-      rca -= rA;
-      lc += dt * (mu.y - u) * konA * (BA - bA);
-      la += dt * (mu.y - u) * (-konA * c - koffA);
-      gr[6] += dt * (mu.y - u) * (BA - bA) * c;
-      gr[7] += dt * (mu.y - u) * (-bA);
-      const double konM = P[SRS_R_K_ON_CMDN], koffM = P[SRS_R_K_OFF_CMDN], BM = P[SRS_R_B_TOT_CMDN];
-      const double rM = konM * (BM - bM) * c - koffM * bM;
-      rca -= rM;
-      lc += dt * (mu.z - u) * konM * (BM - bM);
-      lm += dt * (mu.z - u) * (-konM * c - koffM);
-      gr[2] += dt * (mu.z - u) * (BM - bM) * c;
-      gr[3] += dt * (mu.z - u) * (-bM);
-    } else {
-      const double kon = P[SRS_R_K_ON_CSQN], koff = P[SRS_R_K_OFF_CSQN], B = P[SRS_R_B_T_JSR];
-      const double r = kon * (B - bC) * c - koff * bC;
-      rca -= r;
-      rca += (l.ca_nsr - c) * P[SRS_R_V_REFILL];
-      lc += dt * ((muC - u) * kon * (B - bC) - u * P[SRS_R_V_REFILL]);
-      lC += dt * (muC - u) * (-kon * c - koff);
-    }
-    if (f & F_CHAN) {
-      int lo = 0, hi = d.n_celem;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (d.celem[mid] < j) lo = mid + 1; else hi = mid;
-      }
-      if (lo < d.n_celem && d.celem[lo] == j) {
-        const double a = P[SRS_R_V_RYR] * P[SRS_R_V_CELL];
-        for (int e = d.cstart[lo]; e < d.cstart[lo + 1]; ++e) {
-          const int ch = d.centry[e];
-          if (ch >= 0) {
-            const int i = ch >> 1;
-            if (!ryr[i]) continue;
-            const int ce = d.ryr_ele[i], je = d.ryr_jsr[i];
-            const int p = (ch & 1) ? ce : je;  // the partner element
-            const double cp = X[4 * (size_t)p];
-            const double up = lam[4 * (size_t)p] / sl_buffer(d, p, cp, d.flags[p]).beta;
-            const double dc = X[4 * (size_t)je] - X[4 * (size_t)ce];
-            const double J = a * dc;
-            if (ch & 1) {  // jSR side: r -= J / V0_jsr
-              rca -= J / V0;
-              lc += dt * (u * (-a / V0) + up * (a / d.V0[p]));
-            } else {       // cleft side: r += J / V0_cyt
-              rca += J / V0;
-              lc += dt * (u * (-a / V0) + up * (a / d.V0[p]));
-              gr[10] += dt * (u * P[SRS_R_V_CELL] * dc / V0 - up * P[SRS_R_V_CELL] * dc / d.V0[p]);
-            }
-          } else {
-            const int i = -ch - 1;
-            if (srs_lcc_open(d.lcc_state[i], d.lccv_state[i])) {
-              rca += srs_lcc_flux_raw(l.vm, c, P[SRS_R_CA_0]) / V0;
-              lc += dt * u * srs_lcc_flux_raw(l.vm, 1.0, 0.0) / V0;
-            }
-          }
-        }
-      }
-    }
-    if (f & F_SL) {
-      const double b2 = sb.beta * sb.beta;
-      lc += dt * mu.x * (-rca * sb.dbeta_dc / b2);
-      gr[12] += dt * mu.x * (-rca * sb.dbeta_dkh / b2);
-    }
-    if (SCAN) ld += V0 * scan_adj_sum(nx.cen, nx.g, nx.n_f, nx.gcoef + (size_t)blockIdx.y * nx.gstride, np, j);
-    ((d4*)lam_out)[j] = d4{lc, la, lm, ld};
-    lam_out[4 * np + j] = lT;
-    lam_out[5 * np + j] = lC;
-    if (nx.Xprev) {
-      const double* Xp = nx.Xprev + blockIdx.y * 6 * np;
-      nx.wout[blockIdx.y * np + j] = adj_weight(d, j, Xp[4 * (size_t)j], f, lc, la, lm, ld);
-    }
-  }
-  // block reduction of the gradient partials, accumulated per block
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-  for (int k = 0; k < ADJ_NT; ++k) {
-    double v = gr[k];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane == 0) red[wave][k] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < ADJ_NT) {
-    double s = 0.0;
-    for (int q = 0; q < ADJ_BS / 64; ++q) s += red[q][threadIdx.x];
-    gpart[(size_t)blockIdx.x * ADJ_NT + threadIdx.x] = gp_old + s;
-  }
-}
-
-// The loss gradient of one scan sample in the Fluo adjoint:
-// lambda_Fluo,j += V0_j sum_i gcoef_i w(c_j - p_i) over points within 1 um.
-__global__ __launch_bounds__(ADJ_BS) void k_scan_adj(const RealDesc* __restrict__ D, const int32_t* __restrict__ rid,
-                                                     const double* __restrict__ cen, ScanGeo g, int n_f,
-                                                     const double* __restrict__ gcoef, int gstride,
-                                                     double* __restrict__ lam) {
-  const RealDesc& d = D[rid[blockIdx.y]];
-  const int j = blockIdx.x * ADJ_BS + threadIdx.x;
-  if (j >= d.n) return;
-  const size_t np = d.np;
-  lam += blockIdx.y * 6 * np;
-  lam[4 * (size_t)j + 3] += d.V0[j] * scan_adj_sum(cen, g, n_f, gcoef + (size_t)blockIdx.y * gstride, np, j);
-}
-
-// Copies the current state (X[cur]) of each realization of a table to slot y
-// of a contiguous buffer (to_buf) or back.
-__global__ void k_batch_state(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
-                              const int32_t* __restrict__ rid, double* __restrict__ buf, int to_buf) {
-  const int r = rid[blockIdx.y];
-  const RealDesc& d = D[r];
-  const size_t n6 = 6 * (size_t)d.np;
-  double* X = d.X[L[r].cur];
-  double* B = buf + blockIdx.y * n6;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n6; i += (size_t)gridDim.x * blockDim.x) {
-    if (to_buf) B[i] = X[i];
-    else X[i] = B[i];
-  }
-}
-
-// Points the ping-pong buffers of each realization y of a table at slots of a
-// recomputed segment (slot i of y at seg + (i * nb + y) * 6 np): the current
-// state at slot step - s0, the next one at the slot after, so the sweep writes
-// the segment in place.
-__global__ void k_seg_ptrs(RealDesc* __restrict__ D, const RealLive* __restrict__ L, const int32_t* __restrict__ rid,
-                           double* seg, int64_t s0, int nb) {
-  const int y = blockIdx.x * blockDim.x + threadIdx.x;
-  if (y >= nb) return;
-  const int r = rid[y];
-  const int cur = L[r].cur;
-  const size_t n6 = 6 * (size_t)D[r].np;
-  const size_t i = (size_t)(L[r].step - s0);
-  D[r].X[cur] = seg + (i * nb + y) * n6;
-  D[r].X[cur ^ 1] = seg + ((i + 1) * nb + y) * n6;
-}
-
-// Starts a replayed segment at `step` for each realization of a table: the
-// step counter, schedule rows counted from zero, RyR states of row `step`.
-__global__ void k_seg_start(const RealDesc* __restrict__ D, RealLive* __restrict__ L, const int32_t* __restrict__ rid,
-                            int64_t step) {
-  const int r = rid[blockIdx.x];
-  const RealDesc& d = D[r];
-  for (int i = threadIdx.x; i < d.n_ryr; i += blockDim.x) d.ryr_state[i] = d.sched[step * d.n_ryr + i];
-  if (threadIdx.x == 0) {
-    L[r].step = step;
-    L[r].sched_base = 0;
-  }
-}
-
-// Writes each step's RyR states (the ones the next step's fluxes use) into a
-// schedule, for srs_record_schedule.
-__global__ void k_sched_out(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
-                            const int64_t* __restrict__ base, const size_t* __restrict__ off, int32_t* out,
-                            int64_t n_rows) {
-  const int r = blockIdx.x;
-  const RealDesc& d = D[r];
-  const int64_t row = L[r].step - base[r];  // k_gate already advanced the step counter
-  if (row < 1 || row >= n_rows) return;
-  out += off[r];
-  for (int i = threadIdx.x; i < d.n_ryr; i += blockDim.x) out[row * d.n_ryr + i] = d.ryr_state[i];
-}
 
 // Moves one realization's state between the single-realization layout
 // (np quads, TnC[np], CSQ[np]) and its slot of a group buffer.
@@ -1209,38 +611,6 @@ __global__ void k_slot_copy(const double* __restrict__ src, double* __restrict__
 }
 
 // ----------------------------------------------------------------- host side
-
-#define HIPCHK(expr)                                                                  \
-  do {                                                                                \
-    hipError_t _e = (expr);                                                           \
-    if (_e != hipSuccess) return srs::fail(SRS_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
-  } while (0)
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-struct HostMesh {
-  int32_t n = 0, np = 0;
-  std::vector<int32_t> perm;  // dev -> ref
-  std::vector<int32_t> inv;   // ref -> dev
-  std::vector<int32_t> nbr;   // [4][np] dev numbering
-  std::vector<double> coef;   // [5][np]
-  std::vector<uint8_t> flags;
-  std::vector<double> V0, tt, br;  // br [4][np] or empty
-  // compact operator layout
-  bool compact = false;
-  std::vector<int16_t> off, dq;
-  std::vector<uint16_t> cls;
-  std::vector<double> tab;
-  int32_t n_class = 0;
-  int32_t *d_nbr = nullptr;
-  double *d_coef = nullptr, *d_V0 = nullptr, *d_tt = nullptr, *d_br = nullptr, *d_tab = nullptr;
-  uint8_t* d_flags = nullptr;
-  int16_t *d_off = nullptr, *d_dq = nullptr;
-  uint16_t* d_cls = nullptr;
-};
 
 // Rounds the significand to `bits` bits (class key of a coefficient).
 double quantize(double a, int bits) {
@@ -1320,86 +690,6 @@ void serca_terms(double nsr, const double* P, double* out) {
   out[2] = 6.08463e11 + K_sr * 4.50544e11;
   out[3] = 9.17846e6 - 11478.2 * K_sr;
   out[4] = 0.329904 * K_sr;
-}
-
-struct HostReal {
-  int32_t mesh = 0;
-  std::vector<int32_t> ryr_ele, ryr_jsr, ryr_nb, lcc_ele;  // dev numbering
-  std::vector<int32_t> celem, cstart, centry;
-  int32_t ints[SRS_N_INT];
-  double reals[SRS_N_REAL];
-  uint64_t key = 0, stream = 0;
-  int32_t group = 0;  // realization group (index into the sweep's group table)
-  RealDesc desc{};
-};
-
-struct HostScan {
-  int32_t mesh = 0, n_f = 0, n_ca = 0, n_blocks = 0;
-  srs_scan_desc desc{};
-  ScanGeo geo{};
-  double *d_cen = nullptr, *d_pad = nullptr, *d_part = nullptr, *d_out = nullptr;
-  int32_t *d_near_f = nullptr, *d_near_ca = nullptr;
-};
-
-}  // namespace
-
-struct srs_ctx {
-  int device = 0;
-  int tile = 256;        // elements per k_sweep workgroup
-  int nt_geometry = 0;   // non-temporal loads of operator rows and flags
-  int nt_state = 0;      // non-temporal stores of the new state
-  int plane_pad = 0;     // extra elements per SoA plane
-  int compact = 1;       // compress operator rows when a mesh allows it
-  int occ8 = 0;          // sweep compiled for eight wavefronts per SIMD
-  int group = 1;         // realizations per realization group (1, 2 or 4)
-  int lds = 0;           // stage each tile's quads in LDS for the neighbour gathers (groups of 1)
-  int drop_finished = 1; // sweep without the tiles of groups that have all finished
-  bool compact_all = false;
-  bool layout_fixed = false;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool finalized = false;
-  std::vector<std::unique_ptr<HostMesh>> meshes;
-  std::vector<HostReal> reals;
-  std::vector<void*> allocs;
-  RealDesc* d_desc = nullptr;
-  RealLive* d_live = nullptr;
-  int2* d_tiles = nullptr;   // {group, tile} of the sweep
-  int n_tiles = 0, grid_sweep = 0;
-  // Every group's tiles; when the groups whose members have all finished hold
-  // enough of them, the sweep runs over a list without them (d_tiles_live).
-  std::vector<int2> tiles_all;
-  int2* d_tiles_all = nullptr;
-  int2* d_tiles_live = nullptr;
-  bool tiles_compacted = false;
-  int32_t* d_groups = nullptr;  // [n_groups][group] realization ids, -1 pads
-  std::vector<int32_t> groups;
-  double* stage = nullptr;       // staging buffer for state transfers of grouped realizations
-  size_t stage_len = 0;
-  double last_total_ms = 0, last_sweep_ms = 0;
-  // captured step blocks, keyed by (steps, gate mode)
-  std::map<std::pair<int64_t, uint32_t>, hipGraphExec_t> graphs;
-  int32_t* d_sched = nullptr;
-  std::vector<HostScan> scans;
-};
-
-namespace {
-
-template <class T>
-int dev_alloc(srs_ctx* c, T** p, size_t count) {
-  *p = nullptr;
-  if (count == 0) return SRS_OK;
-  HIPCHK(hipMalloc((void**)p, count * sizeof(T)));
-  c->allocs.push_back((void*)*p);
-  return SRS_OK;
-}
-
-template <class T>
-int dev_upload(srs_ctx* c, T** p, const std::vector<T>& v) {
-  int rc = dev_alloc(c, p, v.size());
-  if (rc) return rc;
-  if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  return SRS_OK;
 }
 
 int check_real(srs_ctx* c, int32_t r) {
@@ -1514,21 +804,56 @@ int tiles_compact(srs_ctx* c) {
   return SRS_OK;
 }
 
-}  // namespace
 
-namespace {
-// Temporary device buffers freed when the call returns.
-struct DevMem {
-  std::vector<void*> ptrs;
-  ~DevMem() { for (void* p : ptrs) (void)hipFree(p); }
-  template <class T>
-  hipError_t alloc(T** p, size_t count) {
-    hipError_t e = hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
-    if (e == hipSuccess) ptrs.push_back((void*)*p);
-    return e;
+// Copies one realization's state between a host buffer in the single
+// realization layout (np quads, then the TnC and CSQ planes) and its slot of
+// the group buffer: contiguous through a device staging buffer, then a
+// scatter (or gather) kernel for the slot.
+int slot_copy(srs_ctx* c, const RealDesc& d, double* dev, double* host, bool to_dev) {
+  const size_t np = d.np, n6 = 6 * np;
+  const hipMemcpyKind kind = to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  if (d.kst == 1) {
+    if (to_dev) HIPCHK(hipMemcpy(dev, host, n6 * sizeof(double), kind));
+    else HIPCHK(hipMemcpy(host, dev, n6 * sizeof(double), kind));
+    return SRS_OK;
   }
-};
-}  // namespace
+  if (c->stage_len < n6) {
+    if (c->stage) HIPCHK(hipFree(c->stage));
+    c->stage = nullptr;
+    c->stage_len = 0;
+    HIPCHK(hipMalloc((void**)&c->stage, n6 * sizeof(double)));
+    c->stage_len = n6;
+  }
+  const unsigned blocks = (unsigned)((n6 + 255) / 256);
+  if (to_dev) {
+    HIPCHK(hipMemcpy(c->stage, host, n6 * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_slot_copy, dim3(blocks), dim3(256), 0, c->stream, (const double*)c->stage, dev, np, d.kst,
+                       d.slot, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    hipLaunchKernelGGL(k_slot_copy, dim3(blocks), dim3(256), 0, c->stream, (const double*)dev, c->stage, np, d.kst,
+                       d.slot, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(host, c->stage, n6 * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return SRS_OK;
+}
+
+int live_rw(srs_ctx* c, int32_t r, RealLive* lv, bool write) {
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (write) {
+    int rc = tiles_full(c);
+    if (rc) return rc;
+  }
+  if (write) HIPCHK(hipMemcpy(c->d_live + r, lv, sizeof(*lv), hipMemcpyHostToDevice));
+  else HIPCHK(hipMemcpy(lv, c->d_live + r, sizeof(*lv), hipMemcpyDeviceToHost));
+  return SRS_OK;
+}
+
+}  // namespace srs_dev
 
 extern "C" {
 
@@ -1862,41 +1187,6 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
   return SRS_OK;
 }
 
-// Copies one realization's state between a host buffer in the single
-// realization layout (np quads, then the TnC and CSQ planes) and its slot of
-// the group buffer: contiguous through a device staging buffer, then a
-// scatter (or gather) kernel for the slot.
-static int slot_copy(srs_ctx* c, const RealDesc& d, double* dev, double* host, bool to_dev) {
-  const size_t np = d.np, n6 = 6 * np;
-  const hipMemcpyKind kind = to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
-  if (d.kst == 1) {
-    if (to_dev) HIPCHK(hipMemcpy(dev, host, n6 * sizeof(double), kind));
-    else HIPCHK(hipMemcpy(host, dev, n6 * sizeof(double), kind));
-    return SRS_OK;
-  }
-  if (c->stage_len < n6) {
-    if (c->stage) HIPCHK(hipFree(c->stage));
-    c->stage = nullptr;
-    c->stage_len = 0;
-    HIPCHK(hipMalloc((void**)&c->stage, n6 * sizeof(double)));
-    c->stage_len = n6;
-  }
-  const unsigned blocks = (unsigned)((n6 + 255) / 256);
-  if (to_dev) {
-    HIPCHK(hipMemcpy(c->stage, host, n6 * sizeof(double), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_slot_copy, dim3(blocks), dim3(256), 0, c->stream, (const double*)c->stage, dev, np, d.kst,
-                       d.slot, 1);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
-  } else {
-    hipLaunchKernelGGL(k_slot_copy, dim3(blocks), dim3(256), 0, c->stream, (const double*)dev, c->stage, np, d.kst,
-                       d.slot, 0);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(host, c->stage, n6 * sizeof(double), hipMemcpyDeviceToHost));
-  }
-  return SRS_OK;
-}
 
 int srs_state_set(srs_ctx* c, int32_t r, const double* X) {
   int rc = check_real(c, r);
@@ -1957,17 +1247,6 @@ int srs_channels_get(srs_ctx* c, int32_t r, int32_t* ryr_state, int32_t* lcc_sta
   return SRS_OK;
 }
 
-static int live_rw(srs_ctx* c, int32_t r, RealLive* lv, bool write) {
-  HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  if (write) {
-    int rc = tiles_full(c);
-    if (rc) return rc;
-  }
-  if (write) HIPCHK(hipMemcpy(c->d_live + r, lv, sizeof(*lv), hipMemcpyHostToDevice));
-  else HIPCHK(hipMemcpy(lv, c->d_live + r, sizeof(*lv), hipMemcpyDeviceToHost));
-  return SRS_OK;
-}
 
 int srs_globals_set(srs_ctx* c, int32_t r, double vm, double ca_nsr) {
   int rc = check_real(c, r);
@@ -2339,546 +1618,6 @@ int srs_time_sweep(srs_ctx* c, int32_t reps, double* sweep_ms, double* gating_ms
   if (sweep_ms) *sweep_ms = s / reps;
   if (gating_ms) *gating_ms = g / reps;
   return SRS_OK;
-}
-
-// ----------------------------------------------------------------- linescan
-
-int srs_ctx_add_scan(srs_ctx* c, int32_t mesh_id, const double* centers, const srs_scan_desc* sd, int32_t* scan_id,
-                     int32_t* n_f_out, int32_t* n_ca_out) {
-  srs::clear_error();
-  if (!c || !centers || !sd || !scan_id) return srs::fail(SRS_E_ARG, "null argument");
-  if (!c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
-  if (mesh_id < 0 || mesh_id >= (int)c->meshes.size()) return srs::fail(SRS_E_ARG, "bad mesh id %d", mesh_id);
-  if (sd->axis < 0 || sd->axis > 2) return srs::fail(SRS_E_ARG, "scan axis must be 0, 1 or 2");
-  if (!(sd->res_x_f > 0) || !(sd->res_x_ca > 0) || !(sd->pad_res > 0) || !(sd->var[0] > 0) || !(sd->var[1] > 0) ||
-      !(sd->var[2] > 0))
-    return srs::fail(SRS_E_ARG, "scan resolutions and PSF variances must be positive");
-  const HostMesh& hm = *c->meshes[mesh_id];
-  HostScan S;
-  S.mesh = mesh_id;
-  S.desc = *sd;
-  // linescan/main.cpp:143-152
-  S.n_f = (int)((sd->x_max - sd->x_min) / sd->res_x_f);
-  S.n_ca = (int)((sd->x_max - sd->x_min) / sd->res_x_ca);
-  if (S.n_f < 0 || S.n_ca < 0 || S.n_f > SCAN_MAX_POINTS || S.n_ca > (1 << 20))
-    return srs::fail(SRS_E_ARG, "scan has %d fluorescence points (at most %d) and %d Ca points", S.n_f,
-                     SCAN_MAX_POINTS, S.n_ca);
-  // axis mapping and PSF widths, linescan/main.cpp:169-194
-  ScanGeo& g = S.geo;
-  g.base[0] = g.base[1] = g.base[2] = 0.0;
-  g.axis = sd->axis;
-  g.x_min = sd->x_min;
-  g.res = sd->res_x_f;
-  if (sd->axis == 0) { g.base[1] = sd->offset1; g.base[2] = sd->offset2; }
-  else if (sd->axis == 1) { g.base[0] = sd->offset1; g.base[2] = sd->offset2; }
-  else { g.base[0] = sd->offset1; g.base[1] = sd->offset2; }
-  if (sd->axis == 2) { g.sx = std::sqrt(sd->var[2]); g.sy = std::sqrt(sd->var[0]); g.sz = std::sqrt(sd->var[1]); }
-  else { g.sx = std::sqrt(sd->var[0]); g.sy = std::sqrt(sd->var[1]); g.sz = std::sqrt(sd->var[2]); }
-  g.alpha = 1.0 / std::pow(2 * 3.14159, 1.5);  // the reference's pi, linescan/parameters.h:45-46
-
-  // padding cells, linescan/main.cpp:119-141 and AddPointGrid :268-306
-  std::vector<std::array<double, 3>> pc;
-  std::vector<double> pv;
-  auto add_grid = [&](double xmin, double xmax, double ymin, double ymax, double zmin, double zmax) {
-    double dx = sd->pad_res, dy = sd->pad_res, dz = sd->pad_res;
-    if (xmin > xmax) dx = -dx;
-    if (ymin > ymax) dy = -dy;
-    if (zmin > zmax) dz = -dz;
-    const int nx = (int)((xmax - xmin) / dx), ny = (int)((ymax - ymin) / dy), nz = (int)((zmax - zmin) / dz);
-    for (int i = 0; i < nx; i++)
-      for (int j = 0; j < ny; j++)
-        for (int k = 0; k < nz; k++) {
-          pc.push_back({xmin + i * dx + 0.5 * dx, ymin + j * dy + 0.5 * dy, zmin + k * dz + 0.5 * dz});
-          pv.push_back(std::fabs(dx * dy * dz));
-        }
-  };
-  const double pmin = sd->pad_min, pmax = sd->pad_max;
-  const double xlo = -sd->domain[0] / 2, xhi = sd->domain[0] / 2, ylo = -sd->domain[1] / 2, yhi = sd->domain[1] / 2;
-  const double zlo = -sd->domain[2] / 2, zhi = sd->domain[2] / 2;
-  add_grid(pmin, pmax, pmin, pmax, zhi, pmax);
-  add_grid(pmin, pmax, pmin, pmax, zlo, pmin);
-  add_grid(xlo, pmin, pmin, pmax, zlo, zhi);
-  add_grid(xhi, pmax, pmin, pmax, zlo, zhi);
-  add_grid(xlo, xhi, ylo, pmin, zlo, zhi);
-  add_grid(xlo, xhi, yhi, pmax, zlo, zhi);
-
-  // Nearest cell of each point (first minimum over mesh cells in reference
-  // order, then padding cells), and the padding part of the blurred sum.
-  const int n = hm.n, np = hm.np;
-  auto nearest = [&](const double p[3]) -> int32_t {
-    double rx = centers[0] - p[0], ry = centers[1] - p[1], rz = centers[2] - p[2];
-    double r_min = rx * rx + ry * ry + rz * rz;
-    int best = 0;
-    for (int j = 1; j < n; ++j) {
-      rx = centers[3 * (size_t)j] - p[0]; ry = centers[3 * (size_t)j + 1] - p[1]; rz = centers[3 * (size_t)j + 2] - p[2];
-      const double dist = rx * rx + ry * ry + rz * rz;
-      if (dist < r_min) { r_min = dist; best = j; }
-    }
-    int pbest = -1;
-    for (size_t k = 0; k < pc.size(); ++k) {
-      rx = pc[k][0] - p[0]; ry = pc[k][1] - p[1]; rz = pc[k][2] - p[2];
-      const double dist = rx * rx + ry * ry + rz * rz;
-      if (dist < r_min) { r_min = dist; pbest = (int)k; }
-    }
-    return pbest >= 0 ? -1 : hm.inv[best];
-  };
-  std::vector<int32_t> near_f(S.n_f), near_ca(S.n_ca);
-  std::vector<double> pad(S.n_f);
-  for (int i = 0; i < S.n_f; ++i) {
-    double p[3] = {g.base[0], g.base[1], g.base[2]};
-    p[g.axis] = sd->x_min + i * sd->res_x_f;
-    near_f[i] = nearest(p);
-    double sum = 0.0;
-    for (size_t k = 0; k < pc.size(); ++k) {
-      const double rx = pc[k][0] - p[0], ry = pc[k][1] - p[1], rz = pc[k][2] - p[2];
-      if (rx * rx + ry * ry + rz * rz < 1.0)
-        sum += pv[k] * sd->f0 *
-               (g.alpha * std::exp(-0.5 * ((rx * rx / (g.sx * g.sx)) + (ry * ry / (g.sy * g.sy)) + (rz * rz / (g.sz * g.sz)))) /
-                (g.sx * g.sy * g.sz));
-    }
-    pad[i] = sum;
-  }
-  for (int i = 0; i < S.n_ca; ++i) {
-    double p[3] = {g.base[0], g.base[1], g.base[2]};
-    p[g.axis] = sd->x_min + i * sd->res_x_ca;
-    near_ca[i] = nearest(p);
-  }
-  std::vector<double> cen(3 * (size_t)np, 1e30);
-  for (int k = 0; k < n; ++k)
-    for (int q = 0; q < 3; ++q) cen[(size_t)q * np + k] = centers[3 * (size_t)hm.perm[k] + q];
-  S.n_blocks = (n + SCAN_BS * SCAN_CELLS_PER_THREAD - 1) / (SCAN_BS * SCAN_CELLS_PER_THREAD);
-  HIPCHK(hipSetDevice(c->device));
-  int rc;
-  if ((rc = dev_upload(c, &S.d_cen, cen))) return rc;
-  if ((rc = dev_upload(c, &S.d_pad, pad))) return rc;
-  if ((rc = dev_upload(c, &S.d_near_f, near_f))) return rc;
-  if ((rc = dev_upload(c, &S.d_near_ca, near_ca))) return rc;
-  if ((rc = dev_alloc(c, &S.d_part, (size_t)S.n_blocks * std::max(S.n_f, 1)))) return rc;
-  if ((rc = dev_alloc(c, &S.d_out, (size_t)S.n_ca + 2 * (size_t)S.n_f + 1))) return rc;
-  *scan_id = (int32_t)c->scans.size();
-  c->scans.push_back(S);
-  if (n_f_out) *n_f_out = S.n_f;
-  if (n_ca_out) *n_ca_out = S.n_ca;
-  return SRS_OK;
-}
-
-int srs_scan_sample(srs_ctx* c, int32_t r, int32_t scan_id, double* ca, double* caf, double* cafb) {
-  int rc = check_real(c, r);
-  if (rc) return rc;
-  if (scan_id < 0 || scan_id >= (int)c->scans.size()) return srs::fail(SRS_E_ARG, "bad scan id %d", scan_id);
-  const HostScan& S = c->scans[scan_id];
-  if (c->reals[r].mesh != S.mesh) return srs::fail(SRS_E_ARG, "realization %d is not on the scan's mesh", r);
-  HIPCHK(hipSetDevice(c->device));
-  if (S.n_f > 0)
-    hipLaunchKernelGGL(k_scan_blur, dim3(S.n_blocks), dim3(SCAN_BS), 0, c->stream, c->d_desc, c->d_live, r, nullptr,
-                       S.d_cen, S.geo, S.n_f, S.d_part);
-  hipLaunchKernelGGL(k_scan_final, dim3(1), dim3(256), 0, c->stream, c->d_desc, c->d_live, r, nullptr, S.n_blocks,
-                     S.n_f, S.n_ca, S.d_part, S.d_pad, S.d_near_f, S.d_near_ca, S.desc.f0, S.desc.c0, S.d_out, 0);
-  HIPCHK(hipGetLastError());
-  std::vector<double> out((size_t)S.n_ca + 2 * (size_t)S.n_f);
-  HIPCHK(hipMemcpyAsync(out.data(), S.d_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  if (ca) std::copy(out.begin(), out.begin() + S.n_ca, ca);
-  if (caf) std::copy(out.begin() + S.n_ca, out.begin() + S.n_ca + S.n_f, caf);
-  if (cafb) std::copy(out.begin() + S.n_ca + S.n_f, out.end(), cafb);
-  return SRS_OK;
-}
-
-// ----------------------------------------------------------------- adjoint
-
-int srs_adjoint_theta(int32_t* idx) {
-  srs::clear_error();
-  if (!idx) return srs::fail(SRS_E_ARG, "null output");
-  const int t[ADJ_NT] = {SRS_R_D_CA,     SRS_R_D_DYE,     SRS_R_K_ON_CMDN, SRS_R_K_OFF_CMDN, SRS_R_K_ON_TRPN,
-                         SRS_R_K_OFF_TRPN, SRS_R_K_ON_ATP, SRS_R_K_OFF_ATP, SRS_R_K_ON_DYE,  SRS_R_K_OFF_DYE,
-                         SRS_R_V_RYR,     SRS_R_A_P,       SRS_R_K_D_SL_HI};
-  for (int k = 0; k < ADJ_NT; ++k) idx[k] = t[k];
-  return SRS_N_THETA == ADJ_NT ? SRS_OK : srs::fail(SRS_E_STATE, "theta count mismatch");
-}
-
-int srs_params_set(srs_ctx* c, int32_t r, const double* reals) {
-  int rc = check_real(c, r);
-  if (rc) return rc;
-  if (!reals) return srs::fail(SRS_E_ARG, "null parameters");
-  HostReal& hr = c->reals[r];
-  memcpy(hr.reals, reals, sizeof(hr.reals));
-  memcpy(hr.desc.P, reals, sizeof(hr.desc.P));
-  HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpy(c->d_desc + r, &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice));
-  RealLive lv;
-  if ((rc = live_rw(c, r, &lv, false))) return rc;
-  serca_terms(lv.ca_nsr, hr.reals, lv.serca);
-  return live_rw(c, r, &lv, true);
-}
-
-static int single_real(srs_ctx* c, int32_t r) {
-  int rc = check_real(c, r);
-  if (rc) return rc;
-  if (c->reals.size() != 1 || c->group != 1)
-    return srs::fail(SRS_E_ARG, "this call needs a context with one realization and group size 1");
-  return SRS_OK;
-}
-
-// n_steps gated steps of the whole context, recording every realization's RyR
-// states per step: realization r's n_steps x n_ryr rows at out + off[r].
-static int record_run(srs_ctx* c, int64_t n_steps, int32_t* out_host) {
-  const int n_real = (int)c->reals.size();
-  HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  std::vector<RealLive> lv(n_real);
-  HIPCHK(hipMemcpy(lv.data(), c->d_live, n_real * sizeof(RealLive), hipMemcpyDeviceToHost));
-  std::vector<int64_t> base(n_real);
-  std::vector<size_t> off(n_real + 1, 0);
-  for (int r = 0; r < n_real; ++r) {
-    if (lv[r].done) return srs::fail(SRS_E_ARG, "record schedule: realization %d has finished its protocol", r);
-    base[r] = lv[r].step;
-    off[r + 1] = off[r] + (size_t)n_steps * c->reals[r].ryr_ele.size();
-  }
-  DevMem M;
-  int32_t* d_out = nullptr;
-  int64_t* d_base = nullptr;
-  size_t* d_off = nullptr;
-  HIPCHK(M.alloc(&d_out, off[n_real]));
-  HIPCHK(M.alloc(&d_base, (size_t)n_real));
-  HIPCHK(M.alloc(&d_off, (size_t)n_real));
-  HIPCHK(hipMemcpy(d_base, base.data(), n_real * sizeof(int64_t), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(d_off, off.data(), n_real * sizeof(size_t), hipMemcpyHostToDevice));
-  for (int r = 0; r < n_real; ++r)
-    if (off[r + 1] > off[r])
-      HIPCHK(hipMemcpy(d_out + off[r], c->reals[r].desc.ryr_state, c->reals[r].ryr_ele.size() * sizeof(int32_t),
-                       hipMemcpyDeviceToDevice));
-  hipError_t e = hipSuccess;
-  for (int64_t k = 0; k < n_steps && e == hipSuccess; ++k) {
-    launch_step(c, G_AFTER_SWEEP);
-    hipLaunchKernelGGL(k_sched_out, dim3(n_real), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_base, d_off,
-                       d_out, n_steps);
-    e = hipGetLastError();
-    if (e == hipSuccess && (k & 1023) == 1023) e = hipStreamSynchronize(c->stream);
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e == hipSuccess) e = hipMemcpy(out_host, d_out, off[n_real] * sizeof(int32_t), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return srs::fail(SRS_E_HIP, "record schedule: %s", hipGetErrorString(e));
-  return SRS_OK;
-}
-
-int srs_record_schedule(srs_ctx* c, int32_t r, int64_t n_steps, int32_t* schedule) {
-  int rc = single_real(c, r);
-  if (rc) return rc;
-  if (n_steps <= 0) return SRS_OK;
-  if (c->reals[r].ryr_ele.empty()) return srs::fail(SRS_E_ARG, "realization has no RyRs");
-  if (!schedule) return srs::fail(SRS_E_ARG, "null schedule");
-  return record_run(c, n_steps, schedule);
-}
-
-int srs_record_schedules(srs_ctx* c, int64_t n_steps, int32_t* schedules) {
-  if (!c) return srs::fail(SRS_E_ARG, "null context");
-  if (!c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
-  if (n_steps <= 0) return SRS_OK;
-  if (!schedules) return srs::fail(SRS_E_ARG, "null schedules");
-  return record_run(c, n_steps, schedules);
-}
-
-// Line-scan loss and adjoint gradient of the realizations `ids` (all on the
-// scan's mesh, group size 1; arguments already checked).  Members run in
-// sub-batches of at most `max_batch` that step together; every other
-// realization is held (done) meanwhile, and all are left as they were.
-// Schedules are each member's n_steps x n_ryr rows, concatenated in member
-// order; targets are [member][sample][point].
-static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t scan_id, int64_t n_steps,
-                         const int32_t* schedules, int32_t n_samples, const int64_t* sample_steps,
-                         const double* targets, int32_t ckpt, int32_t max_batch, double* losses, double* grads,
-                         double* scans) {
-  const HostScan& S = c->scans[scan_id];
-  const HostMesh& hm = *c->meshes[S.mesh];
-  const size_t np = hm.np, n6 = 6 * np;
-  const int nf = S.n_f, nca = S.n_ca, nout = nca + 2 * nf;
-  const int n_mem = (int)ids.size(), n_real = (int)c->reals.size();
-  const bool want_grad = grads != nullptr;
-  const int64_t n_ck = (n_steps + ckpt - 1) / ckpt;
-  const int64_t n_ck_kept = want_grad ? n_ck : 1;  // a loss alone keeps only the start state
-  const int nblk = (hm.n + ADJ_BS - 1) / ADJ_BS;
-  std::vector<size_t> soff(n_mem + 1, 0);
-  size_t max_sched = 0;
-  for (int m = 0; m < n_mem; ++m) {
-    const size_t k = (size_t)n_steps * c->reals[ids[m]].ryr_ele.size();
-    soff[m + 1] = soff[m] + k;
-    max_sched = std::max(max_sched, k);
-  }
-  // device bytes per member of a sub-batch
-  const size_t per = 8 * n6 * (size_t)(n_ck_kept + (want_grad ? ckpt + 2 : 0)) + (want_grad ? 64 * np : 0) +
-                     8 * (size_t)n_samples * nout + 8 * (size_t)S.n_blocks * nf + 8 * (size_t)nblk * ADJ_NT +
-                     8 * (size_t)n_samples * nf + 4 * max_sched + 64;
-  HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  size_t mem_free = 0, mem_total = 0;
-  HIPCHK(hipMemGetInfo(&mem_free, &mem_total));
-  const size_t budget = mem_free / 10 * 9;
-  int bs = max_batch > 0 ? std::min(max_batch, n_mem) : n_mem;
-  bs = (int)std::min<size_t>((size_t)bs, budget / per);
-  if (bs < 1)
-    return srs::fail(SRS_E_ARG, "checkpoint plan needs %.3g GB per realization, %.3g GB free; raise or lower the "
-                     "checkpoint interval (about sqrt(n_steps) needs least)", per / 1e9, mem_free / 1e9);
-
-  // everything restored at the end
-  std::vector<RealLive> live0(n_real);
-  HIPCHK(hipMemcpy(live0.data(), c->d_live, n_real * sizeof(RealLive), hipMemcpyDeviceToHost));
-  std::vector<std::vector<int32_t>> ryr0(n_mem);
-  for (int m = 0; m < n_mem; ++m) {
-    const HostReal& hr = c->reals[ids[m]];
-    ryr0[m].resize(hr.ryr_ele.size());
-    if (!ryr0[m].empty())
-      HIPCHK(hipMemcpy(ryr0[m].data(), hr.desc.ryr_state, ryr0[m].size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-  }
-
-  DevMem M;
-  double *ck = nullptr, *samp = nullptr, *part = nullptr, *dg = nullptr, *gpart = nullptr;
-  double *lam0 = nullptr, *lam1 = nullptr, *seg = nullptr;
-  d4 *w0 = nullptr, *w1 = nullptr;
-  int32_t *d_sched = nullptr, *d_rid = nullptr;
-  int2* d_sub_tiles = nullptr;
-  const int nt_mesh = (hm.n + c->tile - 1) / c->tile;
-  HIPCHK(M.alloc(&ck, (size_t)n_ck_kept * bs * n6));
-  HIPCHK(M.alloc(&d_sub_tiles, (size_t)bs * nt_mesh));
-  HIPCHK(M.alloc(&samp, (size_t)n_samples * bs * nout));
-  HIPCHK(M.alloc(&part, (size_t)bs * S.n_blocks * nf));
-  HIPCHK(M.alloc(&d_sched, std::max<size_t>(1, (size_t)bs * max_sched)));
-  HIPCHK(M.alloc(&d_rid, (size_t)bs));
-  if (want_grad) {
-    HIPCHK(M.alloc(&lam0, (size_t)bs * n6));
-    HIPCHK(M.alloc(&lam1, (size_t)bs * n6));
-    HIPCHK(M.alloc(&w0, (size_t)bs * np));
-    HIPCHK(M.alloc(&w1, (size_t)bs * np));
-    HIPCHK(M.alloc(&seg, (size_t)ckpt * bs * n6));
-    HIPCHK(M.alloc(&gpart, (size_t)bs * nblk * ADJ_NT));
-    HIPCHK(M.alloc(&dg, (size_t)bs * n_samples * nf));
-  }
-  const uint32_t gmode = G_AFTER_SWEEP | G_NO_GATING | G_REPLAY;
-  const unsigned cp_blocks = (unsigned)std::min<size_t>((n6 + 255) / 256, 2048);
-  std::vector<double> sh, gco, gp;
-  std::vector<RealLive> lv(n_real);
-  std::vector<int2> sub_tiles;
-  // the sweep visits only the members' tiles (a held realization's tiles would
-  // still cost a dispatch each); the context's own list is put back on every
-  // return, before the buffers above are freed
-  struct TileRestore {
-    srs_ctx* c;
-    int2* tiles;
-    int n_tiles, grid;
-    ~TileRestore() { c->d_tiles = tiles; c->n_tiles = n_tiles; c->grid_sweep = grid; }
-  } tile_restore{c, c->d_tiles, c->n_tiles, c->grid_sweep};
-  hipError_t e = hipSuccess;
-  const int rc = [&]() -> int {
-  for (int m0 = 0; m0 < n_mem; m0 += bs) {
-    const int nb = std::min(bs, n_mem - m0);
-    const dim3 g_adj(nblk, nb), g_cp(cp_blocks, nb);
-    if (n_real > 1) {
-      sub_tiles.clear();
-      for (int y = 0; y < nb; ++y)
-        for (int t = 0; t < nt_mesh; ++t) sub_tiles.push_back(make_int2(c->reals[ids[m0 + y]].group, t));
-      HIPCHK(hipMemcpy(d_sub_tiles, sub_tiles.data(), sub_tiles.size() * sizeof(int2), hipMemcpyHostToDevice));
-      c->d_tiles = d_sub_tiles;
-      c->n_tiles = (int)sub_tiles.size();
-      c->grid_sweep = (c->n_tiles + NXCD - 1) / NXCD * NXCD;
-    }
-    // members replay their schedules from row 0; the rest are held
-    for (int q = 0; q < n_real; ++q) {
-      lv[q] = live0[q];
-      lv[q].done = 1;
-    }
-    for (int y = 0; y < nb; ++y) {
-      const int r = ids[m0 + y];
-      lv[r].done = 0;
-      lv[r].ffwd = 0;
-      lv[r].step = 0;
-      lv[r].sched_base = 0;
-      HostReal& hr = c->reals[r];
-      RealDesc dsc = hr.desc;
-      dsc.sched = d_sched + (size_t)y * max_sched;
-      dsc.n_sched = n_steps;
-      HIPCHK(hipMemcpy(d_sched + (size_t)y * max_sched, schedules + soff[m0 + y],
-                       (soff[m0 + y + 1] - soff[m0 + y]) * sizeof(int32_t), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(c->d_desc + r, &dsc, sizeof(dsc), hipMemcpyHostToDevice));
-    }
-    HIPCHK(hipMemcpy(c->d_live, lv.data(), n_real * sizeof(RealLive), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_rid, ids.data() + m0, nb * sizeof(int32_t), hipMemcpyHostToDevice));
-    auto scan_into = [&](double* out) {
-      hipLaunchKernelGGL(k_scan_blur, dim3(S.n_blocks, nb), dim3(SCAN_BS), 0, c->stream, c->d_desc, c->d_live, 0,
-                         d_rid, S.d_cen, S.geo, nf, part);
-      hipLaunchKernelGGL(k_scan_final, dim3(1, nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, 0, d_rid,
-                         S.n_blocks, nf, nca, part, S.d_pad, S.d_near_f, S.d_near_ca, S.desc.f0, S.desc.c0, out, nout);
-    };
-    auto ck_at = [&](int64_t k) { return ck + (size_t)k * nb * n6; };
-
-    // forward, with checkpoints and scan samples
-    hipLaunchKernelGGL(k_seg_start, dim3(nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, (int64_t)0);
-    int ks = 0;
-    for (int64_t n = 0; n < n_steps && e == hipSuccess; ++n) {
-      if (n % ckpt == 0 && n / ckpt < n_ck_kept)
-        hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid,
-                           ck_at(n / ckpt), 1);
-      launch_step(c, gmode);
-      if (ks < n_samples && sample_steps[ks] == n + 1) scan_into(samp + (size_t)(ks++) * nb * nout);
-      e = hipGetLastError();
-      if (e == hipSuccess && (n & 4095) == 4095) e = hipStreamSynchronize(c->stream);
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return srs::fail(SRS_E_HIP, "adjoint forward: %s", hipGetErrorString(e));
-    sh.resize((size_t)n_samples * nb * nout);
-    if ((e = hipMemcpy(sh.data(), samp, sh.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
-      return srs::fail(SRS_E_HIP, "adjoint forward: %s", hipGetErrorString(e));
-    // losses and their gradients in the blurred fluorescence
-    gco.assign((size_t)nb * n_samples * nf, 0.0);
-    const double f0 = S.desc.f0;
-    for (int y = 0; y < nb; ++y) {
-      const double* tg = targets + (size_t)(m0 + y) * n_samples * nf;
-      double Ls = 0.0;
-      for (int k = 0; k < n_samples; ++k)
-        for (int i = 0; i < nf; ++i) {
-          const double dff = sh[((size_t)k * nb + y) * nout + nca + nf + i] / f0 - 1.0;
-          const double res = dff - tg[(size_t)k * nf + i];
-          Ls += res * res;
-          gco[((size_t)y * n_samples + k) * nf + i] = 2.0 * res / f0;
-          if (scans) scans[((size_t)(m0 + y) * n_samples + k) * nf + i] = dff;
-        }
-      losses[m0 + y] = Ls;
-    }
-
-    if (want_grad) {
-      e = hipMemcpy(dg, gco.data(), gco.size() * sizeof(double), hipMemcpyHostToDevice);
-      if (e == hipSuccess) e = hipMemsetAsync(lam0, 0, (size_t)nb * n6 * sizeof(double), c->stream);
-      if (e == hipSuccess) e = hipMemsetAsync(gpart, 0, (size_t)nb * nblk * ADJ_NT * sizeof(double), c->stream);
-      double* lam = lam0;   // lambda_{n+1}
-      double* lout = lam1;  // lambda_n
-      d4* wcur = w0;        // w of step n
-      d4* wnext = w1;       // w of step n - 1
-      ks = n_samples - 1;
-      // the loss source of a sample after the last step; the step kernel
-      // adds the others to the lambda it produces
-      if (sample_steps[ks] == n_steps) {
-        hipLaunchKernelGGL(k_scan_adj, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, d_rid, S.d_cen, S.geo, nf,
-                           dg + (size_t)ks * nf, n_samples * nf, lam);
-        --ks;
-      }
-      for (int64_t s0 = (n_ck - 1) * ckpt; s0 >= 0 && e == hipSuccess; s0 -= ckpt) {
-        const int64_t s1 = std::min<int64_t>(n_steps, s0 + ckpt);
-        // recompute the segment from its checkpoint, the sweep writing each
-        // state into its slot of the segment
-        hipLaunchKernelGGL(k_seg_start, dim3(nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, s0);
-        e = hipMemcpyAsync(seg, ck_at(s0 / ckpt), (size_t)nb * n6 * sizeof(double), hipMemcpyDeviceToDevice,
-                           c->stream);
-        for (int64_t n = s0; n < s1 - 1 && e == hipSuccess; ++n) {
-          hipLaunchKernelGGL(k_seg_ptrs, dim3((nb + 63) / 64), dim3(64), 0, c->stream, c->d_desc, c->d_live, d_rid,
-                             seg, s0, nb);
-          launch_step(c, gmode);
-          e = hipGetLastError();
-        }
-        // backward through the segment
-        for (int64_t n = s1 - 1; n >= s0 && e == hipSuccess; --n) {
-          const double* Xn = seg + (size_t)(n - s0) * nb * n6;
-          if (n == s1 - 1)
-            hipLaunchKernelGGL(k_adj_w, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, d_rid, Xn, lam, wcur);
-          const bool scan = ks >= 0 && sample_steps[ks] == n;
-          AdjNext nx{n > s0 ? Xn - nb * n6 : nullptr, wnext, S.d_cen, scan ? dg + (size_t)ks * nf : nullptr,
-                     S.geo, nf, n_samples * nf};
-          if (scan) --ks;
-          auto kern = c->compact_all ? (scan ? k_adj_step<true, true> : k_adj_step<true, false>)
-                                     : (scan ? k_adj_step<false, true> : k_adj_step<false, false>);
-          hipLaunchKernelGGL(kern, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, d_rid, Xn, lam, wcur, n,
-                             lout, gpart, nx);
-          std::swap(lam, lout);
-          std::swap(wcur, wnext);
-          e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-      }
-      if (e == hipSuccess) {
-        gp.resize((size_t)nb * nblk * ADJ_NT);
-        e = hipMemcpy(gp.data(), gpart, gp.size() * sizeof(double), hipMemcpyDeviceToHost);
-      }
-      if (e != hipSuccess) return srs::fail(SRS_E_HIP, "adjoint backward: %s", hipGetErrorString(e));
-      for (int y = 0; y < nb; ++y)
-        for (int k = 0; k < ADJ_NT; ++k) {
-          double sum = 0.0;
-          for (int b = 0; b < nblk; ++b) sum += gp[((size_t)y * nblk + b) * ADJ_NT + k];
-          grads[(size_t)(m0 + y) * ADJ_NT + k] = sum;
-        }
-    }
-    // the members' states as they were: descriptors (the recomputation
-    // pointed their buffers into the segment) and live state (it names the
-    // current buffer) first, then the start state into that buffer
-    for (int y = 0; y < nb; ++y) {
-      const HostReal& hr = c->reals[ids[m0 + y]];
-      HIPCHK(hipMemcpy(c->d_desc + ids[m0 + y], &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice));
-    }
-    HIPCHK(hipMemcpy(c->d_live, live0.data(), n_real * sizeof(RealLive), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, ck_at(0), 0);
-    HIPCHK(hipStreamSynchronize(c->stream));
-  }
-  return SRS_OK;
-  }();
-  // restore descriptors, live state and RyR states (also after a failure)
-  for (int m = 0; m < n_mem; ++m) {
-    const HostReal& hr = c->reals[ids[m]];
-    (void)hipMemcpy(c->d_desc + ids[m], &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice);
-    if (!ryr0[m].empty())
-      (void)hipMemcpy(hr.desc.ryr_state, ryr0[m].data(), ryr0[m].size() * sizeof(int32_t), hipMemcpyHostToDevice);
-  }
-  (void)hipMemcpy(c->d_live, live0.data(), n_real * sizeof(RealLive), hipMemcpyHostToDevice);
-  e = hipDeviceSynchronize();
-  if (rc == SRS_OK && e != hipSuccess) return srs::fail(SRS_E_HIP, "adjoint restore: %s", hipGetErrorString(e));
-  return rc;
-}
-
-static int loss_grad_check(srs_ctx* c, int32_t scan_id, int64_t n_steps, int32_t n_samples,
-                           const int64_t* sample_steps, const double* target, int32_t ckpt, const void* loss) {
-  if (!c) return srs::fail(SRS_E_ARG, "null context");
-  if (!c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
-  if (c->group != 1) return srs::fail(SRS_E_ARG, "the adjoint needs realization groups of size 1");
-  if (scan_id < 0 || scan_id >= (int)c->scans.size()) return srs::fail(SRS_E_ARG, "bad scan id %d", scan_id);
-  if (c->scans[scan_id].n_f <= 0) return srs::fail(SRS_E_ARG, "the scan has no fluorescence points");
-  if (n_steps <= 0 || n_samples <= 0 || !sample_steps || !target || !loss) return srs::fail(SRS_E_ARG, "bad arguments");
-  if (ckpt <= 0 || ckpt > 65536) return srs::fail(SRS_E_ARG, "checkpoint interval must be 1..65536");
-  for (int k = 0; k < n_samples; ++k)
-    if (sample_steps[k] < 1 || sample_steps[k] > n_steps || (k && sample_steps[k] <= sample_steps[k - 1]))
-      return srs::fail(SRS_E_ARG, "sample steps must increase within 1..n_steps");
-  return SRS_OK;
-}
-
-int srs_linescan_loss_grad(srs_ctx* c, int32_t r, int32_t scan_id, int64_t n_steps, const int32_t* schedule,
-                           int32_t n_samples, const int64_t* sample_steps, const double* target, int32_t ckpt,
-                           double* loss, double* grad) {
-  int rc = check_real(c, r);
-  if (rc) return rc;
-  if ((rc = loss_grad_check(c, scan_id, n_steps, n_samples, sample_steps, target, ckpt, loss))) return rc;
-  if (c->reals[r].mesh != c->scans[scan_id].mesh) return srs::fail(SRS_E_ARG, "realization is not on the scan's mesh");
-  if (!c->reals[r].ryr_ele.empty() && !schedule) return srs::fail(SRS_E_ARG, "null schedule");
-  return loss_grad_run(c, {r}, scan_id, n_steps, schedule, n_samples, sample_steps, target, ckpt, 1, loss, grad,
-                       nullptr);
-}
-
-int srs_linescan_loss_grad_batch(srs_ctx* c, int32_t n_real, const int32_t* real_ids, int32_t scan_id,
-                                 int64_t n_steps, const int32_t* schedules, int32_t n_samples,
-                                 const int64_t* sample_steps, const double* targets, int32_t ckpt, int32_t max_batch,
-                                 double* losses, double* grads, double* scans) {
-  int rc = c ? SRS_OK : srs::fail(SRS_E_ARG, "null context");
-  if (rc) return rc;
-  if (n_real <= 0 || !real_ids) return srs::fail(SRS_E_ARG, "no realizations");
-  if (ckpt == 0) ckpt = (int32_t)std::max<double>(1.0, std::ceil(std::sqrt((double)std::max<int64_t>(n_steps, 1))));
-  if ((rc = loss_grad_check(c, scan_id, n_steps, n_samples, sample_steps, targets, ckpt, losses))) return rc;
-  std::vector<int32_t> ids(real_ids, real_ids + n_real);
-  std::vector<char> seen(c->reals.size(), 0);
-  bool any_ryr = false;
-  for (int32_t r : ids) {
-    if ((rc = check_real(c, r))) return rc;
-    if (seen[r]++) return srs::fail(SRS_E_ARG, "realization %d listed twice", r);
-    if (c->reals[r].mesh != c->scans[scan_id].mesh)
-      return srs::fail(SRS_E_ARG, "realization %d is not on the scan's mesh", r);
-    any_ryr |= !c->reals[r].ryr_ele.empty();
-  }
-  if (any_ryr && !schedules) return srs::fail(SRS_E_ARG, "null schedules");
-  return loss_grad_run(c, ids, scan_id, n_steps, schedules, n_samples, sample_steps, targets, ckpt, max_batch, losses,
-                       grads, scans);
 }
 
 }  // extern "C"
