@@ -96,7 +96,7 @@ REF_CPU = "/root/reference/cpu"
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(REF_CPU, "parameters.h")), reason="reference sources not present")
 def test_integration_snippets_compile(tmp_path):
-    """The C bridge in INTEGRATION.md compiles against the reference's own
+    """The C bridge of the integration notes compiles against the reference's own
     cpu/parameters.h (its SimData/GridStruct/ParamStruct field names) and our
     header, and the fitting snippets compile as C99."""
     import shutil

@@ -1,6 +1,6 @@
 """HIP path vs the oracle, through the C ABI (needs an MI355X).
 
-Tolerance: the north-star bound rel <= 1e-10 for FP64 fields.  The sweep is
+Tolerance: rel <= 1e-10 for FP64 fields (the deterministic-mode bar).  The sweep is
 built without FP contraction and in the reference's operation order, so most
 runs are bit-identical; the bound leaves room for the one-ulp differences of
 device exp/pow/log against glibc in the LCC flux and the gating rates.
