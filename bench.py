@@ -53,7 +53,7 @@ def algorithmic_bytes(n_ele):
 
 
 def live_bytes(domain, tropc):
-    """The bytes this build's sweep actually has to move: species that are
+    """The bytes the sweep actually has to move: species that are
     identically zero (TnC outside troponin-bearing cytosol, CSQ outside the
     jSR) are neither read nor written."""
     jsr = np.asarray(domain) != 0
@@ -483,8 +483,8 @@ def main():
                     help="also run one device copy of 2 GiB (HBM byte-counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the spark step does not shard across GPUs "
-              "in this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the spark step is not sharded across GPUs",
+              file=sys.stderr)
         sys.exit(2)
 
     from srs_amd import workloads as W
