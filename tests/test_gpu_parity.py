@@ -568,3 +568,40 @@ def test_checkpoint_resume_continues_bit_for_bit(group):
         assert got[k][2] == lp
         np.testing.assert_array_equal(np.vstack([recs_b[k], got[k][3]]), rec)
     assert lp["step"] == 300 and len(rec) > 10
+
+
+@pytest.mark.parametrize("compact", [1, 0])
+def test_irregular_plane_spacing_gating_run(compact):
+    """A box whose plane spacings are drawn at random (every element row
+    differs, so the compact layout needs many classes and ulp deltas), with a
+    T-tubule hole, the jSR slab and an SL face: a gated run matches the oracle
+    in both row layouts, bit for bit in the channel states."""
+    rng = np.random.default_rng(17)
+    xs = np.concatenate([[0.0], np.cumsum(rng.uniform(0.025, 0.055, 9))])
+    ys = np.concatenate([[0.0], np.cumsum(rng.uniform(0.025, 0.055, 8))])
+    zs = np.concatenate([[0.0, 0.015], 0.015 + np.cumsum(rng.uniform(0.03, 0.05, 5))])
+    cx, cy = xs[-1] / 2, ys[-1] / 2
+    spec = S.BoxSpec(xs=xs, ys=ys, zs=zs, jsr_lo=(cx - 0.07, cy - 0.07, 0.015), jsr_hi=(cx + 0.07, cy + 0.07, 0.06),
+                     sl_face_z0=True, tt_axis_x=0.0, tt_axis_z=0.15, tt_radius=0.06, trpn_radius=0.09, ryr_rows=3,
+                     ryr_cols=3, ryr_spacing=0.031, ryr_center=(cx, cy), ryr_z_cyt=0.0075, ryr_z_jsr=0.03,
+                     lcc_xyz=np.array([[cx + 0.02, cy - 0.03, 0.0075]]))
+    pr = W.prepare(spec, W.spark_params())
+    g = pr.grid
+    p = fast_gating(pr.params)
+    F = initial_fields(pr, seed=23)
+    ctx = S.Context(0)
+    ctx.tune(compact=compact)
+    mid = ctx.add_mesh_grid(g, pr.aij, pr.nij)
+    ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=29, stream=0)
+    ctx.finalize()
+    ctx.state_set(0, F.X)
+    ctx.channels_set(0, F.ryr_state, F.lcc_state, F.lccv_state, F.R)
+    ctx.globals_set(0, F.gv[0], F.gv[1])
+    ctx.explicit_step(400, S.STEP_GRAPH)
+    X, ch = ctx.state_get(0), ctx.channels_get(0)
+    ctx.close()
+    Fo = Oracle().steps(grid_dict(g, pr.aij, pr.nij), p.reals, F.copy(), 400, p.reals[1], gating=True,
+                        rng_kind=RNG_PHILOX, key=29, stream=0)
+    assert len(np.unique(pr.aij.reshape(-1, 5), axis=0)) > g.n_ele // 4  # the rows really vary
+    assert np.array_equal(ch[0], Fo.ryr_state) and np.array_equal(ch[1], Fo.lcc_state)
+    assert rel_err(X, Fo.X) <= TOL
