@@ -59,7 +59,7 @@ __device__ __forceinline__ Loc load_loc(const RealDesc& d, const RealLive& l, co
 // lie in the workgroup's own tile read from the copy staged in LDS.
 template <int FK, int BS>
 __device__ __forceinline__ Loc load_loc_lds(const RealDesc& d, const RealLive& l, const int j, const Geo& g,
-                                            const d4* tq, const int base, const d4& m0) {
+                                            const d4* tq, const int base, const d4& m0, const double tr) {
   const size_t np = d.np;
   const SRS_GLOBAL double* __restrict__ Xo = gp(d.X[l.cur]);
   const SRS_GLOBAL d4* Mo = (const SRS_GLOBAL d4*)Xo;
@@ -68,7 +68,7 @@ __device__ __forceinline__ Loc load_loc_lds(const RealDesc& d, const RealLive& l
   const uint8_t f = FK >= 0 ? (uint8_t)FK : g.f;
   const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
   const bool live_cs = (f & F_JSR) != 0;
-  c.c_tr = live_tr ? Xo[4 * np + j] : 0.0;
+  c.c_tr = live_tr ? tr : 0.0;
   c.c_cs = live_cs ? Xo[5 * np + j] : 0.0;
   auto nq = [&](int nb) -> d4 {
     const unsigned k = (unsigned)(nb - base);
@@ -226,19 +226,25 @@ __device__ __forceinline__ void sweep_tile_lds(const RealDesc* __restrict__ D, c
   const bool in = j < d.n;
   Geo g{};
   d4 m0 = {0.0, 0.0, 0.0, 0.0};
+  double tr = 0.0;
   if (in) {
+    // the element's TnC is loaded with its row and quad, before its flags are
+    // known (it is live in all but jSR and troponin-free cytosol elements), so
+    // that the loads after the barrier are only the neighbour gathers
+    const SRS_GLOBAL double* Xo = gp(d.X[l.cur]);
     g = load_geo<NTG, COMPACT, -1>(d, j);
-    m0 = ((const SRS_GLOBAL d4*)gp(d.X[l.cur]))[j];
+    m0 = ((const SRS_GLOBAL d4*)Xo)[j];
+    tr = Xo[4 * (size_t)d.np + j];
   }
   tq[threadIdx.x] = m0;
   __syncthreads();
   if (!in) return;
   const bool plain = (g.f & F_SPECIAL) == 0 && (g.f & F_TROPC);
   if (__builtin_amdgcn_ballot_w64(!plain) == 0) {
-    const Loc c = load_loc_lds<F_TROPC, BS>(d, l, j, g, tq, base, m0);
+    const Loc c = load_loc_lds<F_TROPC, BS>(d, l, j, g, tq, base, m0, tr);
     step_elem<NTS, F_TROPC>(d, l, j, g, c);
   } else {
-    const Loc c = load_loc_lds<-1, BS>(d, l, j, g, tq, base, m0);
+    const Loc c = load_loc_lds<-1, BS>(d, l, j, g, tq, base, m0, tr);
     step_elem<NTS, -1>(d, l, j, g, c);
   }
 }
