@@ -54,6 +54,15 @@ def test_call_order_and_argument_errors():
         ctx.state_get(3)
     with pytest.raises(S.SrsError, match="negative"):
         ctx.explicit_step(-1)
+    with pytest.raises(S.SrsError, match="negative"):
+        ctx.prepare(-5)
+    ctx.prepare(300, S.STEP_GRAPH | S.STEP_PROTOCOL)  # 128-step and 44-step blocks, nothing stepped
+    assert ctx.loop_get(0)["step"] == 0
+    with pytest.raises(S.SrsError, match="out of range"):
+        ctx.checkpoint(2)
+    ctx.loop_set(0, done=1)
+    with pytest.raises(S.SrsError, match="finished"):
+        ctx.record_schedules(8)
     ctx.close()
 
 
