@@ -198,7 +198,7 @@ def bench_batched(n_real, steps, warmup, order, tune=None, n_tune=512):
     if tune is None:
         sample = list(range(min(n_real, n_tune)))
         tune, trials = autotune(lambda cfg: W.ensemble_context(n_real, order=order, tune=cfg, only=sample)[0],
-                                tiles=(128, 256), compact=(1,), occupancy=(0,), groups=(1, 2, 4))
+                                tiles=(128, 256), compact=(1,), occupancy=(0, 6), groups=(1, 2, 4))
         log(f"batched autotune: {tune}")
     ctx, dts, n_ele, nbytes = W.ensemble_context(n_real, order=order, tune=tune, log=log)
     log(f"batched: {n_real} realizations, {n_ele} elements, group {tune.get('group', 1)}")
@@ -218,7 +218,7 @@ def bench_batched(n_real, steps, warmup, order, tune=None, n_tune=512):
     for m in range(3):
         r = next(k for k, mem in enumerate(members) if mem["mesh"] == m)
         ptune, _ = autotune(lambda cfg: W.ensemble_context(n_real, order=order, tune=cfg, only=[r])[0],
-                            tiles=(128, 256), compact=(1,), occupancy=(0,))
+                            tiles=(128, 256), compact=(1,), occupancy=(0, 6))
         ptunes[m] = ptune
         pctx, pdts, _, _ = W.ensemble_context(n_real, order=order, tune=ptune, only=[r])
         pctx.explicit_step(200, S.STEP_GRAPH)

@@ -254,7 +254,8 @@ int srs_ctx_create(int device, srs_ctx** out);
 #define SRS_TUNE_PLANE_PAD 4     /* extra elements (multiple of 64) between SoA planes */
 #define SRS_TUNE_COMPACT 5       /* 1 (default): store operator rows as class + int16 ulp deltas and
                                     neighbours as int16 offsets where that reproduces them exactly */
-#define SRS_TUNE_OCCUPANCY 6     /* 8: sweep built for eight wavefronts per SIMD (<= 64 VGPRs); 0: default */
+#define SRS_TUNE_OCCUPANCY 6     /* sweep register budget: 0 compiler's choice; 6 or 8: at most 80 or 64
+                                    VGPRs, so that six or eight wavefronts share each SIMD */
 #define SRS_TUNE_GROUP 7         /* 1 (default), 2 or 4: realizations of one mesh whose states are interleaved
                                     per element so that each sweep tile loads the operator rows once for all
                                     of them (set before srs_ctx_finalize) */

@@ -22,7 +22,7 @@ KERNEL_KEYS = ("nt_geometry", "nt_state", "occupancy", "lds")
 
 
 def autotune(make: Callable[[dict], Context], tiles=(128, 256, 512), pads=(0,), compact=(1, 0), nt_geometry=(0, 1),
-             nt_state=(0, 1), occupancy=(0, 8), lds=(0, 1), groups=(1,), reps: int = 20, warm: int = 3,
+             nt_state=(0, 1), occupancy=(0, 6), lds=(0, 1), groups=(1,), reps: int = 20, warm: int = 3,
              log=None) -> tuple[dict, list]:
     """Returns (best configuration, all trials as (config, sweep_ms))."""
     trials = []
@@ -33,8 +33,8 @@ def autotune(make: Callable[[dict], Context], tiles=(128, 256, 512), pads=(0,), 
         ctx = make(layout)
         try:
             for g, s, o, ld in itertools.product(nt_geometry, nt_state, occupancy, lds):
-                if ld and (o or grp != 1):
-                    continue  # the LDS-staged sweep: one register budget, groups of 1
+                if ld and grp != 1:
+                    continue  # the LDS-staged sweep: groups of 1 only
                 ctx.tune(nt_geometry=g, nt_state=s, occupancy=o, lds=ld)
                 ctx.explicit_step(warm)
                 t0 = time.perf_counter()

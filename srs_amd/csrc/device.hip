@@ -29,13 +29,39 @@ struct Loc {
   double c_tr, c_cs;
 };
 
+// A realization's state buffers for one step: read X[cur], write X[cur ^ 1].
+// Both pointers are loaded and one is selected, so that the live index and
+// the pointers arrive in the same round of scalar loads.
+struct Bufs {
+  const SRS_GLOBAL double* xo;
+  SRS_GLOBAL double* xn;
+};
+__device__ __forceinline__ Bufs bufs(double* x0, double* x1, int cur) {
+  return cur ? Bufs{gp(x1), gpw(x0)} : Bufs{gp(x0), gpw(x1)};
+}
+
+// Pins uniform values to this point of the program: the scalar loads that
+// produce them are issued as one batch ahead of the branches that follow,
+// instead of one memory round trip after another where each is first used.
+// The values are folded into one operand of an empty asm statement.
+template <typename T>
+__device__ __forceinline__ uint64_t fold_bits(const T v) {
+  if constexpr (std::is_pointer<T>::value) return (uint64_t)(uintptr_t)v;
+  else return (uint64_t)(int64_t)v;
+}
+template <typename... T>
+__device__ __forceinline__ uint64_t fold(const T... v) {
+  return (fold_bits(v) ^ ...);
+}
+__device__ __forceinline__ void scalars_here(const uint64_t folded) { __asm__ volatile("" ::"s"(folded)); }
+
 template <int FK>
-__device__ __forceinline__ Loc load_loc(const RealDesc& d, const RealLive& l, const int j, const Geo& g) {
+__device__ __forceinline__ Loc load_loc(const RealDesc& d, const SRS_GLOBAL double* __restrict__ Xo, const int j,
+                                        const Geo& g) {
   // Realization-group layout: element e of the realization in slot s of a
   // group of K is quad gidx(e, K, s); the TnC and CSQ planes follow the quads.
   const int K = d.kst, s = d.slot;
   const size_t np = d.np;
-  const SRS_GLOBAL double* __restrict__ Xo = gp(d.X[l.cur]);
   const SRS_GLOBAL d4* Mo = (const SRS_GLOBAL d4*)Xo;
   Loc c;
   c.m0 = Mo[gidx(j, K, s)];
@@ -58,10 +84,10 @@ __device__ __forceinline__ Loc load_loc(const RealDesc& d, const RealLive& l, co
 // As load_loc for a single-realization layout, with the neighbour quads that
 // lie in the workgroup's own tile read from the copy staged in LDS.
 template <int FK, int BS>
-__device__ __forceinline__ Loc load_loc_lds(const RealDesc& d, const RealLive& l, const int j, const Geo& g,
-                                            const d4* tq, const int base, const d4& m0, const double tr) {
+__device__ __forceinline__ Loc load_loc_lds(const RealDesc& d, const SRS_GLOBAL double* __restrict__ Xo, const int j,
+                                            const Geo& g, const d4* tq, const int base, const d4& m0,
+                                            const double tr) {
   const size_t np = d.np;
-  const SRS_GLOBAL double* __restrict__ Xo = gp(d.X[l.cur]);
   const SRS_GLOBAL d4* Mo = (const SRS_GLOBAL d4*)Xo;
   Loc c;
   c.m0 = m0;
@@ -84,12 +110,12 @@ __device__ __forceinline__ Loc load_loc_lds(const RealDesc& d, const RealLive& l
 // The fused step of one element of one realization, from its loaded row and
 // state, in the reference's operation order (cpu/RunSimulation.cpp:591-648).
 template <bool NTS, int FK>
-__device__ __forceinline__ void step_elem(const RealDesc& d, const RealLive& l, const int j, const Geo& g,
-                                          const Loc& c) {
+__device__ __forceinline__ void step_elem(const RealDesc& d, const RealLive& l, const Bufs bf, const int j,
+                                          const Geo& g, const Loc& c) {
   const int K = d.kst, s = d.slot;
   const size_t np = d.np;
-  const SRS_GLOBAL double* __restrict__ Xo = gp(d.X[l.cur]);
-  SRS_GLOBAL double* __restrict__ Xn = gpw(d.X[l.cur ^ 1]);
+  const SRS_GLOBAL double* __restrict__ Xo = bf.xo;
+  SRS_GLOBAL double* __restrict__ Xn = bf.xn;
   const double* P = d.P;
   const double dt = P[SRS_R_T_STEP];
   const uint8_t f = FK >= 0 ? (uint8_t)FK : g.f;
@@ -192,35 +218,33 @@ __device__ __forceinline__ void step_elem(const RealDesc& d, const RealLive& l, 
 constexpr uint8_t F_SPECIAL = F_JSR | F_SL | F_BR | F_CHAN;
 
 // One tile of one realization group: the operator rows are loaded once and
-// applied to the KG realizations of the group, one after another.  Realizations that are done or in the ffwd phase
-// are computed too, into the buffer they do not read next: their state
-// pointer does not flip, so the result is never used.
+// applied to the KG realizations of the group, one after another; -1 marks an
+// empty slot or a member that does not step.
 template <bool NTS, int KG, int FK>
 __device__ __forceinline__ void sweep_group(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
                                             const int* mem, const int j, const Geo& g) {
-  if constexpr (KG == 1) {
-    const Loc c = load_loc<FK>(D[mem[0]], L[mem[0]], j, g);
-    step_elem<NTS, FK>(D[mem[0]], L[mem[0]], j, g, c);
-  } else {
-    // members one after another: the register budget of a single realization;
-    // -1 marks an empty slot or a member that does not step
-    for (int s = 0; s < KG; ++s)
-      if (mem[s] >= 0) {
-        const Loc c = load_loc<FK>(D[mem[s]], L[mem[s]], j, g);
-        step_elem<NTS, FK>(D[mem[s]], L[mem[s]], j, g, c);
-      }
-  }
+  for (int s = 0; s < KG; ++s)
+    if (KG == 1 || mem[s] >= 0) {
+      const RealDesc& d = D[mem[s]];
+      const Bufs bf = bufs(d.X[0], d.X[1], L[mem[s]].cur);
+      const Loc c = load_loc<FK>(d, bf.xo, j, g);
+      step_elem<NTS, FK>(d, L[mem[s]], bf, j, g, c);
+    }
+}
+
+// The sizes and operator-row pointers of a mesh, folded for scalars_here.
+template <bool COMPACT>
+__device__ __forceinline__ uint64_t geo_fold(const RealDesc& d) {
+  if constexpr (COMPACT) return fold(d.n, d.np, d.flags, d.off, d.cls, d.dq, d.tab);
+  else return fold(d.n, d.np, d.flags, d.coef, d.nbr);
 }
 
 // Single-realization groups with LDS staging: the tile's own quads are
 // loaded coalesced into LDS, and neighbour quads inside the tile come from
 // there instead of through the L1 cache.
 template <int BS, bool NTG, bool NTS, bool COMPACT>
-__device__ __forceinline__ void sweep_tile_lds(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int r,
-                                               int t) {
+__device__ __forceinline__ void sweep_tile_lds(const RealDesc& d, const RealLive& l, const Bufs bf, int t) {
   __shared__ d4 tq[BS];
-  const RealDesc& d = D[r];
-  const RealLive& l = L[r];
   const int base = t * BS;
   const int j = base + threadIdx.x;
   const bool in = j < d.n;
@@ -231,21 +255,20 @@ __device__ __forceinline__ void sweep_tile_lds(const RealDesc* __restrict__ D, c
     // the element's TnC is loaded with its row and quad, before its flags are
     // known (it is live in all but jSR and troponin-free cytosol elements), so
     // that the loads after the barrier are only the neighbour gathers
-    const SRS_GLOBAL double* Xo = gp(d.X[l.cur]);
     g = load_geo<NTG, COMPACT, -1>(d, j);
-    m0 = ((const SRS_GLOBAL d4*)Xo)[j];
-    tr = Xo[4 * (size_t)d.np + j];
+    m0 = ((const SRS_GLOBAL d4*)bf.xo)[j];
+    tr = bf.xo[4 * (size_t)d.np + j];
   }
   tq[threadIdx.x] = m0;
   __syncthreads();
   if (!in) return;
   const bool plain = (g.f & F_SPECIAL) == 0 && (g.f & F_TROPC);
   if (__builtin_amdgcn_ballot_w64(!plain) == 0) {
-    const Loc c = load_loc_lds<F_TROPC, BS>(d, l, j, g, tq, base, m0, tr);
-    step_elem<NTS, F_TROPC>(d, l, j, g, c);
+    const Loc c = load_loc_lds<F_TROPC, BS>(d, bf.xo, j, g, tq, base, m0, tr);
+    step_elem<NTS, F_TROPC>(d, l, bf, j, g, c);
   } else {
-    const Loc c = load_loc_lds<-1, BS>(d, l, j, g, tq, base, m0, tr);
-    step_elem<NTS, -1>(d, l, j, g, c);
+    const Loc c = load_loc_lds<-1, BS>(d, bf.xo, j, g, tq, base, m0, tr);
+    step_elem<NTS, -1>(d, l, bf, j, g, c);
   }
 }
 
@@ -279,30 +302,48 @@ __device__ __forceinline__ void sweep_tile(const RealDesc* __restrict__ D, const
   else sweep_group<NTS, KG, -1>(D, L, mem, j, g);
 }
 
-// Two register budgets: the compiler's choice, or at most 64 VGPRs so that
-// eight wavefronts share each SIMD.
+template <int BS, bool NTG, bool NTS, bool COMPACT>
+__device__ __forceinline__ void sweep_lds(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                                          const int2* __restrict__ tiles, int n_tiles, const int* __restrict__ groups,
+                                          int single) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  if (b >= n_tiles) return;
+  const int2 tl = single ? make_int2(0, b) : tiles[b];
+  const int r = single ? 0 : groups[tl.x];
+  const RealDesc& d = D[r];
+  const RealLive& l = L[r];
+  const int skip = l.done | l.ffwd;
+  const Bufs bf = bufs(d.X[0], d.X[1], l.cur);
+  scalars_here(fold(skip, bf.xo, bf.xn) ^ geo_fold<COMPACT>(d));
+  if (skip) return;
+  sweep_tile_lds<BS, NTG, NTS, COMPACT>(d, l, bf, tl.y);
+}
+
+// Register budgets: the compiler's choice, or the largest that lets W
+// wavefronts share each SIMD (64 VGPRs for eight, 80 for six).
 template <int BS, bool NTG, bool NTS, bool COMPACT, int KG>
 __global__ __launch_bounds__(BS) void k_sweep(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
                                               const int2* __restrict__ tiles, int n_tiles,
                                               const int* __restrict__ groups, int single) {
   sweep_tile<BS, NTG, NTS, COMPACT, KG>(D, L, tiles, n_tiles, groups, single);
 }
+template <int BS, bool NTG, bool NTS, bool COMPACT, int KG, int W>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_sweep_w(
+    const RealDesc* __restrict__ D, const RealLive* __restrict__ L, const int2* __restrict__ tiles, int n_tiles,
+    const int* __restrict__ groups, int single) {
+  sweep_tile<BS, NTG, NTS, COMPACT, KG>(D, L, tiles, n_tiles, groups, single);
+}
 template <int BS, bool NTG, bool NTS, bool COMPACT>
 __global__ __launch_bounds__(BS) void k_sweep_lds(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
                                                   const int2* __restrict__ tiles, int n_tiles,
                                                   const int* __restrict__ groups, int single) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  if (b >= n_tiles) return;
-  const int2 tl = single ? make_int2(0, b) : tiles[b];
-  const int r = single ? 0 : groups[tl.x];
-  if (L[r].done | L[r].ffwd) return;
-  sweep_tile_lds<BS, NTG, NTS, COMPACT>(D, L, r, tl.y);
+  sweep_lds<BS, NTG, NTS, COMPACT>(D, L, tiles, n_tiles, groups, single);
 }
-template <int BS, bool NTG, bool NTS, bool COMPACT, int KG>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_sweep_o8(
+template <int BS, bool NTG, bool NTS, bool COMPACT, int W>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_sweep_lds_w(
     const RealDesc* __restrict__ D, const RealLive* __restrict__ L, const int2* __restrict__ tiles, int n_tiles,
     const int* __restrict__ groups, int single) {
-  sweep_tile<BS, NTG, NTS, COMPACT, KG>(D, L, tiles, n_tiles, groups, single);
+  sweep_lds<BS, NTG, NTS, COMPACT>(D, L, tiles, n_tiles, groups, single);
 }
 
 __device__ void write_record(const RealDesc& d, RealLive& l, const double* Ca, double t) {
@@ -707,26 +748,43 @@ int check_real(srs_ctx* c, int32_t r) {
 
 using SweepFn = void (*)(const RealDesc*, const RealLive*, const int2*, int, const int*, int);
 
+// The sweep kernel for a tile size, cache hints, register budget (0, 6 or 8
+// wavefronts per SIMD) and LDS staging (single-realization groups only).
+template <int BS, bool CMP, int KG, int W>
+SweepFn pick_w(int ntg, int nts, int lds) {
+  if constexpr (KG == 1) {
+    if (lds) {
+      if constexpr (W == 0) {
+        if (ntg) return nts ? k_sweep_lds<BS, true, true, CMP> : k_sweep_lds<BS, true, false, CMP>;
+        return nts ? k_sweep_lds<BS, false, true, CMP> : k_sweep_lds<BS, false, false, CMP>;
+      } else {
+        if (ntg) return nts ? k_sweep_lds_w<BS, true, true, CMP, W> : k_sweep_lds_w<BS, true, false, CMP, W>;
+        return nts ? k_sweep_lds_w<BS, false, true, CMP, W> : k_sweep_lds_w<BS, false, false, CMP, W>;
+      }
+    }
+  }
+  if constexpr (W == 0) {
+    if (ntg) return nts ? k_sweep<BS, true, true, CMP, KG> : k_sweep<BS, true, false, CMP, KG>;
+    return nts ? k_sweep<BS, false, true, CMP, KG> : k_sweep<BS, false, false, CMP, KG>;
+  } else {
+    if (ntg) return nts ? k_sweep_w<BS, true, true, CMP, KG, W> : k_sweep_w<BS, true, false, CMP, KG, W>;
+    return nts ? k_sweep_w<BS, false, true, CMP, KG, W> : k_sweep_w<BS, false, false, CMP, KG, W>;
+  }
+}
+
 template <int BS, bool CMP, int KG>
-SweepFn pick3(int ntg, int nts, int occ8, int lds) {
-  if (KG == 1 && lds) {
-    if (ntg) return nts ? k_sweep_lds<BS, true, true, CMP> : k_sweep_lds<BS, true, false, CMP>;
-    return nts ? k_sweep_lds<BS, false, true, CMP> : k_sweep_lds<BS, false, false, CMP>;
-  }
-  if (occ8) {
-    if (ntg) return nts ? k_sweep_o8<BS, true, true, CMP, KG> : k_sweep_o8<BS, true, false, CMP, KG>;
-    return nts ? k_sweep_o8<BS, false, true, CMP, KG> : k_sweep_o8<BS, false, false, CMP, KG>;
-  }
-  if (ntg) return nts ? k_sweep<BS, true, true, CMP, KG> : k_sweep<BS, true, false, CMP, KG>;
-  return nts ? k_sweep<BS, false, true, CMP, KG> : k_sweep<BS, false, false, CMP, KG>;
+SweepFn pick3(int ntg, int nts, int occ, int lds) {
+  if (occ == 8) return pick_w<BS, CMP, KG, 8>(ntg, nts, lds);
+  if (occ == 6) return pick_w<BS, CMP, KG, 6>(ntg, nts, lds);
+  return pick_w<BS, CMP, KG, 0>(ntg, nts, lds);
 }
 
 template <bool CMP, int KG>
 SweepFn pick_tile(const srs_ctx* c) {
   switch (c->tile) {
-    case 128: return pick3<128, CMP, KG>(c->nt_geometry, c->nt_state, c->occ8, c->lds);
-    case 512: return pick3<512, CMP, KG>(c->nt_geometry, c->nt_state, c->occ8, c->lds);
-    default: return pick3<256, CMP, KG>(c->nt_geometry, c->nt_state, c->occ8, c->lds);
+    case 128: return pick3<128, CMP, KG>(c->nt_geometry, c->nt_state, c->occ, c->lds);
+    case 512: return pick3<512, CMP, KG>(c->nt_geometry, c->nt_state, c->occ, c->lds);
+    default: return pick3<256, CMP, KG>(c->nt_geometry, c->nt_state, c->occ, c->lds);
   }
 }
 
@@ -905,7 +963,10 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
       c->compact = value != 0;
       return SRS_OK;
     case SRS_TUNE_NT_GEOMETRY: c->nt_geometry = value != 0; return SRS_OK;
-    case SRS_TUNE_OCCUPANCY: c->occ8 = value == 8; return SRS_OK;
+    case SRS_TUNE_OCCUPANCY:
+      if (value != 0 && value != 6 && value != 8) return srs::fail(SRS_E_ARG, "occupancy must be 0, 6 or 8");
+      c->occ = value;
+      return SRS_OK;
     case SRS_TUNE_NT_STATE: c->nt_state = value != 0; return SRS_OK;
     case SRS_TUNE_LDS: c->lds = value != 0; return SRS_OK;
     case SRS_TUNE_DROP_FINISHED:

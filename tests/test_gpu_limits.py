@@ -50,6 +50,8 @@ def test_call_order_and_argument_errors():
         ctx.finalize()
     with pytest.raises(S.SrsError, match="tile"):
         ctx.tune(tile=256)
+    with pytest.raises(S.SrsError, match="occupancy"):
+        ctx.tune(occupancy=7)
     with pytest.raises(S.SrsError, match="out of range"):
         ctx.state_get(3)
     with pytest.raises(S.SrsError, match="negative"):
