@@ -557,8 +557,8 @@ def main():
             "group": b["tune"].get("group", 1), "tuning": b["tune"],
             "sweep_ms": b["sweep_ms"], "sweep_gbs": bach, "sweep_frac": bach / HBM_PEAK_GBS,
             "sweep_traffic": {"bytes": profiled_traffic(os.path.join(ROOT, "profiles", "k_sweep_batched_traffic.md")),
-                              "profiled_tuning": {"tile": 128, "compact": 1, "group": 4, "nt_geometry": 0,
-                                                  "nt_state": 1}},
+                              "profiled_tuning": {"tile": 256, "compact": 1, "group": 1, "nt_geometry": 0,
+                                                  "nt_state": 1, "occupancy": 6, "lds": 1}},
             "plain_sequential_value": b["plain_rate"], "speedup_vs_plain_sequential": brate / b["plain_rate"],
             "plain_per_mesh": {"h40": b["plain_per_mesh"][0], "h30": b["plain_per_mesh"][1],
                                "h25": b["plain_per_mesh"][2]},
