@@ -462,7 +462,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--batch", type=int, default=2048, help="realizations in the batched leg (0: skip)")
-    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-group", type=int, default=0, choices=(0, 1, 2, 4),
                     help="realizations per realization group in the batched leg (0: autotune the batched sweep)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work (0: skip)")
@@ -545,7 +545,7 @@ def main():
                                "sample": f"{n} ExplicitStep calls with gating on the same {N}-element mesh"}
         log(f"cpu {kind}: {n} steps in {cwall:.2f} s")
     if args.batch > 0:
-        b = bench_batched(args.batch, args.batch_steps, 2, args.order,
+        b = bench_batched(args.batch, args.batch_steps, args.batch_steps, args.order,
                           {**tune, "group": args.batch_group} if args.batch_group else None)
         brate = b["sim_ms"] / b["wall"]
         bach = b["nbytes"] / (b["sweep_ms"] * 1e-3) / 1e9

@@ -170,11 +170,13 @@ def test_protocol_loop_records_match_oracle_loop():
     np.testing.assert_array_equal(recs_d, np.array(recs_o))
 
 
+@pytest.mark.parametrize("lds", [0, 1])
 @pytest.mark.parametrize("case", ["opens", "t_final"])
-def test_ffwd_gating_phase_matches_oracle_loop(case):
+def test_ffwd_gating_phase_matches_oracle_loop(case, lds):
     """flag_ffwd: gating-only steps on the unchanged fields until a channel
     opens (or T_FINAL passes), then the main loop from that time
-    (cpu/RunSimulation.cpp:469-519)."""
+    (cpu/RunSimulation.cpp:469-519).  Both sweep kernels skip the
+    realization while it is in the ffwd phase and after it is done."""
     pr = _tiny(tt=True)
     p = pr.params.copy()
     dt = p.reals[1]
@@ -188,6 +190,8 @@ def test_ffwd_gating_phase_matches_oracle_loop(case):
     O = Oracle()
     recs_o, t_end, nsteps = oracle_loop(O, pr, p, F.copy(), key=33, stream=0)
     with gpu_context([pr], [F.copy()], params=[p], seeds=[33], record_capacity=256) as ctx:
+        if lds:
+            ctx.tune(lds=1, occupancy=6)
         ctx.loop_set(0, n_open=0, ffwd=1)
         ctx.explicit_step(nsteps + 5, S.STEP_PROTOCOL | S.STEP_GRAPH)
         recs_d = ctx.records(0)
