@@ -247,9 +247,11 @@ int srs_ctx_create(int device, srs_ctx** out);
 /* Kernel and layout tunables (no reference counterpart; the reference fixes
  * 256-thread blocks, gpu/RunSimulation.cu:784).  They change speed only,
  * never results.  Layout keys must be set before the first mesh is added,
- * the tile before srs_ctx_finalize; the rest at any time. */
-#define SRS_TUNE_TILE 1          /* elements per sweep workgroup: 128, 256 (default), 512 */
-#define SRS_TUNE_NT_GEOMETRY 2   /* 1: non-temporal loads of operator rows and flags */
+ * the tile before srs_ctx_finalize; the rest at any time.  The defaults are
+ * the fastest setting measured on the 3M-element dyad mesh in either element
+ * order, short of the register budget (which helps in the block order only). */
+#define SRS_TUNE_TILE 1          /* elements per sweep workgroup: 128, 256, 512 (default) */
+#define SRS_TUNE_NT_GEOMETRY 2   /* 1 (default): non-temporal loads of operator rows and flags */
 #define SRS_TUNE_NT_STATE 3      /* 1: non-temporal stores of the new state */
 #define SRS_TUNE_PLANE_PAD 4     /* extra elements (multiple of 64) between SoA planes */
 #define SRS_TUNE_COMPACT 5       /* 1 (default): store operator rows as class + int16 ulp deltas and
@@ -259,8 +261,8 @@ int srs_ctx_create(int device, srs_ctx** out);
 #define SRS_TUNE_GROUP 7         /* 1 (default), 2 or 4: realizations of one mesh whose states are interleaved
                                     per element so that each sweep tile loads the operator rows once for all
                                     of them (set before srs_ctx_finalize) */
-#define SRS_TUNE_LDS 8           /* 1: stage each sweep tile's diffusing species in LDS and read neighbours
-                                    inside the tile from there (groups of 1) */
+#define SRS_TUNE_LDS 8           /* 1 (default): stage each sweep tile's diffusing species in LDS and read
+                                    neighbours inside the tile from there (groups of 1) */
 #define SRS_TUNE_DROP_FINISHED 9 /* 1 (default): srs_explicit_step leaves the tiles of groups whose
                                     realizations have all finished out of the sweep; 0: sweep them all */
 int srs_ctx_tune(srs_ctx* ctx, int32_t key, int32_t value);

@@ -266,14 +266,14 @@ using namespace srs_dev;
 
 struct srs_ctx {
   int device = 0;
-  int tile = 256;        // elements per k_sweep workgroup
-  int nt_geometry = 0;   // non-temporal loads of operator rows and flags
+  int tile = 512;        // elements per k_sweep workgroup
+  int nt_geometry = 1;   // non-temporal loads of operator rows and flags
   int nt_state = 0;      // non-temporal stores of the new state
   int plane_pad = 0;     // extra elements per SoA plane
   int compact = 1;       // compress operator rows when a mesh allows it
   int occ = 0;           // sweep register budget: 0 compiler's choice, 6 or 8 wavefronts per SIMD
   int group = 1;         // realizations per realization group (1, 2 or 4)
-  int lds = 0;           // stage each tile's quads in LDS for the neighbour gathers (groups of 1)
+  int lds = 1;           // stage each tile's quads in LDS for the neighbour gathers (groups of 1)
   int drop_finished = 1; // sweep without the tiles of groups that have all finished
   bool compact_all = false;
   bool layout_fixed = false;

@@ -476,7 +476,8 @@ def test_finished_realizations_leave_the_sweep(group):
                                  dict(group=2), dict(group=4, compact=0, occupancy=8), dict(lds=1),
                                  dict(lds=1, tile=128, compact=0, nt_state=1), dict(lds=1, tile=512, nt_geometry=1),
                                  dict(lds=1, tile=512, occupancy=6), dict(lds=1, tile=256, occupancy=8),
-                                 dict(tile=256, occupancy=6), dict(group=2, occupancy=6)])
+                                 dict(tile=256, occupancy=6), dict(group=2, occupancy=6),
+                                 dict(tile=256, nt_geometry=0, lds=0)])
 def test_tunables_do_not_change_results(cfg):
     pr = _tiny(tt=True)
     p = fast_gating(pr.params)
