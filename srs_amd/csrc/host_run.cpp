@@ -13,6 +13,7 @@
 #include <sys/stat.h>
 #include <sys/types.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -20,6 +21,7 @@
 #include <cstring>
 #include <ctime>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "kinetics.h"
@@ -48,6 +50,44 @@ struct Guard {
     if (grid) srs_grid_free(grid);
   }
 };
+
+// Device element order for the drop-in: elements sorted along a Morton
+// (Z-order) curve through their centroids on a 1024^3 grid over the mesh's
+// bounding box, so that a sweep tile covers a compact patch of the mesh and
+// its neighbour gathers stay in cache.  The order changes where elements are
+// stored, never the arithmetic.  Returns perm[k] = element stored at k.
+std::vector<int32_t> morton_order(const srs_grid* g) {
+  const int n = g->n_ele;
+  std::vector<double> cen(3 * (size_t)n);
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int e = 0; e < n; ++e)
+    for (int a = 0; a < 3; ++a) {
+      double c = 0.0;
+      for (int q = 0; q < 4; ++q) c += g->node_xyz[3 * (size_t)g->ele_nodes[4 * (size_t)e + q] + a];
+      c *= 0.25;
+      cen[3 * (size_t)e + a] = c;
+      lo[a] = std::min(lo[a], c);
+      hi[a] = std::max(hi[a], c);
+    }
+  auto spread = [](uint64_t v) {  // bit b of v to bit 3b
+    uint64_t r = 0;
+    for (int b = 0; b < 10; ++b) r |= ((v >> b) & 1u) << (3 * b);
+    return r;
+  };
+  std::vector<std::pair<uint64_t, int32_t>> key(n);
+  for (int e = 0; e < n; ++e) {
+    uint64_t k = 0;
+    for (int a = 0; a < 3; ++a) {
+      const double w = hi[a] > lo[a] ? (cen[3 * (size_t)e + a] - lo[a]) / (hi[a] - lo[a]) : 0.0;
+      k |= spread((uint64_t)std::lround(w * 1023.0)) << a;
+    }
+    key[e] = {k, e};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> perm(n);
+  for (int k = 0; k < n; ++k) perm[k] = key[k].second;
+  return perm;
+}
 
 // cpu/OutputFuncs.cpp:42-220, streamed instead of a fixed 10 MB buffer.
 int write_vtk(const std::string& path, int run, int rank, int n, double t, const srs_grid* g, const double* X,
@@ -154,7 +194,9 @@ extern "C" int srs_run_cru3d(const char* param_path, const char* mesh_base, cons
   const int n_sims = I[SRS_I_N_SIMS] - start_index;
   if (n_sims <= 0) return SRS_OK;
   if ((rc = srs_ctx_create(I[SRS_I_CUDA_DEVICE], &G.ctx))) return rc;
-  srs_mesh_desc md{N, aij.data(), nij.data(), br.data(), g->domain, g->tropc, g->V0, g->tt_area, SRS_ORDER_RCM, nullptr};
+  const std::vector<int32_t> perm = morton_order(g);
+  srs_mesh_desc md{N, aij.data(), nij.data(), br.data(), g->domain, g->tropc, g->V0, g->tt_area, SRS_ORDER_CUSTOM,
+                   perm.data()};
   int32_t mesh_id;
   if ((rc = srs_ctx_add_mesh(G.ctx, &md, &mesh_id))) return rc;
   srs_chan_desc cd{g->n_ryr, g->ryr_ele, g->ryr_jsr, g->ryr_nb, g->n_lcc, g->lcc_ele};
