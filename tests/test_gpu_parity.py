@@ -367,6 +367,52 @@ def test_cru3d_run_matches_oracle_loop(tmp_path, ffwd):
     assert nsteps > 100
 
 
+
+def test_cru3d_ensemble_runs_match_oracle_loops(tmp_path):
+    """sims_per_proc > 1: the driver runs the ensemble's realizations together
+    in one context (cpu/main.cpp:209-258 runs them one after another); run k
+    has seed 93843 + seed_base + k seed_B and its own Philox stream, and each
+    states file equals the oracle loop of that run alone."""
+    from types import SimpleNamespace
+
+    base = str(tmp_path / "mesh")
+    S.box_mesh_write(W.tiny_box_spec(tt=True), base)
+    g, _ = S.grid_read(base)
+    aij, nij = S.assemble_operator(g)
+    dt = W.stable_dt(aij, 0.25)
+    entries = dict(t_step=repr(dt * 1e6), t_final=repr(200 * dt), states_interval=repr(7 * dt),
+                   RYR_K_PLUS="0.5", RYR_K_MINUS="20", flag_use_clock="false", seed_base="4321", seed_B="83",
+                   flag_output_ryr_states="true", sims_per_proc="3", paramset_title="tiny", protocol="none",
+                   ryr_open_init="1", ryr_open_time=repr(40 * dt))
+    xml = str(tmp_path / "p.xml")
+    _write_params(xml, entries)
+    out = str(tmp_path / "out")
+    S.run_cru3d(xml, base, out)
+    p = S.read_parameters(xml)
+    P = p.reals
+    O = Oracle()
+    nr, nl = g.n_ryr, g.n_lcc
+    X0, _ = S.equilibrium_state(g, P)
+    G = grid_dict(g, aij, nij)
+    pr = SimpleNamespace(grid=g, aij=aij, nij=nij)
+    finals = []
+    for run in range(3):
+        recs_d = np.fromfile(os.path.join(out, f"states_0_{run}.out"), np.float32).reshape(-1, 7 + nr)
+        seed = (93843 + 4321 + 83 * run) & 0xFFFFFFFF
+        rs = np.zeros(nr, np.int32)
+        rs[1] = 1
+        R = np.array([np.log(O.open01(O.philox([0, i, run, 1], [seed, PHILOX_KEY1])[0])) for i in range(nr + nl)])
+        F = Fields(X0.copy(), rs, np.ones(nl, np.int32), np.full(nl, 2, np.int32), R,
+                   [P[S.R["DEFAULT_VM"]], P[S.R["DEFAULT_CA_NSR"]]])
+        rec0 = O.record(0.0, G, F, P, ryr_out=True)
+        recs_o, _, nsteps = oracle_loop(O, pr, p, F, key=seed, stream=run)
+        exp = np.array([rec0] + recs_o)
+        assert recs_d.shape == exp.shape and nsteps > 100
+        np.testing.assert_array_equal(recs_d, exp)
+        finals.append(recs_d)
+    # the runs' own draws: their RyR trajectories differ
+    assert not np.array_equal(finals[0][:, 7:], finals[1][:, 7:])
+
 def test_spark_box_replay_full_mesh():
     """The 56,250-tet spark box, replayed schedule, 200 steps."""
     pr = W.prepare_spark()
