@@ -64,6 +64,7 @@ TUNE_OCCUPANCY = 6
 TUNE_GROUP = 7
 TUNE_LDS = 8
 TUNE_DROP_FINISHED = 9
+TUNE_GATED_SWEEP = 10
 
 
 def _p(a, t):
@@ -329,10 +330,11 @@ class Context:
     def tune(self, **kw):
         """Sweep tunables (srs_ctx_tune): tile=, plane_pad=, compact=, group=
         before the meshes / srs_ctx_finalize; nt_geometry=, nt_state=,
-        occupancy=, lds=, drop_finished= at any time.  None changes results."""
+        occupancy=, lds=, drop_finished=, gated_sweep= at any time.  None changes
+        results."""
         keys = dict(tile=TUNE_TILE, nt_geometry=TUNE_NT_GEOMETRY, nt_state=TUNE_NT_STATE, plane_pad=TUNE_PLANE_PAD,
                     compact=TUNE_COMPACT, occupancy=TUNE_OCCUPANCY, group=TUNE_GROUP, lds=TUNE_LDS,
-                    drop_finished=TUNE_DROP_FINISHED)
+                    drop_finished=TUNE_DROP_FINISHED, gated_sweep=TUNE_GATED_SWEEP)
         for k, v in kw.items():
             check(lib().srs_ctx_tune(self._h, keys[k], int(v)))
         return self

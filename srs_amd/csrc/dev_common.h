@@ -275,6 +275,7 @@ struct srs_ctx {
   int group = 1;         // realizations per realization group (1, 2 or 4)
   int lds = 1;           // stage each tile's quads in LDS for the neighbour gathers (groups of 1)
   int drop_finished = 1; // sweep without the tiles of groups that have all finished
+  int gated = 1;         // fold each step's gating into the next step's sweep (k_sweep_gated) where it applies
   bool compact_all = false;
   bool layout_fixed = false;
   hipStream_t stream = nullptr;

@@ -20,6 +20,8 @@
 // adjoint.hip; the shared types are in dev_common.h.
 #include "dev_common.h"
 
+#include <functional>
+
 namespace srs_dev {
 
 // The state of one element of one realization: its own and its neighbours'
@@ -242,7 +244,7 @@ __device__ __forceinline__ uint64_t geo_fold(const RealDesc& d) {
 // Single-realization groups with LDS staging: the tile's own quads are
 // loaded coalesced into LDS, and neighbour quads inside the tile come from
 // there instead of through the L1 cache.
-template <int BS, bool NTG, bool NTS, bool COMPACT>
+template <int BS, bool NTG, bool NTS, bool COMPACT, bool SKIP_CHAN = false>
 __device__ __forceinline__ void sweep_tile_lds(const RealDesc& d, const RealLive& l, const Bufs bf, int t) {
   __shared__ d4 tq[BS];
   const int base = t * BS;
@@ -262,6 +264,7 @@ __device__ __forceinline__ void sweep_tile_lds(const RealDesc& d, const RealLive
   tq[threadIdx.x] = m0;
   __syncthreads();
   if (!in) return;
+  if (SKIP_CHAN && (g.f & F_CHAN)) return;  // updated by the gating workgroup (sweep_gated)
   const bool plain = (g.f & F_SPECIAL) == 0 && (g.f & F_TROPC);
   if (__builtin_amdgcn_ballot_w64(!plain) == 0) {
     const Loc c = load_loc_lds<F_TROPC, BS>(d, bf.xo, j, g, tq, base, m0, tr);
@@ -384,10 +387,13 @@ __device__ void write_record(const RealDesc& d, RealLive& l, const double* Ca, d
   l.rec_count++;
 }
 
-__global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restrict__ D, RealLive* __restrict__ L,
-                                                       uint32_t mode) {
-  const RealDesc& d = D[blockIdx.x];
-  RealLive& l = L[blockIdx.x];
+// The gating and loop body of realization r by one workgroup of NT threads.
+template <int NT>
+__device__ __forceinline__ void gate_workgroup(const RealDesc* __restrict__ D, RealLive* __restrict__ L, const int r,
+                                               const uint32_t mode) {
+  static_assert(MAX_RYR % NT == 0 && MAX_LCC % NT == 0, "channel slots per thread");
+  const RealDesc& d = D[r];
+  RealLive& l = L[r];
   if (l.done) return;
   __shared__ int s_old[MAX_RYR];
   __shared__ int s_a[MAX_RYR];
@@ -417,11 +423,11 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
     // (RyR i by thread i mod 256, LCC i by thread 255 - i mod 256, so RyRs
     // and LCCs are updated side by side), leaving one memory round trip in
     // front of the arithmetic.
-    constexpr int RPT = MAX_RYR / GATE_THREADS, LPT = MAX_LCC / GATE_THREADS;
+    constexpr int RPT = MAX_RYR / NT, LPT = MAX_LCC / NT;
     double ca_c[RPT], ca_j[RPT], Rr[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
-      const int i = tid + k * GATE_THREADS;
+      const int i = tid + k * NT;
       if (i < nr) {
         const int so = d.ryr_state[i];
         s_old[i] = so;
@@ -435,7 +441,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
     double ca_l[LPT], Rl[LPT];
 #pragma unroll
     for (int k = 0; k < LPT; ++k) {
-      const int i = (GATE_THREADS - 1 - tid) + k * GATE_THREADS;
+      const int i = (NT - 1 - tid) + k * NT;
       if (i < d.n_lcc) {
         ls[k] = d.lcc_state[i];
         lvs[k] = d.lccv_state[i];
@@ -457,7 +463,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
         int changed = 0;
 #pragma unroll
         for (int k = 0; k < RPT; ++k) {
-          const int i = tid + k * GATE_THREADS;
+          const int i = tid + k * NT;
           if (i >= nr) continue;
           int no = 0, ncl = 0;
           for (int q = 0; q < 4; ++q) {
@@ -482,7 +488,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
     // the neighbour counts do not enter the rate, A* = 0 gives X = 1).
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
-      const int i = tid + k * GATE_THREADS;
+      const int i = tid + k * NT;
       if (i >= nr) continue;
       int no = 0, ncl = 0;
       if (coupled) {
@@ -509,7 +515,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
     // LCCs, cpu/RunSimulation.cpp:229-400
 #pragma unroll
     for (int k = 0; k < LPT; ++k) {
-      const int i = (GATE_THREADS - 1 - tid) + k * GATE_THREADS;
+      const int i = (NT - 1 - tid) + k * NT;
       if (i >= d.n_lcc) continue;
       const int s = ls[k], vs = lvs[k];
       double tr[3];
@@ -535,7 +541,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
   if (mode & G_REPLAY) {
     const int64_t k = step + 1 - l.sched_base;  // schedule row of the next step
     if (k < d.n_sched)
-      for (int i = tid; i < nr; i += GATE_THREADS) d.ryr_state[i] = d.sched[k * nr + i];
+      for (int i = tid; i < nr; i += NT) d.ryr_state[i] = d.sched[k * nr + i];
   }
 
   const double iv = P[SRS_R_OUTPUT_STATES_INTERVAL];
@@ -548,11 +554,11 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
   int n_open_all = 0, n_lcc_all = 0;
   double cmax_all = 0.0;
   if ((mode & G_PROTOCOL) && (ff || out_due)) {
-    __shared__ int s_cnt[2][GATE_THREADS / 64];
-    __shared__ double s_max[GATE_THREADS / 64];
+    __shared__ int s_cnt[2][NT / 64];
+    __shared__ double s_max[NT / 64];
     int cr = 0, cl = 0;
     double m = 0.0;
-    for (int b = 0; b < nr; b += GATE_THREADS) {
+    for (int b = 0; b < nr; b += NT) {
       const int i = b + tid;
       const bool open = i < nr && d.ryr_state[i] != 0;
       cr += __builtin_popcountll(__builtin_amdgcn_ballot_w64(open));
@@ -561,8 +567,8 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
         if (c > m) m = c;
       }
     }
-    for (int b = 0; b < d.n_lcc; b += GATE_THREADS) {
-      const int i = b + (GATE_THREADS - 1 - tid);
+    for (int b = 0; b < d.n_lcc; b += NT) {
+      const int i = b + (NT - 1 - tid);
       const bool open = i < d.n_lcc && d.lccv_state[i] == 2 && (d.lcc_state[i] == 6 || d.lcc_state[i] == 12);
       cl += __builtin_popcountll(__builtin_amdgcn_ballot_w64(open));
     }
@@ -578,7 +584,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
     }
     __syncthreads();
     if (tid == 0)
-      for (int w = 0; w < GATE_THREADS / 64; ++w) {
+      for (int w = 0; w < NT / 64; ++w) {
         n_open_all += s_cnt[0][w];
         n_lcc_all += s_cnt[1][w];
         if (s_max[w] > cmax_all) cmax_all = s_max[w];
@@ -629,6 +635,56 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
     l.ca_max_last = cmax_all;
     write_record(d, l, Ca, t);
   }
+}
+
+__global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restrict__ D, RealLive* __restrict__ L,
+                                                       uint32_t mode) {
+  gate_workgroup<GATE_THREADS>(D, L, blockIdx.x, mode);
+}
+
+// One step of a single-realization context with the previous step's gating
+// folded in (cpu/RunSimulation.cpp:650-652 then :591-648).  Workgroup 0 gates
+// on the state this step reads, which is the previous step's result, exactly
+// as k_gate would after that step, then updates the elements that carry
+// channel flux: only their rates read the channel states.  The other
+// workgroups sweep the tiles without those elements.  cur, the state buffer
+// this step reads, is an argument because workgroup 0 flips the live index
+// while the others run; nothing else they read changes without G_PROTOCOL.
+// gmode 0 skips the gating (the first step of a block).
+template <int BS, bool NTG, bool NTS, bool COMPACT>
+__device__ __forceinline__ void sweep_gated(const RealDesc* __restrict__ D, RealLive* __restrict__ L, int n_tiles,
+                                            int cur, uint32_t gmode) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  if (b > n_tiles) return;  // b = 0: the gating workgroup; b = 1..n_tiles: tile b - 1
+  const RealDesc& d = D[0];
+  const RealLive& l = L[0];
+  const int skip = l.done | l.ffwd;
+  const Bufs bf = bufs(d.X[0], d.X[1], cur);
+  scalars_here(fold(skip, bf.xo, bf.xn) ^ geo_fold<COMPACT>(d));
+  if (b == 0) {
+    if (gmode) gate_workgroup<BS>(D, L, 0, gmode);
+    __syncthreads();  // the new channel states, written by this workgroup, before the flux reads them
+    if (skip) return;
+    for (int k = threadIdx.x; k < d.n_celem; k += BS) {
+      const int j = gp(d.celem)[k];
+      const Geo g = load_geo<NTG, COMPACT, -1>(d, j);
+      const Loc c = load_loc<-1>(d, bf.xo, j, g);
+      step_elem<NTS, -1>(d, l, bf, j, g, c);
+    }
+    return;
+  }
+  if (skip) return;
+  sweep_tile_lds<BS, NTG, NTS, COMPACT, true>(d, l, bf, b - 1);
+}
+template <int BS, bool NTG, bool NTS, bool COMPACT>
+__global__ __launch_bounds__(BS) void k_sweep_gated(const RealDesc* __restrict__ D, RealLive* __restrict__ L,
+                                                    int n_tiles, int cur, uint32_t gmode) {
+  sweep_gated<BS, NTG, NTS, COMPACT>(D, L, n_tiles, cur, gmode);
+}
+template <int BS, bool NTG, bool NTS, bool COMPACT, int W>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_sweep_gated_w(
+    const RealDesc* __restrict__ D, RealLive* __restrict__ L, int n_tiles, int cur, uint32_t gmode) {
+  sweep_gated<BS, NTG, NTS, COMPACT>(D, L, n_tiles, cur, gmode);
 }
 
 // Gathers Ca at the channel elements of one realization.
@@ -810,6 +866,46 @@ int launch_step(srs_ctx* c, uint32_t gmode) {
   return SRS_OK;
 }
 
+using GatedFn = void (*)(const RealDesc*, RealLive*, int, int, uint32_t);
+
+template <int BS, bool CMP, int W>
+GatedFn pick_gated_w(int ntg, int nts) {
+  if constexpr (W == 0) {
+    if (ntg) return nts ? k_sweep_gated<BS, true, true, CMP> : k_sweep_gated<BS, true, false, CMP>;
+    return nts ? k_sweep_gated<BS, false, true, CMP> : k_sweep_gated<BS, false, false, CMP>;
+  } else {
+    if (ntg) return nts ? k_sweep_gated_w<BS, true, true, CMP, W> : k_sweep_gated_w<BS, true, false, CMP, W>;
+    return nts ? k_sweep_gated_w<BS, false, true, CMP, W> : k_sweep_gated_w<BS, false, false, CMP, W>;
+  }
+}
+
+template <bool CMP>
+GatedFn pick_gated_tile(const srs_ctx* c) {
+  const bool w6 = c->occ == 6;
+  switch (c->tile) {
+    case 128: return w6 ? pick_gated_w<128, CMP, 6>(c->nt_geometry, c->nt_state) : pick_gated_w<128, CMP, 0>(c->nt_geometry, c->nt_state);
+    case 512: return w6 ? pick_gated_w<512, CMP, 6>(c->nt_geometry, c->nt_state) : pick_gated_w<512, CMP, 0>(c->nt_geometry, c->nt_state);
+    default: return w6 ? pick_gated_w<256, CMP, 6>(c->nt_geometry, c->nt_state) : pick_gated_w<256, CMP, 0>(c->nt_geometry, c->nt_state);
+  }
+}
+
+// One step through k_sweep_gated: tile b - 1 in workgroup b, workgroup 0
+// gates (gmode) and updates the channel elements.
+void launch_gated(srs_ctx* c, int cur, uint32_t gmode) {
+  const GatedFn f = c->compact_all ? pick_gated_tile<true>(c) : pick_gated_tile<false>(c);
+  const int grid = (c->n_tiles + 1 + NXCD - 1) / NXCD * NXCD;
+  hipLaunchKernelGGL(f, dim3(grid), dim3(c->tile), 0, c->stream, c->d_desc, c->d_live, c->n_tiles, cur, gmode);
+}
+
+// m steps through k_sweep_gated: the first without gating (nothing is
+// pending when a block starts), each later one gating the step before it,
+// then k_gate for the last step, so that a block leaves no gating pending.
+// cur is the live state index when the block starts.
+void launch_gated_block(srs_ctx* c, int64_t m, int cur) {
+  for (int64_t k = 0; k < m; ++k) launch_gated(c, cur ^ (int)(k & 1), k ? G_AFTER_SWEEP : 0u);
+  hipLaunchKernelGGL(k_gate, dim3(1), dim3(GATE_THREADS), 0, c->stream, c->d_desc, c->d_live, (uint32_t)G_AFTER_SWEEP);
+}
+
 // Captured step blocks hold the sweep's tile list and grid; they are dropped
 // when either changes.
 int drop_graphs(srs_ctx* c) {
@@ -969,6 +1065,9 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
       return SRS_OK;
     case SRS_TUNE_NT_STATE: c->nt_state = value != 0; return SRS_OK;
     case SRS_TUNE_LDS: c->lds = value != 0; return SRS_OK;
+    case SRS_TUNE_GATED_SWEEP:
+      c->gated = value != 0;
+      return c->finalized ? drop_graphs(c) : SRS_OK;
     case SRS_TUNE_DROP_FINISHED:
       c->drop_finished = value != 0;
       return c->finalized ? tiles_full(c) : SRS_OK;
@@ -1471,13 +1570,17 @@ int srs_checkpoint_load(srs_ctx* c, int32_t r, const void* buf, size_t bytes) {
 
 // The captured block of `block` steps in gate mode `gmode` (made on first use
 // and cached; tiles_compact and tiles_full drop the cache).
-static int graph_for(srs_ctx* c, int64_t block, uint32_t gmode, hipGraphExec_t* out) {
-  auto key = std::make_pair(block, gmode);
+// Graph-cache key bits of the gated blocks (host only, never passed to a kernel).
+constexpr uint32_t G_KEY_GATED = 1u << 8, G_KEY_CUR1 = 1u << 9;
+
+static int graph_cached(srs_ctx* c, int64_t block, uint32_t key_mode, const std::function<void()>& capture,
+                        hipGraphExec_t* out) {
+  auto key = std::make_pair(block, key_mode);
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int64_t k = 0; k < block; ++k) launch_step(c, gmode);
+    capture();
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     hipGraphExec_t ge;
     HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -1488,7 +1591,36 @@ static int graph_for(srs_ctx* c, int64_t block, uint32_t gmode, hipGraphExec_t* 
   return SRS_OK;
 }
 
+static int graph_for(srs_ctx* c, int64_t block, uint32_t gmode, hipGraphExec_t* out) {
+  return graph_cached(c, block, gmode, [&] { for (int64_t k = 0; k < block; ++k) launch_step(c, gmode); }, out);
+}
+
+// A block of steps through k_sweep_gated starting from live state index cur.
+static int graph_gated(srs_ctx* c, int64_t block, int cur, hipGraphExec_t* out) {
+  return graph_cached(c, block, G_AFTER_SWEEP | G_KEY_GATED | (cur ? G_KEY_CUR1 : 0u),
+                      [&] { launch_gated_block(c, block, cur); }, out);
+}
+
+// Whether steps without the loop body run through k_sweep_gated, and the live
+// state index they start from: single-realization contexts with the LDS
+// sweep of groups of one and a register budget of 0 or 6, while the
+// realization is neither finished nor in the ffwd phase (neither changes
+// without the loop body).  Reading the live state waits for the stream.
+static int gated_start(srs_ctx* c, uint32_t flags, bool* ok, int* cur) {
+  *ok = false;
+  if (!c->gated || c->reals.size() != 1 || c->group != 1 || !c->lds || c->occ == 8 || (flags & SRS_STEP_PROTOCOL))
+    return SRS_OK;
+  RealLive lv;
+  int rc = live_rw(c, 0, &lv, false);
+  if (rc) return rc;
+  if (lv.done || lv.ffwd || c->n_tiles == 0) return SRS_OK;
+  *ok = true;
+  *cur = lv.cur;
+  return SRS_OK;
+}
+
 constexpr int64_t kGraphBlock = 128, kGraphMin = 16;
+static_assert(kGraphBlock % 2 == 0, "a full gated block returns the live state index to where it started");
 
 int srs_explicit_step_prepare(srs_ctx* c, int64_t n_steps, uint32_t flags) {
   srs::clear_error();
@@ -1500,6 +1632,17 @@ int srs_explicit_step_prepare(srs_ctx* c, int64_t n_steps, uint32_t flags) {
   int rc = tiles_compact(c);
   if (rc) return rc;
   hipGraphExec_t ge;
+  bool gated;
+  int cur;
+  if ((rc = gated_start(c, flags, &gated, &cur))) return rc;
+  if (gated) {
+    // both starting live indices: the one a later call starts from is not known here
+    for (int p = 0; p < 2; ++p) {
+      if (n_steps >= kGraphBlock && (rc = graph_gated(c, kGraphBlock, p, &ge))) return rc;
+      if (n_steps % kGraphBlock >= kGraphMin && (rc = graph_gated(c, n_steps % kGraphBlock, p, &ge))) return rc;
+    }
+    return SRS_OK;
+  }
   if (n_steps >= kGraphBlock && (rc = graph_for(c, kGraphBlock, gmode, &ge))) return rc;
   if (n_steps % kGraphBlock >= kGraphMin && (rc = graph_for(c, n_steps % kGraphBlock, gmode, &ge))) return rc;
   return SRS_OK;
@@ -1514,22 +1657,31 @@ int srs_explicit_step(srs_ctx* c, int64_t n_steps, uint32_t flags) {
   const uint32_t gmode = G_AFTER_SWEEP | ((flags & SRS_STEP_PROTOCOL) ? G_PROTOCOL : 0u);
   int rc = tiles_compact(c);
   if (rc) return rc;
+  bool gated = false;
+  int cur = 0;
+  if (n_steps > 0 && (rc = gated_start(c, flags, &gated, &cur))) return rc;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   int64_t done = 0;
   if (flags & SRS_STEP_GRAPH) {
     // 128-step blocks, then the rest as one block of its own when it is long enough
     hipGraphExec_t ge;
     if (n_steps >= kGraphBlock) {
-      if ((rc = graph_for(c, kGraphBlock, gmode, &ge))) return rc;
+      if ((rc = gated ? graph_gated(c, kGraphBlock, cur, &ge) : graph_for(c, kGraphBlock, gmode, &ge))) return rc;
       for (; n_steps - done >= kGraphBlock; done += kGraphBlock) HIPCHK(hipGraphLaunch(ge, c->stream));
     }
     if (n_steps - done >= kGraphMin) {
-      if ((rc = graph_for(c, n_steps - done, gmode, &ge))) return rc;
+      const int64_t m = n_steps - done;
+      if ((rc = gated ? graph_gated(c, m, cur, &ge) : graph_for(c, m, gmode, &ge))) return rc;
       HIPCHK(hipGraphLaunch(ge, c->stream));
+      cur ^= (int)(m & 1);
       done = n_steps;
     }
   }
-  for (; done < n_steps; ++done) launch_step(c, gmode);
+  if (gated) {
+    if (done < n_steps) launch_gated_block(c, n_steps - done, cur);
+  } else {
+    for (; done < n_steps; ++done) launch_step(c, gmode);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   HIPCHK(hipEventSynchronize(c->ev1));
@@ -1659,15 +1811,24 @@ int srs_time_sweep(srs_ctx* c, int32_t reps, double* sweep_ms, double* gating_ms
   if (!c || !c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
   if (reps <= 0) return srs::fail(SRS_E_ARG, "reps must be > 0");
   HIPCHK(hipSetDevice(c->device));
+  bool gated;
+  int cur;
+  int rc = gated_start(c, 0, &gated, &cur);
+  if (rc) return rc;
   std::vector<hipEvent_t> ev(3 * reps);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   const uint32_t gmode = G_AFTER_SWEEP;
   for (int k = 0; k < reps; ++k) {
     HIPCHK(hipEventRecord(ev[3 * k], c->stream));
-    launch_sweep(c);
+    // the kernels srs_explicit_step runs: with the gated sweep, each launch
+    // also gates the step before it, and the gating after the last one is
+    // the whole gating time of the reps
+    if (gated) launch_gated(c, cur ^ (k & 1), k ? gmode : 0u);
+    else launch_sweep(c);
     HIPCHK(hipEventRecord(ev[3 * k + 1], c->stream));
-    hipLaunchKernelGGL(k_gate, dim3((unsigned)c->reals.size()), dim3(GATE_THREADS), 0, c->stream, c->d_desc,
-                       c->d_live, gmode);
+    if (!gated || k == reps - 1)
+      hipLaunchKernelGGL(k_gate, dim3((unsigned)c->reals.size()), dim3(GATE_THREADS), 0, c->stream, c->d_desc,
+                         c->d_live, gmode);
     HIPCHK(hipEventRecord(ev[3 * k + 2], c->stream));
   }
   HIPCHK(hipGetLastError());

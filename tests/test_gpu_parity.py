@@ -546,6 +546,46 @@ def test_tunables_do_not_change_results(cfg):
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
+
+@pytest.mark.parametrize("cfg", [dict(tile=512), dict(tile=256, occupancy=6), dict(tile=128, nt_state=1, nt_geometry=0)])
+def test_gated_sweep_equals_separate_gating(cfg):
+    """Each step's gating run inside the next step's sweep launch
+    (k_sweep_gated) against a gating launch after every sweep: identical
+    fields, channel states, residuals and loop state after graph blocks of
+    128 and of odd length, single launches and timed steps, with RyRs and
+    LCCs flipping throughout."""
+    pr = _tiny(tt=True)
+    p = fast_gating(pr.params)
+    F = initial_fields(pr, seed=8)
+    outs = []
+    for gated in (0, 1):
+        ctx = S.Context(0)
+        ctx.tune(tile=cfg["tile"])
+        mid = ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, perm=W.kuhn_block_order(pr.grid))
+        g = pr.grid
+        ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=4, stream=0)
+        ctx.finalize()
+        ctx.tune(lds=1, gated_sweep=gated, **{k: v for k, v in cfg.items() if k != "tile"})
+        ctx.state_set(0, F.X)
+        ctx.channels_set(0, F.ryr_state, F.lcc_state, F.lccv_state, F.R)
+        ctx.globals_set(0, F.gv[0], F.gv[1])
+        seen = []
+        for n, flags in ((300, S.STEP_GRAPH), (7, 0), (17, S.STEP_GRAPH), (1, 0), (129, S.STEP_GRAPH)):
+            ctx.explicit_step(n, flags)
+            seen.append(ctx.channels_get(0)[0].copy())
+        ctx.time_sweep(5)
+        ctx.explicit_step(3)
+        outs.append((ctx.state_get(0), ctx.channels_get(0), ctx.loop_get(0), seen))
+        ctx.close()
+    (x0, c0, l0, s0), (x1, c1, l1, s1) = outs
+    assert np.array_equal(x0, x1)
+    for a, b in zip(c0, c1):
+        assert np.array_equal(a, b)
+    assert l0 == l1 and l1["step"] == 300 + 7 + 17 + 1 + 129 + 5 + 3
+    for a, b in zip(s0, s1):
+        assert np.array_equal(a, b)
+    assert any(not np.array_equal(a, F.ryr_state) for a in s1)  # the RyRs did move
+
 def test_ryr_open_probability_law():
     """At fixed Ca (no channel flux, SERCA and refill off, buffers at
     equilibrium) an uncoupled RyR's open and closed dwell times are geometric
