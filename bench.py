@@ -586,7 +586,7 @@ def main():
     if args.fidelity > 0:
         out["fidelity"] = bench_fidelity(args.fidelity)
     if trials:
-        out["autotune"] = [{**cfg, "sweep_us": round(ms * 1e3, 2)} for cfg, ms in trials]
+        out["autotune"] = [{**cfg, "step_us": round(ms * 1e3, 2)} for cfg, ms in trials]
     print(json.dumps(out))
 
 

@@ -7,7 +7,8 @@ padding (plane_pad), the compact row format, the register budget and LDS
 staging.  Every candidate computes the
 same bits; only speed differs.  Each trial runs on a scratch context built
 by the caller's factory, so the context that is finally used has taken no
-steps.  Timing is the mean sweep-kernel duration from HIP events.
+steps.  Timing is the mean duration of a step's kernels (the sweep and
+the gating) from HIP events.
 """
 from __future__ import annotations
 
@@ -24,7 +25,7 @@ KERNEL_KEYS = ("nt_geometry", "nt_state", "occupancy", "lds")
 def autotune(make: Callable[[dict], Context], tiles=(128, 256, 512), pads=(0,), compact=(1, 0), nt_geometry=(0, 1),
              nt_state=(0, 1), occupancy=(0, 6), lds=(0, 1), groups=(1,), reps: int = 20, warm: int = 3,
              log=None) -> tuple[dict, list]:
-    """Returns (best configuration, all trials as (config, sweep_ms))."""
+    """Returns (best configuration, all trials as (config, step_ms))."""
     trials = []
     for tile, pad, cmp, grp in itertools.product(tiles, pads, compact, groups):
         layout = {"tile": tile, "plane_pad": pad, "compact": cmp}
@@ -38,11 +39,14 @@ def autotune(make: Callable[[dict], Context], tiles=(128, 256, 512), pads=(0,), 
                 ctx.tune(nt_geometry=g, nt_state=s, occupancy=o, lds=ld)
                 ctx.explicit_step(warm)
                 t0 = time.perf_counter()
-                ms, _ = ctx.time_sweep(reps)
+                ms, gms = ctx.time_sweep(reps)
                 cfg = {**layout, "nt_geometry": g, "nt_state": s, "occupancy": o, "lds": ld}
-                trials.append((cfg, ms))
+                # ranked by the step: with the gated sweep the gating is part
+                # of the sweep launch, otherwise it is a launch of its own
+                trials.append((cfg, ms + gms))
                 if log:
-                    log(f"autotune {cfg}: sweep {ms * 1e3:.2f} us ({time.perf_counter() - t0:.2f} s)")
+                    log(f"autotune {cfg}: sweep {ms * 1e3:.2f} us, step {(ms + gms) * 1e3:.2f} us "
+                        f"({time.perf_counter() - t0:.2f} s)")
         finally:
             ctx.close()
     best = min(trials, key=lambda x: x[1])[0]
