@@ -10,6 +10,9 @@
 //             optionally, the RunSimulation loop body for the next time
 //             (while-condition, LCC trigger, voltage clamp, RyR hold,
 //             states-record output).
+// A single realization stepping without the loop body runs each step as one
+// launch instead, k_sweep_gated: its workgroup 0 runs k_gate's body for the
+// previous step before updating the channel elements.
 // The reference runs the same step as nine kernels with a state-sized rate
 // buffer (gpu/RunSimulation.cu:781-852); here the rates never leave registers.
 //
