@@ -7,8 +7,8 @@ padding (plane_pad), the compact row format, the register budget and LDS
 staging.  Every candidate computes the
 same bits; only speed differs.  Each trial runs on a scratch context built
 by the caller's factory, so the context that is finally used has taken no
-steps.  Timing is the mean duration of a step's kernels (the sweep and
-the gating) from HIP events.
+steps.  Timing is the step as it runs, a captured graph of steps timed by
+HIP events around the call.
 """
 from __future__ import annotations
 
@@ -16,7 +16,7 @@ import itertools
 import time
 from typing import Callable
 
-from . import Context
+from . import STEP_GRAPH, Context
 
 LAYOUT_KEYS = ("tile", "plane_pad", "compact", "group")
 KERNEL_KEYS = ("nt_geometry", "nt_state", "occupancy", "lds")
@@ -39,14 +39,19 @@ def autotune(make: Callable[[dict], Context], tiles=(128, 256, 512), pads=(0,), 
                 ctx.tune(nt_geometry=g, nt_state=s, occupancy=o, lds=ld)
                 ctx.explicit_step(warm)
                 t0 = time.perf_counter()
-                ms, gms = ctx.time_sweep(reps)
+                # ranked by the step as it runs: a captured graph of n steps
+                # (with the gated sweep the gating is part of the sweep
+                # launch, otherwise a launch of its own), timed by the
+                # events around the call
+                n = max(reps, 16)
+                ctx.prepare(n, STEP_GRAPH)
+                ctx.explicit_step(n, STEP_GRAPH)
+                total, _ = ctx.last_timing()
+                ms = total / n
                 cfg = {**layout, "nt_geometry": g, "nt_state": s, "occupancy": o, "lds": ld}
-                # ranked by the step: with the gated sweep the gating is part
-                # of the sweep launch, otherwise it is a launch of its own
-                trials.append((cfg, ms + gms))
+                trials.append((cfg, ms))
                 if log:
-                    log(f"autotune {cfg}: sweep {ms * 1e3:.2f} us, step {(ms + gms) * 1e3:.2f} us "
-                        f"({time.perf_counter() - t0:.2f} s)")
+                    log(f"autotune {cfg}: step {ms * 1e3:.2f} us ({time.perf_counter() - t0:.2f} s)")
         finally:
             ctx.close()
     best = min(trials, key=lambda x: x[1])[0]

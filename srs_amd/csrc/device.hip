@@ -1061,13 +1061,20 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
       if (c->layout_fixed) return srs::fail(SRS_E_STATE, "compact layout must be chosen before the first mesh");
       c->compact = value != 0;
       return SRS_OK;
-    case SRS_TUNE_NT_GEOMETRY: c->nt_geometry = value != 0; return SRS_OK;
+    // kernel keys: captured step graphs hold the old kernels, so they are dropped
+    case SRS_TUNE_NT_GEOMETRY:
+      c->nt_geometry = value != 0;
+      return c->finalized ? drop_graphs(c) : SRS_OK;
     case SRS_TUNE_OCCUPANCY:
       if (value != 0 && value != 6 && value != 8) return srs::fail(SRS_E_ARG, "occupancy must be 0, 6 or 8");
       c->occ = value;
-      return SRS_OK;
-    case SRS_TUNE_NT_STATE: c->nt_state = value != 0; return SRS_OK;
-    case SRS_TUNE_LDS: c->lds = value != 0; return SRS_OK;
+      return c->finalized ? drop_graphs(c) : SRS_OK;
+    case SRS_TUNE_NT_STATE:
+      c->nt_state = value != 0;
+      return c->finalized ? drop_graphs(c) : SRS_OK;
+    case SRS_TUNE_LDS:
+      c->lds = value != 0;
+      return c->finalized ? drop_graphs(c) : SRS_OK;
     case SRS_TUNE_GATED_SWEEP:
       c->gated = value != 0;
       return c->finalized ? drop_graphs(c) : SRS_OK;
@@ -1829,10 +1836,11 @@ int srs_time_sweep(srs_ctx* c, int32_t reps, double* sweep_ms, double* gating_ms
     if (gated) launch_gated(c, cur ^ (k & 1), k ? gmode : 0u);
     else launch_sweep(c);
     HIPCHK(hipEventRecord(ev[3 * k + 1], c->stream));
-    if (!gated || k == reps - 1)
+    if (!gated || k == reps - 1) {
       hipLaunchKernelGGL(k_gate, dim3((unsigned)c->reals.size()), dim3(GATE_THREADS), 0, c->stream, c->d_desc,
                          c->d_live, gmode);
-    HIPCHK(hipEventRecord(ev[3 * k + 2], c->stream));
+      HIPCHK(hipEventRecord(ev[3 * k + 2], c->stream));
+    }
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1840,7 +1848,7 @@ int srs_time_sweep(srs_ctx* c, int32_t reps, double* sweep_ms, double* gating_ms
   for (int k = 0; k < reps; ++k) {
     float a = 0, b = 0;
     HIPCHK(hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]));
-    HIPCHK(hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]));
+    if (!gated || k == reps - 1) HIPCHK(hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]));
     s += a;
     g += b;
   }
