@@ -250,7 +250,8 @@ int srs_ctx_create(int device, srs_ctx** out);
  * the tile before srs_ctx_finalize; the rest at any time.  The defaults are
  * the fastest setting measured on the 3M-element dyad mesh in either element
  * order, short of the register budget (which helps in the block order only). */
-#define SRS_TUNE_TILE 1          /* elements per sweep workgroup: 128, 256, 512 (default) */
+#define SRS_TUNE_TILE 1          /* elements per sweep workgroup: 128, 256, 512; by default srs_ctx_finalize
+                                    takes 512 when a realization's mesh has 2^20 elements or more, else 256 */
 #define SRS_TUNE_NT_GEOMETRY 2   /* 1 (default): non-temporal loads of operator rows and flags */
 #define SRS_TUNE_NT_STATE 3      /* 1: non-temporal stores of the new state */
 #define SRS_TUNE_PLANE_PAD 4     /* extra elements (multiple of 64) between SoA planes */

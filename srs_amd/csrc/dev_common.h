@@ -266,7 +266,7 @@ using namespace srs_dev;
 
 struct srs_ctx {
   int device = 0;
-  int tile = 512;        // elements per k_sweep workgroup
+  int tile = 0;          // elements per k_sweep workgroup; 0: chosen by srs_ctx_finalize
   int nt_geometry = 1;   // non-temporal loads of operator rows and flags
   int nt_state = 0;      // non-temporal stores of the new state
   int plane_pad = 0;     // extra elements per SoA plane

@@ -1240,6 +1240,13 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
   if (c->reals.empty()) return srs::fail(SRS_E_STATE, "no realizations");
   HIPCHK(hipSetDevice(c->device));
   int rc;
+  if (c->tile == 0) {
+    // 512-element tiles pay on meshes of a million elements and more (the
+    // 3M-element dyad); smaller meshes keep more workgroups in flight with 256
+    int n_max = 0;
+    for (const HostReal& hr : c->reals) n_max = std::max(n_max, c->meshes[hr.mesh]->n);
+    c->tile = n_max >= (1 << 20) ? 512 : 256;
+  }
   c->compact_all = !c->meshes.empty();
   for (auto& hmp : c->meshes) c->compact_all &= hmp->compact;
   for (auto& hmp : c->meshes) {
