@@ -391,7 +391,10 @@ __device__ void write_record(const RealDesc& d, RealLive& l, const double* Ca, d
 }
 
 // The gating and loop body of realization r by one workgroup of NT threads.
-template <int NT>
+// LEAN loads each LCC's inputs when it is updated rather than with the RyRs'
+// before the first barrier: one more memory round trip, fewer live registers.
+// PROTO = false leaves out the loop body (callers that never pass G_PROTOCOL).
+template <int NT, bool LEAN = false, bool PROTO = true>
 __device__ __forceinline__ void gate_workgroup(const RealDesc* __restrict__ D, RealLive* __restrict__ L, const int r,
                                                const uint32_t mode) {
   static_assert(MAX_RYR % NT == 0 && MAX_LCC % NT == 0, "channel slots per thread");
@@ -442,8 +445,7 @@ __device__ __forceinline__ void gate_workgroup(const RealDesc* __restrict__ D, R
     }
     int ls[LPT], lvs[LPT];
     double ca_l[LPT], Rl[LPT];
-#pragma unroll
-    for (int k = 0; k < LPT; ++k) {
+    auto load_lcc = [&](const int k) {
       const int i = (NT - 1 - tid) + k * NT;
       if (i < d.n_lcc) {
         ls[k] = d.lcc_state[i];
@@ -451,6 +453,10 @@ __device__ __forceinline__ void gate_workgroup(const RealDesc* __restrict__ D, R
         ca_l[k] = Ca[cq(d, d.lcc_ele[i])];
         Rl[k] = d.R[nr + i];
       }
+    };
+    if constexpr (!LEAN) {
+#pragma unroll
+      for (int k = 0; k < LPT; ++k) load_lcc(k);
     }
     __syncthreads();
     // RyRs: the reference updates them one after another (later RyRs see
@@ -520,6 +526,7 @@ __device__ __forceinline__ void gate_workgroup(const RealDesc* __restrict__ D, R
     for (int k = 0; k < LPT; ++k) {
       const int i = (NT - 1 - tid) + k * NT;
       if (i >= d.n_lcc) continue;
+      if constexpr (LEAN) load_lcc(k);
       const int s = ls[k], vs = lvs[k];
       double tr[3];
       int32_t tg[3];
@@ -556,7 +563,7 @@ __device__ __forceinline__ void gate_workgroup(const RealDesc* __restrict__ D, R
   // result).  Each thread reads the channel entries it wrote itself above.
   int n_open_all = 0, n_lcc_all = 0;
   double cmax_all = 0.0;
-  if ((mode & G_PROTOCOL) && (ff || out_due)) {
+  if (PROTO && (mode & G_PROTOCOL) && (ff || out_due)) {
     __shared__ int s_cnt[2][NT / 64];
     __shared__ double s_max[NT / 64];
     int cr = 0, cl = 0;
@@ -599,7 +606,7 @@ __device__ __forceinline__ void gate_workgroup(const RealDesc* __restrict__ D, R
   l.step = step + 1;
   l.t = t;
   l.dt_last = dt;
-  if (!(mode & G_PROTOCOL)) return;
+  if (!PROTO || !(mode & G_PROTOCOL)) return;
 
   if (ff) {  // the rest of the ffwd loop body, cpu/RunSimulation.cpp:480-518
     const int n_open = n_open_all, n_open_lcc = n_lcc_all;
@@ -654,7 +661,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
 // this step reads, is an argument because workgroup 0 flips the live index
 // while the others run; nothing else they read changes without G_PROTOCOL.
 // gmode 0 skips the gating (the first step of a block).
-template <int BS, bool NTG, bool NTS, bool COMPACT>
+template <int BS, bool NTG, bool NTS, bool COMPACT, bool LEAN>
 __device__ __forceinline__ void sweep_gated(const RealDesc* __restrict__ D, RealLive* __restrict__ L, int n_tiles,
                                             int cur, uint32_t gmode) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -665,7 +672,7 @@ __device__ __forceinline__ void sweep_gated(const RealDesc* __restrict__ D, Real
   const Bufs bf = bufs(d.X[0], d.X[1], cur);
   scalars_here(fold(skip, bf.xo, bf.xn) ^ geo_fold<COMPACT>(d));
   if (b == 0) {
-    if (gmode) gate_workgroup<BS>(D, L, 0, gmode);
+    if (gmode) gate_workgroup<BS, LEAN, false>(D, L, 0, gmode);
     __syncthreads();  // the new channel states, written by this workgroup, before the flux reads them
     if (skip) return;
     for (int k = threadIdx.x; k < d.n_celem; k += BS) {
@@ -679,15 +686,19 @@ __device__ __forceinline__ void sweep_gated(const RealDesc* __restrict__ D, Real
   if (skip) return;
   sweep_tile_lds<BS, NTG, NTS, COMPACT, true>(d, l, bf, b - 1);
 }
+// Register budgets as for the sweep.  The gating workgroup's latency is on
+// the critical path of a small mesh's step, so the compiler's budget keeps
+// all of its loads in one round trip; with a budget the LCC loads wait for
+// the RyR update (LEAN), which is what lets the kernel fit 80 VGPRs.
 template <int BS, bool NTG, bool NTS, bool COMPACT>
 __global__ __launch_bounds__(BS) void k_sweep_gated(const RealDesc* __restrict__ D, RealLive* __restrict__ L,
                                                     int n_tiles, int cur, uint32_t gmode) {
-  sweep_gated<BS, NTG, NTS, COMPACT>(D, L, n_tiles, cur, gmode);
+  sweep_gated<BS, NTG, NTS, COMPACT, false>(D, L, n_tiles, cur, gmode);
 }
 template <int BS, bool NTG, bool NTS, bool COMPACT, int W>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_sweep_gated_w(
     const RealDesc* __restrict__ D, RealLive* __restrict__ L, int n_tiles, int cur, uint32_t gmode) {
-  sweep_gated<BS, NTG, NTS, COMPACT>(D, L, n_tiles, cur, gmode);
+  sweep_gated<BS, NTG, NTS, COMPACT, true>(D, L, n_tiles, cur, gmode);
 }
 
 // Gathers Ca at the channel elements of one realization.
