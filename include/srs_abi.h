@@ -266,10 +266,10 @@ int srs_ctx_create(int device, srs_ctx** out);
                                     neighbours inside the tile from there (groups of 1) */
 #define SRS_TUNE_DROP_FINISHED 9 /* 1 (default): srs_explicit_step leaves the tiles of groups whose
                                     realizations have all finished out of the sweep; 0: sweep them all */
-#define SRS_TUNE_GATED_SWEEP 10  /* 1 (default): a single-realization context stepping without
-                                    SRS_STEP_PROTOCOL, with the LDS sweep (register budget 0 or 6), runs
-                                    each step's gating inside the next step's sweep launch; 0: a gating
-                                    launch after every sweep */
+#define SRS_TUNE_GATED_SWEEP 10  /* 1 (default): a single-realization context with the LDS sweep runs each
+                                    step's gating (and, with SRS_STEP_PROTOCOL, the loop body) inside the
+                                    next step's sweep launch (register budget 0 or 6; 0 with the loop
+                                    body); 0: a gating launch after every sweep */
 int srs_ctx_tune(srs_ctx* ctx, int32_t key, int32_t value);
 
 /* Replaces FreeDomainStruct / FreeGlobalStruct (gpu/RunSimulation.cu:874-901). */

@@ -659,9 +659,14 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
 // channel flux: only their rates read the channel states.  The other
 // workgroups sweep the tiles without those elements.  cur, the state buffer
 // this step reads, is an argument because workgroup 0 flips the live index
-// while the others run; nothing else they read changes without G_PROTOCOL.
+// while the others run.  With the loop body (PROTO, G_PROTOCOL) workgroup 0
+// also changes Vm and channel states, which only the channel elements read,
+// and may finish the realization: workgroups that see `done` skip their
+// tile, the others write the buffer the finished realization never reads
+// again.  A block starts only outside the ffwd phase, which the loop body
+// can leave but not enter, so every step of it flips the live index.
 // gmode 0 skips the gating (the first step of a block).
-template <int BS, bool NTG, bool NTS, bool COMPACT, bool LEAN>
+template <int BS, bool NTG, bool NTS, bool COMPACT, bool LEAN, bool PROTO>
 __device__ __forceinline__ void sweep_gated(const RealDesc* __restrict__ D, RealLive* __restrict__ L, int n_tiles,
                                             int cur, uint32_t gmode) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -672,7 +677,7 @@ __device__ __forceinline__ void sweep_gated(const RealDesc* __restrict__ D, Real
   const Bufs bf = bufs(d.X[0], d.X[1], cur);
   scalars_here(fold(skip, bf.xo, bf.xn) ^ geo_fold<COMPACT>(d));
   if (b == 0) {
-    if (gmode) gate_workgroup<BS, LEAN, false>(D, L, 0, gmode);
+    if (gmode) gate_workgroup<BS, LEAN, PROTO>(D, L, 0, gmode);
     __syncthreads();  // the new channel states, written by this workgroup, before the flux reads them
     if (skip) return;
     for (int k = threadIdx.x; k < d.n_celem; k += BS) {
@@ -693,12 +698,18 @@ __device__ __forceinline__ void sweep_gated(const RealDesc* __restrict__ D, Real
 template <int BS, bool NTG, bool NTS, bool COMPACT>
 __global__ __launch_bounds__(BS) void k_sweep_gated(const RealDesc* __restrict__ D, RealLive* __restrict__ L,
                                                     int n_tiles, int cur, uint32_t gmode) {
-  sweep_gated<BS, NTG, NTS, COMPACT, false>(D, L, n_tiles, cur, gmode);
+  sweep_gated<BS, NTG, NTS, COMPACT, false, false>(D, L, n_tiles, cur, gmode);
 }
 template <int BS, bool NTG, bool NTS, bool COMPACT, int W>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_sweep_gated_w(
     const RealDesc* __restrict__ D, RealLive* __restrict__ L, int n_tiles, int cur, uint32_t gmode) {
-  sweep_gated<BS, NTG, NTS, COMPACT, true>(D, L, n_tiles, cur, gmode);
+  sweep_gated<BS, NTG, NTS, COMPACT, true, false>(D, L, n_tiles, cur, gmode);
+}
+// with the loop body (SRS_STEP_PROTOCOL), compiler's register budget only
+template <int BS, bool NTG, bool NTS, bool COMPACT>
+__global__ __launch_bounds__(BS) void k_sweep_gated_p(const RealDesc* __restrict__ D, RealLive* __restrict__ L,
+                                                      int n_tiles, int cur, uint32_t gmode) {
+  sweep_gated<BS, NTG, NTS, COMPACT, false, true>(D, L, n_tiles, cur, gmode);
 }
 
 // Gathers Ca at the channel elements of one realization.
@@ -893,8 +904,21 @@ GatedFn pick_gated_w(int ntg, int nts) {
   }
 }
 
+template <int BS, bool CMP>
+GatedFn pick_gated_p(int ntg, int nts) {
+  if (ntg) return nts ? k_sweep_gated_p<BS, true, true, CMP> : k_sweep_gated_p<BS, true, false, CMP>;
+  return nts ? k_sweep_gated_p<BS, false, true, CMP> : k_sweep_gated_p<BS, false, false, CMP>;
+}
+
 template <bool CMP>
-GatedFn pick_gated_tile(const srs_ctx* c) {
+GatedFn pick_gated_tile(const srs_ctx* c, bool proto) {
+  if (proto) {
+    switch (c->tile) {
+      case 128: return pick_gated_p<128, CMP>(c->nt_geometry, c->nt_state);
+      case 512: return pick_gated_p<512, CMP>(c->nt_geometry, c->nt_state);
+      default: return pick_gated_p<256, CMP>(c->nt_geometry, c->nt_state);
+    }
+  }
   const bool w6 = c->occ == 6;
   switch (c->tile) {
     case 128: return w6 ? pick_gated_w<128, CMP, 6>(c->nt_geometry, c->nt_state) : pick_gated_w<128, CMP, 0>(c->nt_geometry, c->nt_state);
@@ -904,9 +928,10 @@ GatedFn pick_gated_tile(const srs_ctx* c) {
 }
 
 // One step through k_sweep_gated: tile b - 1 in workgroup b, workgroup 0
-// gates (gmode) and updates the channel elements.
-void launch_gated(srs_ctx* c, int cur, uint32_t gmode) {
-  const GatedFn f = c->compact_all ? pick_gated_tile<true>(c) : pick_gated_tile<false>(c);
+// gates (gmode) and updates the channel elements.  proto picks the kernel
+// with the loop body.
+void launch_gated(srs_ctx* c, int cur, uint32_t gmode, bool proto) {
+  const GatedFn f = c->compact_all ? pick_gated_tile<true>(c, proto) : pick_gated_tile<false>(c, proto);
   const int grid = (c->n_tiles + 1 + NXCD - 1) / NXCD * NXCD;
   hipLaunchKernelGGL(f, dim3(grid), dim3(c->tile), 0, c->stream, c->d_desc, c->d_live, c->n_tiles, cur, gmode);
 }
@@ -914,10 +939,12 @@ void launch_gated(srs_ctx* c, int cur, uint32_t gmode) {
 // m steps through k_sweep_gated: the first without gating (nothing is
 // pending when a block starts), each later one gating the step before it,
 // then k_gate for the last step, so that a block leaves no gating pending.
-// cur is the live state index when the block starts.
-void launch_gated_block(srs_ctx* c, int64_t m, int cur) {
-  for (int64_t k = 0; k < m; ++k) launch_gated(c, cur ^ (int)(k & 1), k ? G_AFTER_SWEEP : 0u);
-  hipLaunchKernelGGL(k_gate, dim3(1), dim3(GATE_THREADS), 0, c->stream, c->d_desc, c->d_live, (uint32_t)G_AFTER_SWEEP);
+// cur is the live state index when the block starts; gmode is the step's
+// gating mode (G_AFTER_SWEEP, with G_PROTOCOL for the loop body).
+void launch_gated_block(srs_ctx* c, int64_t m, int cur, uint32_t gmode) {
+  const bool proto = (gmode & G_PROTOCOL) != 0;
+  for (int64_t k = 0; k < m; ++k) launch_gated(c, cur ^ (int)(k & 1), k ? gmode : 0u, proto);
+  hipLaunchKernelGGL(k_gate, dim3(1), dim3(GATE_THREADS), 0, c->stream, c->d_desc, c->d_live, gmode);
 }
 
 // Captured step blocks hold the sweep's tile list and grid; they are dropped
@@ -1624,20 +1651,20 @@ static int graph_for(srs_ctx* c, int64_t block, uint32_t gmode, hipGraphExec_t* 
 }
 
 // A block of steps through k_sweep_gated starting from live state index cur.
-static int graph_gated(srs_ctx* c, int64_t block, int cur, hipGraphExec_t* out) {
-  return graph_cached(c, block, G_AFTER_SWEEP | G_KEY_GATED | (cur ? G_KEY_CUR1 : 0u),
-                      [&] { launch_gated_block(c, block, cur); }, out);
+static int graph_gated(srs_ctx* c, int64_t block, int cur, uint32_t gmode, hipGraphExec_t* out) {
+  return graph_cached(c, block, gmode | G_KEY_GATED | (cur ? G_KEY_CUR1 : 0u),
+                      [&] { launch_gated_block(c, block, cur, gmode); }, out);
 }
 
-// Whether steps without the loop body run through k_sweep_gated, and the live
-// state index they start from: single-realization contexts with the LDS
-// sweep of groups of one and a register budget of 0 or 6, while the
-// realization is neither finished nor in the ffwd phase (neither changes
-// without the loop body).  Reading the live state waits for the stream.
+// Whether steps run through k_sweep_gated, and the live state index they
+// start from: single-realization contexts with the LDS sweep of groups of
+// one and a register budget of 0 or 6 (0 with the loop body), while the
+// realization is neither finished nor in the ffwd phase.  Reading the live
+// state waits for the stream.
 static int gated_start(srs_ctx* c, uint32_t flags, bool* ok, int* cur) {
   *ok = false;
-  if (!c->gated || c->reals.size() != 1 || c->group != 1 || !c->lds || c->occ == 8 || (flags & SRS_STEP_PROTOCOL))
-    return SRS_OK;
+  const int occ_max = (flags & SRS_STEP_PROTOCOL) ? 0 : 6;
+  if (!c->gated || c->reals.size() != 1 || c->group != 1 || !c->lds || c->occ > occ_max) return SRS_OK;
   RealLive lv;
   int rc = live_rw(c, 0, &lv, false);
   if (rc) return rc;
@@ -1666,8 +1693,9 @@ int srs_explicit_step_prepare(srs_ctx* c, int64_t n_steps, uint32_t flags) {
   if (gated) {
     // both starting live indices: the one a later call starts from is not known here
     for (int p = 0; p < 2; ++p) {
-      if (n_steps >= kGraphBlock && (rc = graph_gated(c, kGraphBlock, p, &ge))) return rc;
-      if (n_steps % kGraphBlock >= kGraphMin && (rc = graph_gated(c, n_steps % kGraphBlock, p, &ge))) return rc;
+      if (n_steps >= kGraphBlock && (rc = graph_gated(c, kGraphBlock, p, gmode, &ge))) return rc;
+      if (n_steps % kGraphBlock >= kGraphMin && (rc = graph_gated(c, n_steps % kGraphBlock, p, gmode, &ge)))
+        return rc;
     }
     return SRS_OK;
   }
@@ -1694,19 +1722,20 @@ int srs_explicit_step(srs_ctx* c, int64_t n_steps, uint32_t flags) {
     // 128-step blocks, then the rest as one block of its own when it is long enough
     hipGraphExec_t ge;
     if (n_steps >= kGraphBlock) {
-      if ((rc = gated ? graph_gated(c, kGraphBlock, cur, &ge) : graph_for(c, kGraphBlock, gmode, &ge))) return rc;
+      if ((rc = gated ? graph_gated(c, kGraphBlock, cur, gmode, &ge) : graph_for(c, kGraphBlock, gmode, &ge)))
+        return rc;
       for (; n_steps - done >= kGraphBlock; done += kGraphBlock) HIPCHK(hipGraphLaunch(ge, c->stream));
     }
     if (n_steps - done >= kGraphMin) {
       const int64_t m = n_steps - done;
-      if ((rc = gated ? graph_gated(c, m, cur, &ge) : graph_for(c, m, gmode, &ge))) return rc;
+      if ((rc = gated ? graph_gated(c, m, cur, gmode, &ge) : graph_for(c, m, gmode, &ge))) return rc;
       HIPCHK(hipGraphLaunch(ge, c->stream));
       cur ^= (int)(m & 1);
       done = n_steps;
     }
   }
   if (gated) {
-    if (done < n_steps) launch_gated_block(c, n_steps - done, cur);
+    if (done < n_steps) launch_gated_block(c, n_steps - done, cur, gmode);
   } else {
     for (; done < n_steps; ++done) launch_step(c, gmode);
   }
@@ -1851,7 +1880,7 @@ int srs_time_sweep(srs_ctx* c, int32_t reps, double* sweep_ms, double* gating_ms
     // the kernels srs_explicit_step runs: with the gated sweep, each launch
     // also gates the step before it, and the gating after the last one is
     // the whole gating time of the reps
-    if (gated) launch_gated(c, cur ^ (k & 1), k ? gmode : 0u);
+    if (gated) launch_gated(c, cur ^ (k & 1), k ? gmode : 0u, false);
     else launch_sweep(c);
     HIPCHK(hipEventRecord(ev[3 * k + 1], c->stream));
     if (!gated || k == reps - 1) {

@@ -144,9 +144,12 @@ def test_realization_groups_equal_single_runs(group):
         assert rel_err(got[k][0], Fo.X) <= TOL, k
 
 
-def test_protocol_loop_records_match_oracle_loop():
+@pytest.mark.parametrize("gated", [0, 1])
+def test_protocol_loop_records_match_oracle_loop(gated):
     """The device-side RunSimulation loop body (clamp, RyR hold, states output,
-    while-condition, final record) against an oracle-driven loop."""
+    while-condition, final record) against an oracle-driven loop, with the
+    loop body in the gating launch after each sweep or in workgroup 0 of the
+    next sweep launch (the realization finishes inside a block)."""
     pr = _tiny(tt=True)
     p = fast_gating(pr.params)
     dt = p.reals[1]
@@ -161,6 +164,7 @@ def test_protocol_loop_records_match_oracle_loop():
     Fd.gv[0] = p.reals[S.R["V_CLAMP"]]
     Fd.ryr_state[1] = 1
     with gpu_context([pr], [Fd], params=[p], seeds=[21], record_capacity=256) as ctx:
+        ctx.tune(gated_sweep=gated)
         ctx.loop_set(0, n_open=int(F.ryr_state.sum()))
         ctx.explicit_step(nsteps + 5, S.STEP_PROTOCOL | S.STEP_GRAPH)
         recs_d = ctx.records(0)
