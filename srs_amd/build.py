@@ -10,6 +10,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -22,6 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HOST_SRCS = ["host_common.cpp", "host_params.cpp", "host_mesh.cpp", "host_run.cpp"]
 HIP_SRCS = ["device.hip", "linescan.hip", "adjoint.hip"]
+DEVICE_UNITS = (0, 1, 2, 3, 4)
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-I", INCLUDE, "-I", CSRC]
 HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -35,15 +37,20 @@ def _run(cmd):
 
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    jobs = []
     for src in HOST_SRCS:
         obj = os.path.join(BUILD, src + ".o")
-        _run(["g++", *COMMON, "-Wall", "-I", "/opt/rocm/include", "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append((obj, ["g++", *COMMON, "-Wall", "-I", "/opt/rocm/include", "-c", os.path.join(CSRC, src), "-o", obj]))
     for src in HIP_SRCS:
-        obj = os.path.join(BUILD, src + ".o")
-        _run([HIPCC, *COMMON, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        # device.hip holds the sweep kernels' many instantiations: it is
+        # compiled as five units (SRS_UNIT) that build side by side
+        for unit in (DEVICE_UNITS if src == "device.hip" else (None,)):
+            obj = os.path.join(BUILD, src + ("" if unit is None else f".{unit}") + ".o")
+            defs = [] if unit is None else [f"-DSRS_UNIT={unit}"]
+            jobs.append((obj, [HIPCC, *COMMON, *HIP_FLAGS, *defs, "-c", os.path.join(CSRC, src), "-o", obj]))
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1, 8)) as ex:
+        list(ex.map(lambda j: _run(j[1]), jobs))
+    objs = [obj for obj, _ in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs])
     os.replace(tmp, LIB)

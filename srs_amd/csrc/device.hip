@@ -23,6 +23,10 @@
 // adjoint.hip; the shared types are in dev_common.h.
 #include "dev_common.h"
 
+#ifndef SRS_UNIT
+#define SRS_UNIT 0
+#endif
+
 #include <functional>
 
 namespace srs_dev {
@@ -647,10 +651,12 @@ __device__ __forceinline__ void gate_workgroup(const RealDesc* __restrict__ D, R
   }
 }
 
+#if SRS_UNIT == 0
 __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restrict__ D, RealLive* __restrict__ L,
                                                        uint32_t mode) {
   gate_workgroup<GATE_THREADS>(D, L, blockIdx.x, mode);
 }
+#endif
 
 // One step of a single-realization context with the previous step's gating
 // folded in (cpu/RunSimulation.cpp:650-652 then :591-648).  Workgroup 0 gates
@@ -712,6 +718,101 @@ __global__ __launch_bounds__(BS) void k_sweep_gated_p(const RealDesc* __restrict
   sweep_gated<BS, NTG, NTS, COMPACT, false, true>(D, L, n_tiles, cur, gmode);
 }
 
+// ------------------------------------------------------------ kernel choice
+// The sweep kernels are instantiated in five compilation units of this file
+// (SRS_UNIT 0: host code and the gating kernel; 1: the sweeps; 2, 3, 4: the
+// gated sweeps of 128-, 256- and 512-element tiles), which the build
+// compiles in parallel.
+
+using SweepFn = void (*)(const RealDesc*, const RealLive*, const int2*, int, const int*, int);
+using GatedFn = void (*)(const RealDesc*, RealLive*, int, int, uint32_t);
+SweepFn pick_sweep(const srs_ctx* c);
+template <int BS>
+GatedFn pick_gated_bs(const srs_ctx* c, bool proto);
+
+#if SRS_UNIT == 1
+// The sweep kernel for a tile size, cache hints, register budget (0, 6 or 8
+// wavefronts per SIMD) and LDS staging (single-realization groups only).
+template <int BS, bool CMP, int KG, int W>
+SweepFn pick_w(int ntg, int nts, int lds) {
+  if constexpr (KG == 1) {
+    if (lds) {
+      if constexpr (W == 0) {
+        if (ntg) return nts ? k_sweep_lds<BS, true, true, CMP> : k_sweep_lds<BS, true, false, CMP>;
+        return nts ? k_sweep_lds<BS, false, true, CMP> : k_sweep_lds<BS, false, false, CMP>;
+      } else {
+        if (ntg) return nts ? k_sweep_lds_w<BS, true, true, CMP, W> : k_sweep_lds_w<BS, true, false, CMP, W>;
+        return nts ? k_sweep_lds_w<BS, false, true, CMP, W> : k_sweep_lds_w<BS, false, false, CMP, W>;
+      }
+    }
+  }
+  if constexpr (W == 0) {
+    if (ntg) return nts ? k_sweep<BS, true, true, CMP, KG> : k_sweep<BS, true, false, CMP, KG>;
+    return nts ? k_sweep<BS, false, true, CMP, KG> : k_sweep<BS, false, false, CMP, KG>;
+  } else {
+    if (ntg) return nts ? k_sweep_w<BS, true, true, CMP, KG, W> : k_sweep_w<BS, true, false, CMP, KG, W>;
+    return nts ? k_sweep_w<BS, false, true, CMP, KG, W> : k_sweep_w<BS, false, false, CMP, KG, W>;
+  }
+}
+
+template <int BS, bool CMP, int KG>
+SweepFn pick3(int ntg, int nts, int occ, int lds) {
+  if (occ == 8) return pick_w<BS, CMP, KG, 8>(ntg, nts, lds);
+  if (occ == 6) return pick_w<BS, CMP, KG, 6>(ntg, nts, lds);
+  return pick_w<BS, CMP, KG, 0>(ntg, nts, lds);
+}
+
+template <bool CMP, int KG>
+SweepFn pick_tile(const srs_ctx* c) {
+  switch (c->tile) {
+    case 128: return pick3<128, CMP, KG>(c->nt_geometry, c->nt_state, c->occ, c->lds);
+    case 512: return pick3<512, CMP, KG>(c->nt_geometry, c->nt_state, c->occ, c->lds);
+    default: return pick3<256, CMP, KG>(c->nt_geometry, c->nt_state, c->occ, c->lds);
+  }
+}
+
+template <int KG>
+SweepFn pick_kg(const srs_ctx* c) { return c->compact_all ? pick_tile<true, KG>(c) : pick_tile<false, KG>(c); }
+
+SweepFn pick_sweep(const srs_ctx* c) {
+  return c->group == 4 ? pick_kg<4>(c) : c->group == 2 ? pick_kg<2>(c) : pick_kg<1>(c);
+}
+#endif  // SRS_UNIT == 1
+
+#if SRS_UNIT >= 2
+template <int BS, bool CMP, int W>
+GatedFn pick_gated_w(int ntg, int nts) {
+  if constexpr (W == 0) {
+    if (ntg) return nts ? k_sweep_gated<BS, true, true, CMP> : k_sweep_gated<BS, true, false, CMP>;
+    return nts ? k_sweep_gated<BS, false, true, CMP> : k_sweep_gated<BS, false, false, CMP>;
+  } else {
+    if (ntg) return nts ? k_sweep_gated_w<BS, true, true, CMP, W> : k_sweep_gated_w<BS, true, false, CMP, W>;
+    return nts ? k_sweep_gated_w<BS, false, true, CMP, W> : k_sweep_gated_w<BS, false, false, CMP, W>;
+  }
+}
+
+template <int BS, bool CMP>
+GatedFn pick_gated_p(int ntg, int nts) {
+  if (ntg) return nts ? k_sweep_gated_p<BS, true, true, CMP> : k_sweep_gated_p<BS, true, false, CMP>;
+  return nts ? k_sweep_gated_p<BS, false, true, CMP> : k_sweep_gated_p<BS, false, false, CMP>;
+}
+
+template <int BS, bool CMP>
+GatedFn pick_gated_cmp(const srs_ctx* c, bool proto) {
+  if (proto) return pick_gated_p<BS, CMP>(c->nt_geometry, c->nt_state);
+  return c->occ == 6 ? pick_gated_w<BS, CMP, 6>(c->nt_geometry, c->nt_state)
+                     : pick_gated_w<BS, CMP, 0>(c->nt_geometry, c->nt_state);
+}
+
+template <int BS>
+GatedFn pick_gated_bs(const srs_ctx* c, bool proto) {
+  return c->compact_all ? pick_gated_cmp<BS, true>(c, proto) : pick_gated_cmp<BS, false>(c, proto);
+}
+// one tile size per unit: 128 in unit 2, 256 in unit 3, 512 in unit 4
+template GatedFn pick_gated_bs<SRS_UNIT == 2 ? 128 : SRS_UNIT == 3 ? 256 : 512>(const srs_ctx*, bool);
+#endif  // SRS_UNIT >= 2
+
+#if SRS_UNIT == 0
 // Gathers Ca at the channel elements of one realization.
 __global__ void k_channel_ca(const RealDesc* __restrict__ D, const RealLive* __restrict__ L, int r, double* out) {
   const RealDesc& d = D[r];
@@ -827,55 +928,6 @@ int check_real(srs_ctx* c, int32_t r) {
   return SRS_OK;
 }
 
-using SweepFn = void (*)(const RealDesc*, const RealLive*, const int2*, int, const int*, int);
-
-// The sweep kernel for a tile size, cache hints, register budget (0, 6 or 8
-// wavefronts per SIMD) and LDS staging (single-realization groups only).
-template <int BS, bool CMP, int KG, int W>
-SweepFn pick_w(int ntg, int nts, int lds) {
-  if constexpr (KG == 1) {
-    if (lds) {
-      if constexpr (W == 0) {
-        if (ntg) return nts ? k_sweep_lds<BS, true, true, CMP> : k_sweep_lds<BS, true, false, CMP>;
-        return nts ? k_sweep_lds<BS, false, true, CMP> : k_sweep_lds<BS, false, false, CMP>;
-      } else {
-        if (ntg) return nts ? k_sweep_lds_w<BS, true, true, CMP, W> : k_sweep_lds_w<BS, true, false, CMP, W>;
-        return nts ? k_sweep_lds_w<BS, false, true, CMP, W> : k_sweep_lds_w<BS, false, false, CMP, W>;
-      }
-    }
-  }
-  if constexpr (W == 0) {
-    if (ntg) return nts ? k_sweep<BS, true, true, CMP, KG> : k_sweep<BS, true, false, CMP, KG>;
-    return nts ? k_sweep<BS, false, true, CMP, KG> : k_sweep<BS, false, false, CMP, KG>;
-  } else {
-    if (ntg) return nts ? k_sweep_w<BS, true, true, CMP, KG, W> : k_sweep_w<BS, true, false, CMP, KG, W>;
-    return nts ? k_sweep_w<BS, false, true, CMP, KG, W> : k_sweep_w<BS, false, false, CMP, KG, W>;
-  }
-}
-
-template <int BS, bool CMP, int KG>
-SweepFn pick3(int ntg, int nts, int occ, int lds) {
-  if (occ == 8) return pick_w<BS, CMP, KG, 8>(ntg, nts, lds);
-  if (occ == 6) return pick_w<BS, CMP, KG, 6>(ntg, nts, lds);
-  return pick_w<BS, CMP, KG, 0>(ntg, nts, lds);
-}
-
-template <bool CMP, int KG>
-SweepFn pick_tile(const srs_ctx* c) {
-  switch (c->tile) {
-    case 128: return pick3<128, CMP, KG>(c->nt_geometry, c->nt_state, c->occ, c->lds);
-    case 512: return pick3<512, CMP, KG>(c->nt_geometry, c->nt_state, c->occ, c->lds);
-    default: return pick3<256, CMP, KG>(c->nt_geometry, c->nt_state, c->occ, c->lds);
-  }
-}
-
-template <int KG>
-SweepFn pick_kg(const srs_ctx* c) { return c->compact_all ? pick_tile<true, KG>(c) : pick_tile<false, KG>(c); }
-
-SweepFn pick_sweep(const srs_ctx* c) {
-  return c->group == 4 ? pick_kg<4>(c) : c->group == 2 ? pick_kg<2>(c) : pick_kg<1>(c);
-}
-
 void launch_sweep(srs_ctx* c) {
   if (c->grid_sweep == 0) return;  // every realization has finished
   // one realization: tile b is tile b of realization 0, no table lookups
@@ -891,47 +943,12 @@ int launch_step(srs_ctx* c, uint32_t gmode) {
   return SRS_OK;
 }
 
-using GatedFn = void (*)(const RealDesc*, RealLive*, int, int, uint32_t);
-
-template <int BS, bool CMP, int W>
-GatedFn pick_gated_w(int ntg, int nts) {
-  if constexpr (W == 0) {
-    if (ntg) return nts ? k_sweep_gated<BS, true, true, CMP> : k_sweep_gated<BS, true, false, CMP>;
-    return nts ? k_sweep_gated<BS, false, true, CMP> : k_sweep_gated<BS, false, false, CMP>;
-  } else {
-    if (ntg) return nts ? k_sweep_gated_w<BS, true, true, CMP, W> : k_sweep_gated_w<BS, true, false, CMP, W>;
-    return nts ? k_sweep_gated_w<BS, false, true, CMP, W> : k_sweep_gated_w<BS, false, false, CMP, W>;
-  }
-}
-
-template <int BS, bool CMP>
-GatedFn pick_gated_p(int ntg, int nts) {
-  if (ntg) return nts ? k_sweep_gated_p<BS, true, true, CMP> : k_sweep_gated_p<BS, true, false, CMP>;
-  return nts ? k_sweep_gated_p<BS, false, true, CMP> : k_sweep_gated_p<BS, false, false, CMP>;
-}
-
-template <bool CMP>
-GatedFn pick_gated_tile(const srs_ctx* c, bool proto) {
-  if (proto) {
-    switch (c->tile) {
-      case 128: return pick_gated_p<128, CMP>(c->nt_geometry, c->nt_state);
-      case 512: return pick_gated_p<512, CMP>(c->nt_geometry, c->nt_state);
-      default: return pick_gated_p<256, CMP>(c->nt_geometry, c->nt_state);
-    }
-  }
-  const bool w6 = c->occ == 6;
-  switch (c->tile) {
-    case 128: return w6 ? pick_gated_w<128, CMP, 6>(c->nt_geometry, c->nt_state) : pick_gated_w<128, CMP, 0>(c->nt_geometry, c->nt_state);
-    case 512: return w6 ? pick_gated_w<512, CMP, 6>(c->nt_geometry, c->nt_state) : pick_gated_w<512, CMP, 0>(c->nt_geometry, c->nt_state);
-    default: return w6 ? pick_gated_w<256, CMP, 6>(c->nt_geometry, c->nt_state) : pick_gated_w<256, CMP, 0>(c->nt_geometry, c->nt_state);
-  }
-}
-
 // One step through k_sweep_gated: tile b - 1 in workgroup b, workgroup 0
 // gates (gmode) and updates the channel elements.  proto picks the kernel
 // with the loop body.
 void launch_gated(srs_ctx* c, int cur, uint32_t gmode, bool proto) {
-  const GatedFn f = c->compact_all ? pick_gated_tile<true>(c, proto) : pick_gated_tile<false>(c, proto);
+  const GatedFn f = c->tile == 128 ? pick_gated_bs<128>(c, proto)
+                    : c->tile == 512 ? pick_gated_bs<512>(c, proto) : pick_gated_bs<256>(c, proto);
   const int grid = (c->n_tiles + 1 + NXCD - 1) / NXCD * NXCD;
   hipLaunchKernelGGL(f, dim3(grid), dim3(c->tile), 0, c->stream, c->d_desc, c->d_live, c->n_tiles, cur, gmode);
 }
@@ -1054,8 +1071,11 @@ int live_rw(srs_ctx* c, int32_t r, RealLive* lv, bool write) {
   return SRS_OK;
 }
 
+#endif  // SRS_UNIT == 0
+
 }  // namespace srs_dev
 
+#if SRS_UNIT == 0
 extern "C" {
 
 int srs_ctx_create(int device, srs_ctx** out) {
@@ -1907,3 +1927,4 @@ int srs_time_sweep(srs_ctx* c, int32_t reps, double* sweep_ms, double* gating_ms
 }
 
 }  // extern "C"
+#endif  // SRS_UNIT == 0
