@@ -570,17 +570,31 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
     };
     auto ck_at = [&](int64_t k) { return ck + (size_t)k * nb * n6; };
 
-    // forward, with checkpoints and scan samples
+    // forward, with checkpoints and scan samples; a single realization runs
+    // the steps between two of those through the gated sweep (one launch per
+    // step, the replay of each step in the next one's workgroup 0)
     hipLaunchKernelGGL(k_seg_start, dim3(nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, (int64_t)0);
+    const bool gated = c->gated && n_real == 1 && c->group == 1 && c->lds && c->occ <= 6;
+    int cur = lv[ids[m0]].cur;
     int ks = 0;
-    for (int64_t n = 0; n < n_steps && e == hipSuccess; ++n) {
+    for (int64_t n = 0; n < n_steps && e == hipSuccess;) {
       if (n % ckpt == 0 && n / ckpt < n_ck_kept)
         hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid,
                            ck_at(n / ckpt), 1);
-      launch_step(c, gmode);
-      if (ks < n_samples && sample_steps[ks] == n + 1) scan_into(samp + (size_t)(ks++) * nb * nout);
+      int64_t m = 1;
+      if (gated) {
+        int64_t next = std::min<int64_t>(n_steps, (n / ckpt + 1) * ckpt);
+        if (ks < n_samples && sample_steps[ks] > n) next = std::min<int64_t>(next, sample_steps[ks]);
+        m = std::max<int64_t>(1, next - n);
+        launch_gated_block(c, m, cur, gmode);
+        cur ^= (int)(m & 1);
+      } else {
+        launch_step(c, gmode);
+      }
+      n += m;
+      if (ks < n_samples && sample_steps[ks] == n) scan_into(samp + (size_t)(ks++) * nb * nout);
       e = hipGetLastError();
-      if (e == hipSuccess && (n & 4095) == 4095) e = hipStreamSynchronize(c->stream);
+      if (e == hipSuccess && n / 4096 != (n - m) / 4096) e = hipStreamSynchronize(c->stream);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return srs::fail(SRS_E_HIP, "adjoint forward: %s", hipGetErrorString(e));

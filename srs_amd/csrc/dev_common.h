@@ -340,6 +340,9 @@ struct DevMem {
 int check_real(srs_ctx* c, int32_t r);
 void launch_sweep(srs_ctx* c);
 int launch_step(srs_ctx* c, uint32_t gmode);
+// m steps of a single-realization context through the gated sweep, starting
+// from live state index cur, ending with the gating of the last step
+void launch_gated_block(srs_ctx* c, int64_t m, int cur, uint32_t gmode);
 int drop_graphs(srs_ctx* c);
 int tiles_full(srs_ctx* c);
 int tiles_compact(srs_ctx* c);
