@@ -174,13 +174,15 @@ def test_protocol_loop_records_match_oracle_loop(gated):
     np.testing.assert_array_equal(recs_d, np.array(recs_o))
 
 
-@pytest.mark.parametrize("lds", [0, 1])
+@pytest.mark.parametrize("lds,chunk", [(0, 0), (1, 0), (1, 37)])
 @pytest.mark.parametrize("case", ["opens", "t_final"])
-def test_ffwd_gating_phase_matches_oracle_loop(case, lds):
+def test_ffwd_gating_phase_matches_oracle_loop(case, lds, chunk):
     """flag_ffwd: gating-only steps on the unchanged fields until a channel
     opens (or T_FINAL passes), then the main loop from that time
     (cpu/RunSimulation.cpp:469-519).  Both sweep kernels skip the
-    realization while it is in the ffwd phase and after it is done."""
+    realization while it is in the ffwd phase and after it is done.  In
+    calls of 37 steps the calls after the ffwd phase run through the gated
+    sweep, the first ones through separate gating launches."""
     pr = _tiny(tt=True)
     p = pr.params.copy()
     dt = p.reals[1]
@@ -197,7 +199,11 @@ def test_ffwd_gating_phase_matches_oracle_loop(case, lds):
         if lds:
             ctx.tune(lds=1, occupancy=6)
         ctx.loop_set(0, n_open=0, ffwd=1)
-        ctx.explicit_step(nsteps + 5, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        if chunk:
+            for _ in range((nsteps + 5 + chunk - 1) // chunk):
+                ctx.explicit_step(chunk, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        else:
+            ctx.explicit_step(nsteps + 5, S.STEP_PROTOCOL | S.STEP_GRAPH)
         recs_d = ctx.records(0)
         ls = ctx.loop_get(0)
         X = ctx.state_get(0)
