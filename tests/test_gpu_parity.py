@@ -458,6 +458,27 @@ def test_no_channels_and_done_realizations():
     assert rel_err(X0, Fo.X) <= TOL
 
 
+
+def test_single_realization_without_channels_and_done():
+    """The gated sweep's edge cases on a single realization: no channel
+    elements at all (the gating workgroup has nothing to update) against the
+    oracle, and a finished realization, which no call steps."""
+    pr = _tiny(ryr=0, lcc=0)
+    F = initial_fields(pr)
+    with gpu_context([pr], [F]) as ctx:
+        ctx.explicit_step(20)
+        ctx.explicit_step(150, S.STEP_GRAPH)
+        X0 = ctx.state_get(0)
+        ls = ctx.loop_get(0)
+        assert ls["step"] == 170
+        ctx.loop_set(0, ls["t"], ls["dt_last"], ls["step"], 1)
+        ctx.explicit_step(40, S.STEP_GRAPH)
+        ctx.explicit_step(3)
+        assert np.array_equal(ctx.state_get(0), X0) and ctx.loop_get(0)["step"] == 170
+    Fo = Oracle().steps(grid_dict(pr.grid, pr.aij, pr.nij), pr.params.reals, F.copy(), 170, pr.params.reals[1],
+                        gating=True, rng_kind=RNG_PHILOX, key=1000, stream=0)
+    assert rel_err(X0, Fo.X) <= TOL
+
 @pytest.mark.parametrize("group", [1, 2])
 def test_finished_realizations_leave_the_sweep(group):
     """Protocol runs that end at different times, stepped in chunks: the
