@@ -72,10 +72,12 @@ def output_due(t, dt_last, iv):
     return math.floor(t / iv) > math.floor((t - dt_last) / iv)
 
 
-def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, max_steps=10 ** 7, rng_kind=RNG_PHILOX):
+def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, max_steps=10 ** 7, rng_kind=RNG_PHILOX,
+                t_lcc_close=None):
     """RunSimulation's main loop with the oracle's step (Philox gating).
     Returns the records written inside and after the loop (the t = 0 record
-    before the loop is the caller's)."""
+    before the loop is the caller's).  t_lcc_close: T_LCC_Close of an LCC
+    triggered at t = 0 (cpu/RunSimulation.cpp:459-463, 525-528)."""
     G = grid_dict(pr.grid, pr.aij, pr.nij)
     P = p.reals
     Iv = p.ints
@@ -108,6 +110,9 @@ def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, ma
                 raise RuntimeError("oracle ffwd loop did not terminate")
     while t <= P[S.R["T_FINAL"]] and (not fid or ((lcc_open or n_open > 0 or ca_max > P[S.R["C_MIN"]])
                                                   and n_open < Iv[S.I["N_RYR_FIDELITY"]])):
+        if Iv[S.I["FLAG_LCC"]] and lcc_open and t_lcc_close is not None and t >= t_lcc_close:
+            lcc_open = 0
+            F.lcc_state[int(Iv[S.I["LCC_INDEX"]])] = 1
         if Iv[S.I["FLAG_V_CLAMP"]]:
             F.gv[0] = P[S.R["V_CLAMP"]] if t <= P[S.R["T_CLAMP"]] else P[S.R["DEFAULT_VM"]]
         if ir != -1 and t < P[S.R["INIT_RYR_HOLD"]] and not F.ryr_state[ir]:

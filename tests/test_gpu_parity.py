@@ -174,6 +174,34 @@ def test_protocol_loop_records_match_oracle_loop(gated):
     np.testing.assert_array_equal(recs_d, np.array(recs_o))
 
 
+
+@pytest.mark.parametrize("gated", [0, 1])
+def test_lcc_trigger_protocol_matches_oracle_loop(gated):
+    """The LCC trigger protocol: an LCC held open from t = 0 is closed by the
+    loop body once t reaches T_LCC_Close (cpu/RunSimulation.cpp:459-463,
+    525-528), with the loop body in separate gating launches or in the gated
+    sweep."""
+    pr = _tiny(tt=True)
+    p = fast_gating(pr.params)
+    dt = p.reals[1]
+    p.setr(T_FINAL=400 * dt, OUTPUT_STATES_INTERVAL=13 * dt, V_CLAMP=0.0, DEFAULT_VM=0.0)
+    p.seti(INIT_RYR=-1, FLAG_V_CLAMP=0, FLAG_RYR_OUT=1, FLAG_LCC=1, LCC_INDEX=0)
+    F = initial_fields(pr, ryr_open=(), vm=0.0, lcc_state=1, seed=6)
+    F.lcc_state[0] = 6
+    tc = 150.5 * dt
+    O = Oracle()
+    recs_o, t_end, nsteps = oracle_loop(O, pr, p, F.copy(), key=31, stream=0, t_lcc_close=tc)
+    with gpu_context([pr], [F.copy()], params=[p], seeds=[31], record_capacity=256) as ctx:
+        ctx.tune(gated_sweep=gated)
+        ctx.loop_set(0, lcc_open=1, t_lcc_close=tc)
+        ctx.explicit_step(nsteps + 5, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        recs_d = ctx.records(0)
+        ls = ctx.loop_get(0)
+    assert ls["done"] == 1 and ls["lcc_open"] == 0 and ls["step"] == nsteps and ls["t"] == t_end
+    assert recs_d.shape[0] == len(recs_o)
+    np.testing.assert_array_equal(recs_d, np.array(recs_o))
+    assert nsteps > 151
+
 @pytest.mark.parametrize("lds,chunk", [(0, 0), (1, 0), (1, 37)])
 @pytest.mark.parametrize("case", ["opens", "t_final"])
 def test_ffwd_gating_phase_matches_oracle_loop(case, lds, chunk):
