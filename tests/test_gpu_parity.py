@@ -202,6 +202,36 @@ def test_lcc_trigger_protocol_matches_oracle_loop(gated):
     np.testing.assert_array_equal(recs_d, np.array(recs_o))
     assert nsteps > 151
 
+
+@pytest.mark.parametrize("gated", [0, 1])
+@pytest.mark.parametrize("c_min", [0.0, 0.5])
+def test_fidelity_stop_matches_oracle_loop(gated, c_min):
+    """The fidelity protocol's while-condition (cpu/RunSimulation.cpp:522):
+    the run stops at the first states output with N_RYR_FIDELITY RyRs open
+    (c_min 0) or once no RyR is open and the largest cleft Ca has fallen to
+    C_MIN (c_min 0.5, above the resting Ca), with N_open and the maximum
+    refreshed only at outputs."""
+    pr = _tiny(tt=True)
+    p = pr.params.copy()
+    dt = p.reals[1]
+    c0 = float(pr.X0[pr.grid.ryr_ele[0]])
+    p.setr(RYR_K_PLUS=1.0 / (40 * dt * c0 ** p.reals[S.R["RYR_ETA"]]), RYR_K_MINUS=1.0 / (90 * dt),
+           T_FINAL=3000 * dt, OUTPUT_STATES_INTERVAL=11 * dt, C_MIN=c_min, V_CLAMP=0.0, DEFAULT_VM=0.0)
+    p.seti(INIT_RYR=-1, FLAG_V_CLAMP=0, FLAG_RYR_OUT=1, FLAG_FIDELITY_SIM=1, N_RYR_FIDELITY=2)
+    F = initial_fields(pr, ryr_open=(0,), vm=0.0, lcc_state=1, seed=12)
+    O = Oracle()
+    recs_o, t_end, nsteps = oracle_loop(O, pr, p, F.copy(), key=41, stream=0)
+    with gpu_context([pr], [F.copy()], params=[p], seeds=[41], record_capacity=512) as ctx:
+        ctx.tune(gated_sweep=gated)
+        ctx.loop_set(0, n_open=int(F.ryr_state.sum()))
+        ctx.explicit_step(nsteps + 5, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        recs_d = ctx.records(0)
+        ls = ctx.loop_get(0)
+    assert ls["done"] == 1 and ls["step"] == nsteps and ls["t"] == t_end
+    assert recs_d.shape[0] == len(recs_o)
+    np.testing.assert_array_equal(recs_d, np.array(recs_o))
+    assert t_end < p.reals[S.R["T_FINAL"]]  # the fidelity condition, not T_FINAL, ended it
+
 @pytest.mark.parametrize("lds,chunk", [(0, 0), (1, 0), (1, 37)])
 @pytest.mark.parametrize("case", ["opens", "t_final"])
 def test_ffwd_gating_phase_matches_oracle_loop(case, lds, chunk):
