@@ -241,70 +241,27 @@ def bench_adjoint(t_sim=5.0, n_points=64, n_samples=50, ckpt=256):
     the cluster centre sampled 50 times, target from theta moved by +-10%.
     Times one loss evaluation and one loss + gradient (checkpoints every
     `ckpt` steps, recomputation)."""
-    import srs_amd as S
     from srs_amd import workloads as W
 
-    pr = W.prepare_spark()
-    g = pr.grid
-    p = pr.params.copy()
-    dt = float(p.reals[S.R["T_STEP"]])
-    n_steps = int(round(t_sim / dt))
-    ctx = S.Context(0)
-    mid = ctx.add_mesh_grid(g, pr.aij, pr.nij, perm=W.kuhn_block_order(g))
-    ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=38376427, stream=0)
-    ctx.finalize()
-    rs, ls, lvs, R = W.initial_channels(pr)
-    ctx.state_set(0, pr.X0)
-    ctx.channels_set(0, rs, ls, lvs, R)
-    ctx.globals_set(0, 0.0, p.reals[S.R["DEFAULT_CA_NSR"]])
     t0 = time.perf_counter()
-    schedule = ctx.record_schedule(0, n_steps)
-    t_rec = time.perf_counter() - t0
-    ctx.channels_set(0, schedule[0], ls, lvs, R)
-    cen = g.centroids()
-    c0 = np.array([0.5, 0.5, 0.3])
-    d = S.read_scan_parameters(None)
-    d.axis = 0
-    rc = cen[g.ryr_ele].mean(axis=0) - c0
-    d.offset1, d.offset2 = float(rc[1]), float(rc[2])
-    d.x_min, d.x_max = -0.48, 0.48
-    d.res_x_f = (d.x_max - d.x_min) / (n_points + 0.5)  # the point count truncates (x_max - x_min) / res
-    d.res_x_ca = 0.01
-    d.domain[0], d.domain[1], d.domain[2] = 1.0, 1.0, 0.6
-    d.pad_min, d.pad_max, d.pad_res = -1.5, 1.5, 0.1
-    sid, nf, _ = ctx.add_scan(mid, cen - c0, d)
-    samples = np.linspace(n_steps // n_samples, n_steps, n_samples).astype(np.int64)
-    idx = S.adjoint_theta()
-    sgn = np.where(np.random.default_rng(7).random(len(idx)) < 0.5, -1.0, 1.0)
-    Pt = p.reals.copy()
-    Pt[idx] *= 1 + 0.1 * sgn
-    # target: dF/F0 of the same frozen-channel run at the moved parameters
-    ctx.params_set(0, Pt)
-    ctx.state_set(0, pr.X0)
-    tgt, prev = [], 0
-    for smp in samples:
-        ctx.replay(0, schedule[prev:smp])
-        prev = int(smp)
-        tgt.append(S.dff0(ctx.scan_sample(0, sid)[2], d.f0))
-    tgt = np.array(tgt)
-    ctx.state_set(0, pr.X0)
-    ctx.channels_set(0, schedule[0], ls, lvs, R)
-    L0 = ctx.loss_grad(0, sid, schedule, samples, tgt, ckpt, grad=False)  # zero at the moved parameters
-    ctx.params_set(0, p.reals)
+    prob, info = W.spark_adjoint_problem(t_sim, n_points, n_samples, ckpt)
+    t_setup = time.perf_counter() - t0
+    th = prob.theta()
     t0 = time.perf_counter()
-    Lf = ctx.loss_grad(0, sid, schedule, samples, tgt, ckpt, grad=False)
+    Lf = prob.loss(th)
     t_fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
-    L, grad = ctx.loss_grad(0, sid, schedule, samples, tgt, ckpt)
+    L, grad = prob.loss_grad(th)
     t_grad = time.perf_counter() - t0
-    ctx.close()
-    return dict(workload=f"spark_box h=40 nm ({g.n_ele} elements), {t_sim} ms = {n_steps} steps, RyR schedule frozen, "
-                         f"{nf}-point line scan x {n_samples} samples, checkpoint every {ckpt} steps, "
-                         f"{len(idx)} parameters",
-                n_steps=n_steps, forward_s=t_fwd, loss_and_gradient_s=t_grad, schedule_record_s=t_rec,
+    prob.ctx.close()
+    n_steps = info["n_steps"]
+    return dict(workload=f"spark_box h=40 nm ({info['n_ele']} elements), {t_sim} ms = {n_steps} steps, RyR schedule "
+                         f"frozen, {info['n_f']}-point line scan x {n_samples} samples, checkpoint every {ckpt} steps, "
+                         f"{len(th)} parameters",
+                n_steps=n_steps, forward_s=t_fwd, loss_and_gradient_s=t_grad, setup_s=t_setup,
                 gradient_over_forward=t_grad / t_fwd, us_per_step_forward=1e6 * t_fwd / n_steps,
                 us_per_step_adjoint=1e6 * (t_grad - t_fwd) / n_steps, loss=L, loss_check=abs(L - Lf) <= 1e-12 * L,
-                grad_norm=float(np.linalg.norm(grad * p.reals[idx])), loss_at_target_parameters=L0)
+                grad_norm=float(np.linalg.norm(grad * th)), loss_at_target_parameters=info["loss_at_target"])
 
 
 def bench_adjoint_batch(n_real=256, n_steps=256, n_points=64, n_samples=8, max_batch=128):

@@ -233,6 +233,13 @@ int srs_states_record(double t, int32_t n_ryr, const int32_t* ryr_state, const d
 int srs_run_cru3d(const char* param_path, const char* mesh_base, const char* out_dir, int32_t rank,
                   int32_t start_index, int32_t gpu_layout, double* wall_seconds);
 
+/* Element order the cru3d driver gives its context: elements sorted along a
+ * Morton curve through their centroids (1024 cells per axis over the bounding
+ * box), perm[k] = element stored at position k.  No reference counterpart
+ * (the reference keeps its BFS order on the device, cpu/InitFuncs.cpp:679-731);
+ * the order moves data, never changes results. */
+int srs_morton_order(int32_t n_ele, const int32_t* ele_nodes, const double* node_xyz, int32_t* perm);
+
 /* ======================================================= device layer */
 
 typedef struct srs_ctx srs_ctx;
@@ -274,6 +281,11 @@ int srs_ctx_tune(srs_ctx* ctx, int32_t key, int32_t value);
 
 /* Replaces FreeDomainStruct / FreeGlobalStruct (gpu/RunSimulation.cu:874-901). */
 int srs_ctx_destroy(srs_ctx* ctx);
+/* States records in the GPU program's layout: WriteToStateFiles of gpu/
+ * (gpu/OutputFuncs.cpp:277-284) appends Ca_cyt at each RyR after the seven
+ * fields, before the optional RyR states.  0 (the default) keeps the CPU
+ * layout (cpu/OutputFuncs.cpp:244-350).  Before srs_ctx_finalize. */
+int srs_ctx_record_layout(srs_ctx* ctx, int32_t gpu_layout);
 
 /* Geometry of one mesh, shared by all realizations on it.  Replaces the
  * DeviceDomainStruct upload of InitializeDomainStruct (gpu/InitFuncs.cu:1001-1093,

@@ -245,6 +245,14 @@ def bfs_order(n_ele, ele1, ele2) -> np.ndarray:
     return out
 
 
+def morton_order(g: Grid) -> np.ndarray:
+    """The device element order of the cru3d driver: perm[k] = element stored at k."""
+    perm = np.zeros(g.n_ele, np.int32)
+    check(lib().srs_morton_order(g.n_ele, _i32(np.ascontiguousarray(g.ele_nodes, np.int32)),
+                                 _f64(np.ascontiguousarray(g.node_xyz, np.float64)), _i32(perm)))
+    return perm
+
+
 def assemble_operator(g: Grid) -> tuple[np.ndarray, np.ndarray]:
     """InitializeExplicitSparseMatrixFast: aij[5N], nij[4N] in the reference packing."""
     aij = np.zeros(5 * g.n_ele, np.float64)
@@ -283,10 +291,11 @@ def states_record(t, ryr_state, ca_cyt, ca_jsr, lcc_state, lccv_state, ca_lcc, v
     return rec[:n].copy()
 
 
-def run_cru3d(param: str, mesh: str, out_dir: str | None, rank: int = 0, start: int = 0) -> float:
+def run_cru3d(param: str, mesh: str, out_dir: str | None, rank: int = 0, start: int = 0,
+              gpu_layout: bool = False) -> float:
     secs = C.c_double(0)
-    check(lib().srs_run_cru3d(param.encode(), mesh.encode(), out_dir.encode() if out_dir else None, rank, start, 0,
-                              C.byref(secs)))
+    check(lib().srs_run_cru3d(param.encode(), mesh.encode(), out_dir.encode() if out_dir else None, rank, start,
+                              int(gpu_layout), C.byref(secs)))
     return secs.value
 
 
@@ -397,6 +406,10 @@ class Context:
                                             C.c_uint64(seed & (2**64 - 1)), C.c_uint64(stream), C.byref(rid)))
         self.n_real += 1
         return rid.value
+
+    def record_layout(self, gpu_layout: bool):
+        """States records with Ca_cyt at each RyR (the GPU program's layout); before finalize."""
+        check(lib().srs_ctx_record_layout(self._h, int(gpu_layout)))
 
     def finalize(self, record_capacity: int = 0):
         check(lib().srs_ctx_finalize(self._h, record_capacity))

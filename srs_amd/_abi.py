@@ -96,12 +96,14 @@ _PROTOS = {
     "srs_equilibrium_state": (C.c_int, [C.c_int32, i32p, i32p, f64p, f64p, f64p]),
     "srs_box_mesh_write": (C.c_int, [C.POINTER(SrsBoxSpec), C.c_char_p]),
     "srs_box_grid": (C.c_int, [C.POINTER(SrsBoxSpec), C.c_int, C.POINTER(C.POINTER(SrsGrid))]),
+    "srs_morton_order": (C.c_int, [C.c_int32, i32p, f64p, i32p]),
     "srs_states_record": (C.c_int, [C.c_double, C.c_int32, i32p, f64p, f64p, C.c_int32, i32p, i32p, f64p,
                                     C.c_double, f64p, C.c_int32, C.c_int32, f32p]),
     "srs_run_cru3d": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32,
                                 C.POINTER(C.c_double)]),
     "srs_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "srs_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "srs_ctx_record_layout": (C.c_int, [C.c_void_p, C.c_int32]),
     "srs_ctx_tune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "srs_ctx_add_mesh": (C.c_int, [C.c_void_p, C.POINTER(SrsMeshDesc), i32p]),
     "srs_ctx_add_realization": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(SrsChanDesc), i32p, f64p,

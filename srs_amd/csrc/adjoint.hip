@@ -149,9 +149,12 @@ __global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict_
     // diffusion sums (for the D gradients)
     const double ac = c * g.a0 + m1.x * g.a1 + m2.x * g.a2 + m3.x * g.a3 + m4.x * g.a4;
     const double ad = bD * g.a0 + m1.w * g.a1 + m2.w * g.a2 + m3.w * g.a3 + m4.w * g.a4;
+    // boundary rates are D times a D-free plane: they enter the D gradients
+    const double bu_c = (f & F_BR) ? br_unit(d.br_mesh[j], d.brd[0]) : 0.0;
+    const double bu_d = (f & F_BR) ? br_unit(d.br_mesh[np + j], d.brd[1]) : 0.0;
     double rca = P[SRS_R_D_CA] * ac + ((f & F_BR) ? d.br[j] : 0.0);
-    gr[0] += dt * u * ac;          // D_CA
-    gr[1] += dt * mu.w * ad;       // D_DYE
+    gr[0] += dt * u * (ac + bu_c);       // D_CA
+    gr[1] += dt * mu.w * (ad + bu_d);    // D_DYE
     if (!(f & F_JSR)) {
       if (f & F_TROPC) {
         const double kon = P[SRS_R_K_ON_TRPN], koff = P[SRS_R_K_OFF_TRPN], B = P[SRS_R_B_TOT_TRPN];
@@ -365,6 +368,23 @@ int srs_params_set(srs_ctx* c, int32_t r, const double* reals) {
   memcpy(hr.desc.P, reals, sizeof(hr.desc.P));
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
+  const HostMesh& hm = *c->meshes[hr.mesh];
+  if (!hm.br.empty()) {
+    // boundary rates are D Face_C b / V0: at a D other than the one the
+    // mesh's rates were built with, the realization steps with rates of its own
+    const double D[4] = {reals[SRS_R_D_CA], reals[SRS_R_D_DYE], reals[SRS_R_D_ATP], reals[SRS_R_D_CMDN]};
+    if (std::equal(D, D + 4, hm.br_d)) {
+      hr.desc.br = hm.d_br;
+    } else {
+      if (!hr.d_br_own && (rc = dev_alloc(c, &hr.d_br_own, hm.br.size()))) return rc;
+      std::vector<double> own(hm.br.size());
+      const size_t np = hm.np;
+      for (int q = 0; q < 4; ++q)
+        for (size_t k = 0; k < np; ++k) own[q * np + k] = br_scaled(hm.br[q * np + k], D[q], hm.br_d[q]);
+      HIPCHK(hipMemcpy(hr.d_br_own, own.data(), own.size() * sizeof(double), hipMemcpyHostToDevice));
+      hr.desc.br = hr.d_br_own;
+    }
+  }
   HIPCHK(hipMemcpy(c->d_desc + r, &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice));
   RealLive lv;
   if ((rc = live_rw(c, r, &lv, false))) return rc;

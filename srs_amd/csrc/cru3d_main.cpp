@@ -1,5 +1,7 @@
 // cru3d_main.cpp -- command-line driver with the reference's options
-// (cpu/main.cpp:87-102): -param P.xml -mesh base [-out dir] [-start i] [-rank r].
+// (cpu/main.cpp:87-102): -param P.xml -mesh base [-out dir] [-start i] [-rank r],
+// plus -layout gpu for states records in the GPU program's layout
+// (gpu/OutputFuncs.cpp:277-284).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -10,7 +12,7 @@ int main(int argc, char** argv) {
   const char* param = "TestSet.xml";  // DEFAULT_PARAMETER_FILE, cpu/parameters.h:105
   const char* mesh = "mesh";
   const char* out = nullptr;
-  int rank = 0, start = 0;
+  int rank = 0, start = 0, gpu_layout = 0;
   for (int i = 1; i < argc; i += 2) {
     if (i + 1 >= argc) {
       fprintf(stderr, "Error: option %s needs a value\n", argv[i]);
@@ -21,13 +23,15 @@ int main(int argc, char** argv) {
     else if (!strcmp(argv[i], "-param")) param = argv[i + 1];
     else if (!strcmp(argv[i], "-out")) out = argv[i + 1];
     else if (!strcmp(argv[i], "-mesh")) mesh = argv[i + 1];
+    else if (!strcmp(argv[i], "-layout") && !strcmp(argv[i + 1], "gpu")) gpu_layout = 1;
+    else if (!strcmp(argv[i], "-layout") && !strcmp(argv[i + 1], "cpu")) gpu_layout = 0;
     else {
       fprintf(stderr, "Error: unrecognized command line option: %s\n", argv[i]);
       return 1;
     }
   }
   double secs = 0;
-  const int rc = srs_run_cru3d(param, mesh, out, rank, start, 0, &secs);
+  const int rc = srs_run_cru3d(param, mesh, out, rank, start, gpu_layout, &secs);
   if (rc != SRS_OK) {
     fprintf(stderr, "Error (%d): %s\n", rc, srs_last_error());
     return 1;

@@ -63,7 +63,9 @@ struct RealDesc {
   const uint8_t* flags;   // [np]
   const double* V0;       // [np]
   const double* tt;       // [np]
-  const double* br;       // [4][np] Ca, Fluo, ATP, CaM boundary rates, or null
+  const double* br;       // [4][np] Ca, Fluo, ATP, CaM boundary rates at this realization's D, or null
+  const double* br_mesh;  // [4][np] the mesh's boundary rates, built with the D in brd
+  double brd[4];          // the D_Ca, D_Fluo, D_ATP, D_CaM the mesh's boundary rates were built with
   double* X[2];           // [6][np] ping-pong state
   int32_t n, np;
   int32_t kst, slot;      // realization group: K realizations interleaved, this one in slot `slot`
@@ -88,6 +90,18 @@ struct RealDesc {
   double P[SRS_N_REAL];
   int32_t I[SRS_N_INT];
 };
+
+// A boundary rate at a realization's current D.  The mesh's planes hold
+// D_b Face_C b / V0 summed over an element's bound faces
+// (cpu/InitFuncs.cpp:1019-1024), with D_b the coefficient they were built
+// with (brd); the rate is linear in D, so a realization whose D has since been
+// changed (srs_params_set) gets planes of its own scaled by D / D_b, and one
+// whose D is unchanged uses the mesh's bits as they are.
+__host__ __device__ inline double br_scaled(double v, double D, double D_b) {
+  return (D == D_b || D_b == 0.0) ? v : v / D_b * D;
+}
+// The D-free part Face_C b / V0 of a boundary rate (its derivative in D).
+__host__ __device__ inline double br_unit(double v, double D_b) { return D_b == 0.0 ? 0.0 : v / D_b; }
 
 // Offset of element e's Ca in a realization's state buffer (its quad's first
 // double); Fluo is at +3.
@@ -225,6 +239,11 @@ struct HostMesh {
   std::vector<double> coef;   // [5][np]
   std::vector<uint8_t> flags;
   std::vector<double> V0, tt, br;  // br [4][np] or empty
+  // The D of the four diffusing species that br was built with
+  // (D Face_C b / V0, cpu/InitFuncs.cpp:1019-1024): those of the first
+  // realization added on the mesh, which every later one must share.
+  double br_d[4] = {0, 0, 0, 0};
+  bool br_d_set = false;
   // compact operator layout
   bool compact = false;
   std::vector<int16_t> off, dq;
@@ -240,6 +259,7 @@ struct HostMesh {
 
 struct HostReal {
   int32_t mesh = 0;
+  double* d_br_own = nullptr;  // boundary rates rescaled to this realization's D (srs_params_set)
   std::vector<int32_t> ryr_ele, ryr_jsr, ryr_nb, lcc_ele;  // dev numbering
   std::vector<int32_t> celem, cstart, centry;
   int32_t ints[SRS_N_INT];
@@ -276,6 +296,7 @@ struct srs_ctx {
   int lds = 1;           // stage each tile's quads in LDS for the neighbour gathers (groups of 1)
   int drop_finished = 1; // sweep without the tiles of groups that have all finished
   int gated = 1;         // fold each step's gating into the next step's sweep (k_sweep_gated) where it applies
+  int gpu_layout = 0;     // states records in the GPU program's layout (Ca_cyt at each RyR appended)
   bool compact_all = false;
   bool layout_fixed = false;
   hipStream_t stream = nullptr;

@@ -1148,6 +1148,15 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
   }
 }
 
+int srs_ctx_record_layout(srs_ctx* c, int32_t gpu_layout) {
+  srs::clear_error();
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  if (c->finalized) return srs::fail(SRS_E_STATE, "the record layout is fixed at srs_ctx_finalize");
+  if (gpu_layout != 0 && gpu_layout != 1) return srs::fail(SRS_E_ARG, "gpu_layout must be 0 or 1");
+  c->gpu_layout = gpu_layout;
+  return SRS_OK;
+}
+
 int srs_ctx_destroy(srs_ctx* c) {
   if (!c) return SRS_OK;
   (void)hipSetDevice(c->device);
@@ -1286,6 +1295,20 @@ int srs_ctx_add_realization(srs_ctx* c, int32_t mesh_id, const srs_chan_desc* ch
     hr.centry.push_back(entries[k].second);
   }
   hr.cstart.push_back((int32_t)entries.size());
+  if (!hm.br.empty()) {
+    // The mesh's boundary rates were built with one set of D; every
+    // realization on the mesh must start from it (srs_params_set may move it
+    // later, and the rates follow).
+    const double D[4] = {reals[SRS_R_D_CA], reals[SRS_R_D_DYE], reals[SRS_R_D_ATP], reals[SRS_R_D_CMDN]};
+    if (!hm.br_d_set) {
+      std::copy(D, D + 4, hm.br_d);
+      hm.br_d_set = true;
+    } else if (!std::equal(D, D + 4, hm.br_d)) {
+      return srs::fail(SRS_E_ARG, "realizations on a mesh with boundary rates must share the D the rates were "
+                                  "built with (D_Ca %g, D_Fluo %g, D_ATP %g, D_CaM %g)", hm.br_d[0], hm.br_d[1],
+                       hm.br_d[2], hm.br_d[3]);
+    }
+  }
   *real_id = (int32_t)c->reals.size();
   c->reals.push_back(std::move(hr));
   return SRS_OK;
@@ -1364,6 +1387,8 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
     HostMesh& hm = *c->meshes[hr.mesh];
     RealDesc d{};
     d.nbr = hm.d_nbr; d.coef = hm.d_coef; d.flags = hm.d_flags; d.V0 = hm.d_V0; d.tt = hm.d_tt; d.br = hm.d_br;
+    d.br_mesh = hm.d_br;
+    std::copy(hm.br_d, hm.br_d + 4, d.brd);
     d.off = hm.d_off; d.cls = hm.d_cls; d.dq = hm.d_dq; d.tab = hm.d_tab;
     d.n_class = hm.n_class;
     d.n = hm.n; d.np = hm.np;
@@ -1390,8 +1415,10 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
     double* pr;
     if ((rc = dev_upload(c, &pr, zr))) return rc; d.R = pr;
     d.ryr_out = hr.ints[SRS_I_FLAG_RYR_OUT] ? 1 : 0;
-    d.gpu_layout = 0;
-    d.rec_len = 7 + (d.ryr_out ? d.n_ryr : 0);
+    d.gpu_layout = c->gpu_layout;
+    d.rec_len = 7 + (d.gpu_layout ? d.n_ryr : 0) + (d.ryr_out ? d.n_ryr : 0);
+    if (record_capacity > 0 && d.rec_len > 255)  // cpu/OutputFuncs.cpp:441-444
+      return srs::fail(SRS_E_ARG, "states record of %d floats exceeds the 255-float record limit", d.rec_len);
     d.rec_cap = record_capacity;
     if (record_capacity > 0) {
       float* fr;

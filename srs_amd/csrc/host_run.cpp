@@ -51,44 +51,6 @@ struct Guard {
   }
 };
 
-// Device element order for the drop-in: elements sorted along a Morton
-// (Z-order) curve through their centroids on a 1024^3 grid over the mesh's
-// bounding box, so that a sweep tile covers a compact patch of the mesh and
-// its neighbour gathers stay in cache.  The order changes where elements are
-// stored, never the arithmetic.  Returns perm[k] = element stored at k.
-std::vector<int32_t> morton_order(const srs_grid* g) {
-  const int n = g->n_ele;
-  std::vector<double> cen(3 * (size_t)n);
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int e = 0; e < n; ++e)
-    for (int a = 0; a < 3; ++a) {
-      double c = 0.0;
-      for (int q = 0; q < 4; ++q) c += g->node_xyz[3 * (size_t)g->ele_nodes[4 * (size_t)e + q] + a];
-      c *= 0.25;
-      cen[3 * (size_t)e + a] = c;
-      lo[a] = std::min(lo[a], c);
-      hi[a] = std::max(hi[a], c);
-    }
-  auto spread = [](uint64_t v) {  // bit b of v to bit 3b
-    uint64_t r = 0;
-    for (int b = 0; b < 10; ++b) r |= ((v >> b) & 1u) << (3 * b);
-    return r;
-  };
-  std::vector<std::pair<uint64_t, int32_t>> key(n);
-  for (int e = 0; e < n; ++e) {
-    uint64_t k = 0;
-    for (int a = 0; a < 3; ++a) {
-      const double w = hi[a] > lo[a] ? (cen[3 * (size_t)e + a] - lo[a]) / (hi[a] - lo[a]) : 0.0;
-      k |= spread((uint64_t)std::lround(w * 1023.0)) << a;
-    }
-    key[e] = {k, e};
-  }
-  std::sort(key.begin(), key.end());
-  std::vector<int32_t> perm(n);
-  for (int k = 0; k < n; ++k) perm[k] = key[k].second;
-  return perm;
-}
-
 // cpu/OutputFuncs.cpp:42-220, streamed instead of a fixed 10 MB buffer.
 int write_vtk(const std::string& path, int run, int rank, int n, double t, const srs_grid* g, const double* X,
               const int32_t* ryr_state) {
@@ -135,6 +97,45 @@ int append_records(const std::string& path, const float* rec, size_t n_floats) {
 }
 
 }  // namespace
+
+extern "C" int srs_morton_order(int32_t n_ele, const int32_t* ele_nodes, const double* node_xyz, int32_t* perm) {
+  // Device element order for the drop-in: elements sorted along a Morton
+  // (Z-order) curve through their centroids on a 1024^3 grid over the mesh's
+  // bounding box, so that a sweep tile covers a compact patch of the mesh and
+  // its neighbour gathers stay in cache.  The order changes where elements are
+  // stored, never the arithmetic.  perm[k] = element stored at k.
+  srs::clear_error();
+  if (n_ele < 0 || (n_ele > 0 && (!ele_nodes || !node_xyz || !perm))) return srs::fail(SRS_E_ARG, "null argument");
+  const int n = n_ele;
+  std::vector<double> cen(3 * (size_t)n);
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int e = 0; e < n; ++e)
+    for (int a = 0; a < 3; ++a) {
+      double c = 0.0;
+      for (int q = 0; q < 4; ++q) c += node_xyz[3 * (size_t)ele_nodes[4 * (size_t)e + q] + a];
+      c *= 0.25;
+      cen[3 * (size_t)e + a] = c;
+      lo[a] = std::min(lo[a], c);
+      hi[a] = std::max(hi[a], c);
+    }
+  auto spread = [](uint64_t v) {  // bit b of v to bit 3b
+    uint64_t r = 0;
+    for (int b = 0; b < 10; ++b) r |= ((v >> b) & 1u) << (3 * b);
+    return r;
+  };
+  std::vector<std::pair<uint64_t, int32_t>> key(n);
+  for (int e = 0; e < n; ++e) {
+    uint64_t k = 0;
+    for (int a = 0; a < 3; ++a) {
+      const double w = hi[a] > lo[a] ? (cen[3 * (size_t)e + a] - lo[a]) / (hi[a] - lo[a]) : 0.0;
+      k |= spread((uint64_t)std::lround(w * 1023.0)) << a;
+    }
+    key[e] = {k, e};
+  }
+  std::sort(key.begin(), key.end());
+  for (int k = 0; k < n; ++k) perm[k] = key[k].second;
+  return SRS_OK;
+}
 
 extern "C" int srs_run_cru3d(const char* param_path, const char* mesh_base, const char* out_dir, int32_t rank,
                              int32_t start_index, int32_t gpu_layout, double* wall_seconds) {
@@ -194,7 +195,8 @@ extern "C" int srs_run_cru3d(const char* param_path, const char* mesh_base, cons
   const int n_sims = I[SRS_I_N_SIMS] - start_index;
   if (n_sims <= 0) return SRS_OK;
   if ((rc = srs_ctx_create(I[SRS_I_CUDA_DEVICE], &G.ctx))) return rc;
-  const std::vector<int32_t> perm = morton_order(g);
+  std::vector<int32_t> perm(N);
+  if ((rc = srs_morton_order(N, g->ele_nodes, g->node_xyz, perm.data()))) return rc;
   srs_mesh_desc md{N, aij.data(), nij.data(), br.data(), g->domain, g->tropc, g->V0, g->tt_area, SRS_ORDER_CUSTOM,
                    perm.data()};
   int32_t mesh_id;
@@ -214,13 +216,13 @@ extern "C" int srs_run_cru3d(const char* param_path, const char* mesh_base, cons
   const double iv = P[SRS_R_OUTPUT_STATES_INTERVAL];
   const int64_t cap = (int64_t)std::ceil(P[SRS_R_T_FINAL] / iv) + 16;
   if (cap > (1 << 26)) return srs::fail(SRS_E_ARG, "too many states outputs requested (%lld)", (long long)cap);
-  if ((rc = srs_ctx_finalize(G.ctx, (int32_t)cap))) return rc;
-
   const int nr = g->n_ryr, nl = g->n_lcc;
   const int ryr_out = I[SRS_I_FLAG_RYR_OUT] ? 1 : 0;
+  gpu_layout = gpu_layout ? 1 : 0;
   const int rec_len = 7 + (gpu_layout ? nr : 0) + (ryr_out ? nr : 0);
   if (rec_len > 255) return srs::fail(SRS_E_ARG, "states record of %d floats exceeds the 255-float limit", rec_len);
-  if (gpu_layout) return srs::fail(SRS_E_ARG, "the GPU states layout is available through srs_states_record only");
+  if ((rc = srs_ctx_record_layout(G.ctx, gpu_layout))) return rc;
+  if ((rc = srs_ctx_finalize(G.ctx, (int32_t)cap))) return rc;
   std::vector<std::string> state_files(n_sims);
   std::vector<int> grid_counter(n_sims, 0);
   std::vector<float> rec(rec_len);
@@ -283,7 +285,7 @@ extern "C" int srs_run_cru3d(const char* param_path, const char* mesh_base, cons
     for (int i = 0; i < nr; ++i) { ca_c[i] = X0[g->ryr_ele[i]]; ca_j[i] = X0[g->ryr_jsr[i]]; }
     for (int i = 0; i < nl; ++i) ca_l[i] = X0[g->lcc_ele[i]];
     srs_record_fill(0.0, nr, rs.data(), ca_c.data(), ca_j.data(), nl, ls.data(), lvs.data(), ca_l.data(),
-                    P[SRS_R_DEFAULT_VM], P, ryr_out, 0, rec.data());
+                    P[SRS_R_DEFAULT_VM], P, ryr_out, gpu_layout, rec.data());
     state_files[k] = dir + "/states_" + std::to_string(rank) + "_" + std::to_string(run) + ".out";
     {
       FILE* f = fopen(state_files[k].c_str(), "wb");

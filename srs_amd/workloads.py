@@ -240,6 +240,42 @@ def initial_channels(pr: Prepared, open_ryr=None, seed=0):
     return rs, np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R
 
 
+def ensemble_meshes():
+    """The three ensemble meshes (h = 40/30/25 nm): (grid, aij, nij, dt) each,
+    dt = 0.9 x the stability bound at D_Ca = 0.22."""
+    out = []
+    for m in range(3):
+        g = box_grid(ensemble_mesh_spec(m))
+        aij, nij = assemble_operator(g)
+        out.append((g, aij, nij, stable_dt(aij, 0.22)))
+    return out
+
+
+def ensemble_member(mem: dict, mesh, r: int, seed: int = 38376427, t_final: float = 50.0, with_state: bool = True):
+    """Everything one ensemble realization starts from: its parameters (dt of
+    its mesh, SR load as CA_SR, the middle RyR of its cluster as the initial
+    one), its channel arrays (a subset of the mesh's 12 x 12 lattice), its
+    Philox key and stream, and (with_state) the equilibrium fields, initial
+    channel states, ln U residuals and globals."""
+    g, aij, nij, dt = mesh
+    idx, nb = lattice_subset(mem["mask"])
+    nr = len(idx)
+    p = spark_params(t_final=t_final)
+    p.setr(T_STEP=dt, INIT_RYR_HOLD=dt, C_0_CA_JSR=mem["ca_sr"], DEFAULT_CA_NSR=mem["ca_sr"])
+    p.seti(INIT_RYR=int(nr // 2))
+    out = dict(p=p, ryr_ele=g.ryr_ele[idx], ryr_jsr=g.ryr_jsr[idx], ryr_nb=nb, lcc_ele=g.lcc_ele, key=seed + r,
+               stream=r, n_ryr=nr)
+    if with_state:
+        X0, _ = equilibrium_state(g, p.reals)
+        rs = np.zeros(nr, np.int32)
+        rs[nr // 2] = 1
+        rng = np.random.default_rng(seed + r)
+        R = np.log(rng.random(nr + g.n_lcc) * (1 - 2**-53) + 2**-54)
+        out.update(X0=X0, ryr_state=rs, lcc_state=np.ones(g.n_lcc, np.int32),
+                   lccv_state=np.full(g.n_lcc, 2, np.int32), R=R, gv=(0.0, float(p.reals[16])))
+    return out
+
+
 def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0, t_final: float = 50.0,
                      order: str = "block", tune: dict | None = None, only=None, log=None):
     """The batched Monte Carlo workload in one context: returns (ctx, dt per
@@ -257,45 +293,34 @@ def ensemble_context(n_real: int, seed: int = 38376427, record_capacity: int = 0
     ctx = Context(0)
     tune = dict(tune or {})
     ctx.tune(**{k: tune[k] for k in ("tile", "plane_pad", "compact", "group") if k in tune})
-    meshes = []
-    for m in range(3):
-        g = box_grid(ensemble_mesh_spec(m))
-        aij, nij = assemble_operator(g)
+    meshes = ensemble_meshes()
+    mids = []
+    for g, aij, nij, _ in meshes:
         if order == "block":
-            mid = ctx.add_mesh(aij, nij, g.domain, g.tropc, g.V0, g.tt_area, perm=kuhn_block_order(g))
+            mids.append(ctx.add_mesh(aij, nij, g.domain, g.tropc, g.V0, g.tt_area, perm=kuhn_block_order(g)))
         else:
-            mid = ctx.add_mesh(aij, nij, g.domain, g.tropc, g.V0, g.tt_area,
-                               order=ORDER_RCM if order == "rcm" else ORDER_AS_GIVEN)
-        meshes.append((g, aij, nij, mid, stable_dt(aij, 0.22)))
+            mids.append(ctx.add_mesh(aij, nij, g.domain, g.tropc, g.V0, g.tt_area,
+                                     order=ORDER_RCM if order == "rcm" else ORDER_AS_GIVEN))
     dts, n_ele, nbytes = [], 0, 0
-    plist = []
     for m in sorted({members[r]["mesh"] for r in picked}):
         nbytes += 65 * meshes[m][0].n_ele
-    for r in picked:
-        mem = members[r]
-        g, aij, nij, mid, dt = meshes[mem["mesh"]]
-        idx, nb = lattice_subset(mem["mask"])
-        p = spark_params(t_final=t_final)
-        p.setr(T_STEP=dt, INIT_RYR_HOLD=dt, C_0_CA_JSR=mem["ca_sr"], DEFAULT_CA_NSR=mem["ca_sr"])
-        p.seti(INIT_RYR=int(len(idx) // 2))
-        ctx.add_realization(mid, g.ryr_ele[idx], g.ryr_jsr[idx], nb, g.lcc_ele, p, seed=seed + r, stream=r)
-        dts.append(dt)
-        n_ele += g.n_ele
-        nbytes += 96 * g.n_ele
-        plist.append((g, p, len(idx)))
-        if log and len(plist) % 256 == 0:
-            log(f"  added {len(plist)} realizations")
+    for k, r in enumerate(picked):
+        m = members[r]["mesh"]
+        e = ensemble_member(members[r], meshes[m], r, seed, t_final, with_state=False)
+        ctx.add_realization(mids[m], e["ryr_ele"], e["ryr_jsr"], e["ryr_nb"], e["lcc_ele"], e["p"], seed=e["key"],
+                            stream=e["stream"])
+        dts.append(meshes[m][3])
+        n_ele += meshes[m][0].n_ele
+        nbytes += 96 * meshes[m][0].n_ele
+        if log and (k + 1) % 256 == 0:
+            log(f"  added {k + 1} realizations")
     ctx.finalize(record_capacity)
-    for k, (g, p, nr) in enumerate(plist):
-        r = picked[k]
-        X0, _ = equilibrium_state(g, p.reals)
-        ctx.state_set(k, X0)
-        rs = np.zeros(nr, np.int32)
-        rs[nr // 2] = 1
-        rng = np.random.default_rng(seed + r)
-        R = np.log(rng.random(nr + g.n_lcc) * (1 - 2**-53) + 2**-54)
-        ctx.channels_set(k, rs, np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R)
-        ctx.globals_set(k, 0.0, p.reals[16])
+    for k, r in enumerate(picked):
+        m = members[r]["mesh"]
+        e = ensemble_member(members[r], meshes[m], r, seed, t_final)
+        ctx.state_set(k, e["X0"])
+        ctx.channels_set(k, e["ryr_state"], e["lcc_state"], e["lccv_state"], e["R"])
+        ctx.globals_set(k, *e["gv"])
         if log and (k + 1) % 256 == 0:
             log(f"  initialised {k + 1} realizations")
     ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state", "occupancy", "lds") if k in tune})
@@ -331,3 +356,66 @@ def kuhn_block_order(g: Grid, block: int = 64) -> np.ndarray:
     perm = np.empty(n, np.int32)
     perm[pos] = np.arange(n, dtype=np.int32)
     return perm
+
+
+def spark_adjoint_problem(t_sim: float = 5.0, n_points: int = 64, n_samples: int = 50, ckpt: int = 256,
+                          seed: int = 7, perturb: float = 0.1):
+    """The fitting workload on the h = 40 nm spark box: t_sim ms with the RyR
+    trajectory frozen from one gated run (LCC states held), a line scan of
+    n_points along x through the cluster centre sampled n_samples times, and a
+    target made by the same frozen-channel run with every parameter of theta
+    moved by +-perturb (signs drawn from `seed`).  Returns the
+    LinescanProblem (its context holds the realization and the scan) and
+    a dict with the sizes and the loss at the generating parameters."""
+    from . import R, adjoint_theta, dff0, read_scan_parameters
+    from . import Context
+    from .fit import LinescanProblem
+
+    pr = prepare_spark()
+    g = pr.grid
+    p = pr.params.copy()
+    dt = float(p.reals[R["T_STEP"]])
+    n_steps = int(round(t_sim / dt))
+    ctx = Context(0)
+    mid = ctx.add_mesh_grid(g, pr.aij, pr.nij, perm=kuhn_block_order(g))
+    ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=38376427, stream=0)
+    ctx.finalize()
+    rs, ls, lvs, Rr = initial_channels(pr)
+    ctx.state_set(0, pr.X0)
+    ctx.channels_set(0, rs, ls, lvs, Rr)
+    ctx.globals_set(0, 0.0, p.reals[R["DEFAULT_CA_NSR"]])
+    schedule = ctx.record_schedule(0, n_steps)
+    ctx.channels_set(0, schedule[0], ls, lvs, Rr)
+    cen = g.centroids()
+    c0 = np.array([0.5, 0.5, 0.3])  # the linescan program assumes a centred mesh
+    d = read_scan_parameters(None)
+    d.axis = 0
+    rc = cen[g.ryr_ele].mean(axis=0) - c0
+    d.offset1, d.offset2 = float(rc[1]), float(rc[2])
+    d.x_min, d.x_max = -0.48, 0.48
+    d.res_x_f = (d.x_max - d.x_min) / (n_points + 0.5)  # the point count truncates (x_max - x_min) / res
+    d.res_x_ca = 0.01
+    d.domain[0], d.domain[1], d.domain[2] = 1.0, 1.0, 0.6
+    d.pad_min, d.pad_max, d.pad_res = -1.5, 1.5, 0.1
+    sid, nf, _ = ctx.add_scan(mid, cen - c0, d)
+    samples = np.linspace(n_steps // n_samples, n_steps, n_samples).astype(np.int64)
+    idx = adjoint_theta()
+    sgn = np.where(np.random.default_rng(seed).random(len(idx)) < 0.5, -1.0, 1.0)
+    Pt = p.reals.copy()
+    Pt[idx] *= 1 + perturb * sgn
+    # target: dF/F0 of the same frozen-channel run at the moved parameters
+    ctx.params_set(0, Pt)
+    ctx.state_set(0, pr.X0)
+    tgt, prev = [], 0
+    for smp in samples:
+        ctx.replay(0, schedule[prev:smp])
+        prev = int(smp)
+        tgt.append(dff0(ctx.scan_sample(0, sid)[2], d.f0))
+    tgt = np.array(tgt)
+    ctx.state_set(0, pr.X0)
+    ctx.channels_set(0, schedule[0], ls, lvs, Rr)
+    L0 = ctx.loss_grad(0, sid, schedule, samples, tgt, ckpt, grad=False)
+    ctx.params_set(0, p.reals)
+    prob = LinescanProblem(ctx, sid, schedule, samples, tgt, p, pr.X0, ckpt=ckpt)
+    return prob, dict(n_steps=n_steps, n_ele=g.n_ele, n_f=nf, theta_true=Pt[idx], loss_at_target=L0,
+                      n_openings=int(schedule.sum()))

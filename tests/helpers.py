@@ -73,11 +73,13 @@ def output_due(t, dt_last, iv):
 
 
 def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, max_steps=10 ** 7, rng_kind=RNG_PHILOX,
-                t_lcc_close=None):
+                t_lcc_close=None, gpu_layout=False, on_top=None):
     """RunSimulation's main loop with the oracle's step (Philox gating).
     Returns the records written inside and after the loop (the t = 0 record
     before the loop is the caller's).  t_lcc_close: T_LCC_Close of an LCC
-    triggered at t = 0 (cpu/RunSimulation.cpp:459-463, 525-528)."""
+    triggered at t = 0 (cpu/RunSimulation.cpp:459-463, 525-528).  on_top(t,
+    dt_last, F): called at the top of every main-loop iteration after the
+    protocols, where the reference writes its grid files (:545-550)."""
     G = grid_dict(pr.grid, pr.aij, pr.nij)
     P = p.reals
     Iv = p.ints
@@ -101,7 +103,7 @@ def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, ma
             n_lcc = int(np.sum((F.lccv_state == 2) & ((F.lcc_state == 6) | (F.lcc_state == 12))))
             stop = n_open > 0 or n_lcc > 0 or t > P[S.R["T_FINAL"]]
             if not stop and output_due(t, dt_last, P[S.R["OUTPUT_STATES_INTERVAL"]]):
-                recs.append(O.record(t, G, F, P, ryr_out=ryr_out))
+                recs.append(O.record(t, G, F, P, ryr_out=ryr_out, gpu_layout=gpu_layout))
             if Iv[S.I["FLAG_V_CLAMP"]]:
                 F.gv[0] = P[S.R["V_CLAMP"]] if t <= P[S.R["T_CLAMP"]] else P[S.R["DEFAULT_VM"]]
             if stop:
@@ -117,6 +119,8 @@ def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, ma
             F.gv[0] = P[S.R["V_CLAMP"]] if t <= P[S.R["T_CLAMP"]] else P[S.R["DEFAULT_VM"]]
         if ir != -1 and t < P[S.R["INIT_RYR_HOLD"]] and not F.ryr_state[ir]:
             F.ryr_state[ir] = 1
+        if on_top is not None:
+            on_top(t, dt_last, F)
         if output_due(t, dt_last, P[S.R["OUTPUT_STATES_INTERVAL"]]):
             n_open = int(F.ryr_state.sum())
             Ca = F.X[: G["n_ele"]]
@@ -124,7 +128,7 @@ def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, ma
             for e in G["ryr_ele"]:
                 if Ca[e] > ca_max:
                     ca_max = Ca[e]
-            recs.append(O.record(t, G, F, P, ryr_out=ryr_out))
+            recs.append(O.record(t, G, F, P, ryr_out=ryr_out, gpu_layout=gpu_layout))
         O.steps(G, P, F, 1, dt, gating=not Iv[S.I["FLAG_NO_GATING"]], rng_kind=rng_kind, key=key, stream=stream,
                 step0=step)
         step += 1
@@ -132,5 +136,5 @@ def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, ma
         dt_last = dt
         if step > max_steps:
             raise RuntimeError("oracle loop did not terminate")
-    recs.append(O.record(t, G, F, P, ryr_out=ryr_out))
+    recs.append(O.record(t, G, F, P, ryr_out=ryr_out, gpu_layout=gpu_layout))
     return recs, t, step

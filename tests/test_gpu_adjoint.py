@@ -8,6 +8,7 @@ import pytest
 import srs_amd as S
 from srs_amd import workloads as W
 from srs_amd.fit import LinescanProblem, linescan_loss_fn
+from tests.helpers import rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -437,4 +438,116 @@ def test_batch_fit_reduces_the_mean_loss():
     np.testing.assert_array_equal(t.grad.numpy(), g)
     th = fit(prob, steps=6, lr=0.02)
     assert prob.loss(th) < L
+    ctx.close()
+
+
+@pytest.mark.timeout(900)
+def test_spark_box_fitting_workload_gradient_matches_central_differences():
+    """The fitting workload at its own size: the 56,250-element spark box,
+    5 ms (83,519 steps) with the RyR trajectory frozen from a gated run, a
+    64-point line scan x 50 samples, checkpoints every 256 steps, all 13
+    parameters: adjoint against central differences of the same device
+    forward run, |theta g - theta g_fd| <= 1e-5 relative (components three
+    orders below the largest held to 1e-5 of the largest); the loss vanishes
+    at the generating parameters."""
+    print("fitting workload: setup (schedule, target)", flush=True)
+    prob, info = W.spark_adjoint_problem()
+    assert info["n_steps"] == 83519 and info["n_f"] == 64 and info["n_openings"] > 0
+    assert info["loss_at_target"] == 0.0
+    th = prob.theta()
+    L0, g = prob.loss_grad(th)
+    assert L0 > 0
+    fd = np.zeros_like(g)
+    eps = 1e-5
+    for k in range(len(th)):
+        tp, tm = th.copy(), th.copy()
+        tp[k] *= 1 + eps
+        tm[k] *= 1 - eps
+        fd[k] = (prob.loss(tp) - prob.loss(tm)) / (2 * eps * th[k])
+        print(f"  theta {k}: adjoint {th[k] * g[k]:.6e}, central difference {th[k] * fd[k]:.6e}", flush=True)
+    prob.ctx.close()
+    sens, sens_fd = th * g, th * fd
+    scale = np.abs(sens_fd).max()
+    assert scale > 0
+    err = np.abs(sens - sens_fd) / np.maximum(np.abs(sens_fd), 1e-3 * scale)
+    assert err.max() <= 1e-5, (err, sens, sens_fd)
+
+
+def _bound_ctx(g, aij, nij, p, bnd, sched, R):
+    """A one-realization context whose mesh carries boundary rates built at
+    p's D (cpu/InitFuncs.cpp:1012-1026) from the boundary values bnd."""
+    br = S.boundary_rates(g, p.reals, bnd)
+    assert np.any(br != 0)
+    ctx = S.Context(0)
+    mid = ctx.add_mesh_grid(g, aij, nij, br=br, perm=W.kuhn_block_order(g))
+    ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=5, stream=0)
+    ctx.finalize()
+    ctx.channels_set(0, sched[0], np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R)
+    ctx.globals_set(0, 0.0, p.reals[S.R["DEFAULT_CA_NSR"]])
+    cen, d = _scan_desc(g)
+    sid, nf, _ = ctx.add_scan(mid, cen, d)
+    return ctx, sid, nf
+
+
+def test_boundary_rates_follow_d_in_step_and_gradient():
+    """Bound faces (flag_noflux off) with boundary values above the initial
+    state: the boundary rates D Face_C b / V0 move with D.  A run whose D_Ca
+    and D_Fluo are changed by srs_params_set equals the oracle run with the
+    rates rebuilt at the new D (rel <= 1e-12: the planes are rescaled, not
+    rebuilt), and so does its loss; the adjoint's D gradients match central
+    differences of losses from contexts whose rates are rebuilt at each
+    perturbed D."""
+    from oracle.oracle import Fields, Oracle, grid_dict
+
+    spec = W.tiny_box_spec(nx=8, ny=8, nz=5, tt=True, ryr=2, lcc=1)
+    g = S.box_grid(spec, no_flux=False)
+    aij, nij = S.assemble_operator(g)
+    p = W.spark_params()
+    dt = W.stable_dt(aij, 0.25)
+    p.setr(T_STEP=dt, INIT_RYR_HOLD=dt, V_RYR=20 * p.reals[S.R["V_RYR"]])
+    X0, bnd = S.equilibrium_state(g, p.reals)
+    bnd = bnd * np.array([3.0, 1.0, 1.0, 1.0, 1.0, 1.2])  # Ca and Fluo flow in through the bound faces
+    n = 400
+    sched = np.zeros((n, g.n_ryr), np.int32)
+    sched[: n // 2, 0] = 1
+    R = np.log(np.random.default_rng(3).random(g.n_ryr + g.n_lcc))
+    samples = np.linspace(n // 8, n, 8).astype(np.int64)
+    # the step: D moved after the mesh was added, against the oracle with the rates rebuilt
+    p1 = p.copy().setr(D_CA=0.3, D_DYE=0.05)
+    ctx, sid, nf = _bound_ctx(g, aij, nij, p, bnd, sched, R)
+    ctx.params_set(0, p1.reals)
+    ctx.state_set(0, X0)
+    ctx.replay(0, sched)
+    Xd = ctx.state_get(0)
+    F = Fields(X0, sched[0], np.ones(g.n_lcc), np.full(g.n_lcc, 2), R, [0.0, p.reals[S.R["DEFAULT_CA_NSR"]]])
+    Fo = Oracle().steps(grid_dict(g, aij, nij, S.boundary_rates(g, p1.reals, bnd)), p1.reals, F, n, dt, gating=False,
+                        schedule=sched)
+    assert rel_err(Xd, Fo.X) <= 1e-12
+    assert rel_err(Xd, X0) > 1e-3
+    # the loss (target zero: the squared dF/F0) and its D gradients
+    tgt = np.zeros((len(samples), nf))
+
+    def loss(c, s, P):
+        c.params_set(0, P)
+        c.state_set(0, X0)
+        return c.loss_grad(0, s, sched, samples, tgt, 64, grad=False)
+
+    ctx.params_set(0, p.reals)
+    ctx.state_set(0, X0)
+    L, gr = ctx.loss_grad(0, sid, sched, samples, tgt, 64)
+    L1 = loss(ctx, sid, p1.reals)
+    c1, s1, _ = _bound_ctx(g, aij, nij, p1, bnd, sched, R)
+    assert abs(loss(c1, s1, p1.reals) - L1) <= 1e-12 * L1
+    c1.close()
+    eps = 1e-5
+    for k, name in ((0, "D_CA"), (1, "D_DYE")):
+        th = p.reals[S.R[name]]
+        Ls = []
+        for sgn in (1, -1):
+            q = p.copy().setr(**{name: th * (1 + sgn * eps)})
+            c2, s2, _ = _bound_ctx(g, aij, nij, q, bnd, sched, R)
+            Ls.append(loss(c2, s2, q.reals))
+            c2.close()
+        fd = (Ls[0] - Ls[1]) / (2 * eps * th)
+        assert abs(gr[k] - fd) <= 1e-5 * abs(fd), (name, gr[k], fd)
     ctx.close()

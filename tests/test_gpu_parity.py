@@ -14,7 +14,7 @@ import pytest
 import srs_amd as S
 from oracle.oracle import PHILOX_KEY1, RNG_GLIBC, RNG_PHILOX, Fields, Oracle, grid_dict
 from srs_amd import workloads as W
-from tests.helpers import fast_gating, gpu_context, initial_fields, oracle_loop, rel_err
+from tests.helpers import fast_gating, gpu_context, initial_fields, oracle_loop, output_due, rel_err
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
@@ -435,6 +435,148 @@ def test_cru3d_run_matches_oracle_loop(tmp_path, ffwd):
     assert nsteps > 100
 
 
+
+def test_cru3d_random_initial_ryr_and_gpu_layout(tmp_path):
+    """ryr_open_init = -1 without a clamp or LCC trigger: the driver opens a
+    rate-weighted random RyR (OpenRandomRyR, cpu/InitFuncs.cpp:1420-1475,
+    linear phi) with its Philox draw, and writes the states file in the GPU
+    program's layout (Ca_cyt at each RyR after the seven fields,
+    gpu/OutputFuncs.cpp:277-284; the -layout gpu option of cru3d_amd).  The
+    oracle's OpenRandomRyR with the same uniform picks the same RyR, and the
+    records equal the oracle loop's in that layout."""
+    import ctypes as C
+    import subprocess
+    from types import SimpleNamespace
+
+    base = str(tmp_path / "mesh")
+    S.box_mesh_write(W.tiny_box_spec(nx=6, ny=6, tt=True, ryr=3), base)
+    g, _ = S.grid_read(base)
+    aij, nij = S.assemble_operator(g)
+    dt = W.stable_dt(aij, 0.25)
+    entries = dict(t_step=repr(dt * 1e6), t_final=repr(250 * dt), states_interval=repr(9 * dt),
+                   RYR_K_PLUS="0.5", RYR_K_MINUS="20", flag_use_clock="false", seed_base="777",
+                   flag_output_ryr_states="true", sims_per_proc="2", paramset_title="tiny", protocol="none",
+                   ryr_open_init="-1")
+    xml = str(tmp_path / "p.xml")
+    _write_params(xml, entries)
+    out = str(tmp_path / "out")
+    S.run_cru3d(xml, base, out, gpu_layout=True)
+    out2 = str(tmp_path / "out2")
+    exe = os.path.join(os.path.dirname(S.__file__), "cru3d_amd")
+    subprocess.run([exe, "-param", xml, "-mesh", base, "-out", out2, "-layout", "gpu"], check=True,
+                   stdout=subprocess.DEVNULL)
+    p = S.read_parameters(xml)
+    P = p.reals
+    O = Oracle()
+    nr, nl = g.n_ryr, g.n_lcc
+    X0, _ = S.equilibrium_state(g, P)
+    G = grid_dict(g, aij, nij)
+    pr = SimpleNamespace(grid=g, aij=aij, nij=nij)
+    picked = []
+    for run in range(2):
+        recs_d = np.fromfile(os.path.join(out, f"states_0_{run}.out"), np.float32).reshape(-1, 7 + 2 * nr)
+        recs_e = np.fromfile(os.path.join(out2, f"states_0_{run}.out"), np.float32)
+        np.testing.assert_array_equal(recs_d.ravel(), recs_e)
+        seed = (93843 + 777 + 83 * run) & 0xFFFFFFFF  # seed_B's default is 83
+        rs = np.zeros(nr, np.int32)
+        F = Fields(X0.copy(), rs, np.ones(nl, np.int32), np.full(nl, 2, np.int32), np.zeros(nr + nl),
+                   [P[S.R["DEFAULT_VM"]], P[S.R["DEFAULT_CA_NSR"]]])
+        u = O.open01(O.philox([0, 0, run, 3], [seed, PHILOX_KEY1])[0])
+        ch = O._chan(G, F)
+        k = O.L.orc_open_random_ryr(F.X.ctypes.data_as(C.POINTER(C.c_double)), g.n_ele,
+                                    P.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ch), u)
+        assert F.ryr_state.sum() == 1 and F.ryr_state[k] == 1
+        picked.append(k)
+        F.R[:] = [np.log(O.open01(O.philox([0, i, run, 1], [seed, PHILOX_KEY1])[0])) for i in range(nr + nl)]
+        rec0 = O.record(0.0, G, F, P, ryr_out=True, gpu_layout=True)
+        recs_o, _, nsteps = oracle_loop(O, pr, p, F, key=seed, stream=run, gpu_layout=True)
+        exp = np.array([rec0] + recs_o)
+        assert recs_d.shape == exp.shape and nsteps > 100
+        np.testing.assert_array_equal(recs_d, exp)
+        np.testing.assert_array_equal(recs_d[0, 7:7 + nr], X0[g.ryr_ele].astype(np.float32))
+    assert recs_d[0, 7 + nr:].sum() == 1  # one RyR open at t = 0
+
+def _vtk_text(g, X, ryr_state, run, rank, n, t):
+    """WriteToGridFiles restated (cpu/OutputFuncs.cpp:42-220): legacy-ASCII
+    VTK, points and cells, then cell data C_CA, C_DYE (printf %f), B_RYR (0
+    none, 1 closed, 2 open), B_JSR and B_LCC."""
+    N = g.n_ele
+    xyz = g.node_xyz.reshape(-1, 3)
+    en = g.ele_nodes.reshape(-1, 4)
+    out = ["# vtk DataFile Version 3.0\n", "Grid data for run %d-%d-%d at t = %f\n" % (run, rank, n, t), "ASCII\n\n",
+           "DATASET UNSTRUCTURED_GRID\n", "POINTS %d float\n" % len(xyz)]
+    out += ["%f %f %f\n" % tuple(p) for p in xyz]
+    out.append("\nCELLS %d %d\n" % (N, 5 * N))
+    out += ["4 %d %d %d %d\n" % tuple(e) for e in en]
+    out.append("\nCELL_TYPES %d\n" % N)
+    out += ["10\n"] * N
+    out.append("\nCELL_DATA %d\n" % N)
+    out.append("SCALARS C_CA float\nLOOKUP_TABLE default\n")
+    out += ["%f\n" % v for v in X[:N]]
+    out.append("SCALARS C_DYE float\nLOOKUP_TABLE default\n")
+    out += ["%f\n" % v for v in X[5 * N:6 * N]]
+    b = np.zeros(N, int)
+    for i, e in enumerate(g.ryr_ele):
+        b[e] = 2 if ryr_state[i] else 1
+    out.append("SCALARS B_RYR int\nLOOKUP_TABLE default\n")
+    out += ["%d\n" % v for v in b]
+    out.append("SCALARS B_JSR int\nLOOKUP_TABLE default\n")
+    out += ["%d\n" % (1 if d == 1 else 0) for d in g.domain]
+    b = np.zeros(N, int)
+    b[g.lcc_ele] = 1
+    out.append("SCALARS B_LCC int\nLOOKUP_TABLE default\n")
+    out += ["%d\n" % v for v in b]
+    return "".join(out)
+
+
+def test_cru3d_grid_files_match_restated_writer(tmp_path):
+    """flag_output_grid: the driver writes grid #0 before the loop, one at
+    every floor crossing of grid_interval (at the top of the loop, after the
+    protocols, cpu/RunSimulation.cpp:545-550) and the final one; each file
+    equals the restated writer applied to the oracle loop's state at that
+    time, byte for byte."""
+    from types import SimpleNamespace
+
+    base = str(tmp_path / "mesh")
+    S.box_mesh_write(W.tiny_box_spec(tt=True), base)
+    g, _ = S.grid_read(base)
+    aij, nij = S.assemble_operator(g)
+    dt = W.stable_dt(aij, 0.25)
+    entries = dict(t_step=repr(dt * 1e6), t_final=repr(130 * dt), states_interval=repr(9 * dt),
+                   grid_interval=repr(41 * dt), flag_output_grid="true", RYR_K_PLUS="0.5", RYR_K_MINUS="20",
+                   flag_use_clock="false", seed_base="55", sims_per_proc="1", paramset_title="tiny",
+                   protocol="none", ryr_open_init="0", ryr_open_time=repr(30 * dt))
+    xml = str(tmp_path / "p.xml")
+    _write_params(xml, entries)
+    out = str(tmp_path / "out")
+    S.run_cru3d(xml, base, out)
+    p = S.read_parameters(xml)
+    P = p.reals
+    O = Oracle()
+    nr, nl = g.n_ryr, g.n_lcc
+    seed = (93843 + 55) & 0xFFFFFFFF
+    rs = np.zeros(nr, np.int32)
+    rs[0] = 1
+    R = np.array([np.log(O.open01(O.philox([0, i, 0, 1], [seed, PHILOX_KEY1])[0])) for i in range(nr + nl)])
+    X0, _ = S.equilibrium_state(g, P)
+    F = Fields(X0, rs, np.ones(nl, np.int32), np.full(nl, 2, np.int32), R, [P[S.R["DEFAULT_VM"]],
+                                                                        P[S.R["DEFAULT_CA_NSR"]]])
+    want = [_vtk_text(g, F.X, F.ryr_state, 0, 0, 0, 0.0)]
+    giv = P[S.R["OUTPUT_GRID_INTERVAL"]]
+
+    def on_top(t, dt_last, Fc):
+        if output_due(t, dt_last, giv):
+            want.append(_vtk_text(g, Fc.X, Fc.ryr_state, 0, 0, len(want), t))
+
+    pr = SimpleNamespace(grid=g, aij=aij, nij=nij)
+    _, t_end, nsteps = oracle_loop(O, pr, p, F, key=seed, stream=0, on_top=on_top)
+    want.append(_vtk_text(g, F.X, F.ryr_state, 0, 0, len(want), t_end))
+    assert len(want) == 5 and nsteps > 120
+    for n, text in enumerate(want):
+        with open(os.path.join(out, f"grid_0_0_{n}.vtk")) as f:
+            got = f.read()
+        assert got == text, n
+    assert not os.path.exists(os.path.join(out, f"grid_0_0_{len(want)}.vtk"))
 
 def test_cru3d_ensemble_runs_match_oracle_loops(tmp_path):
     """sims_per_proc > 1: the driver runs the ensemble's realizations together

@@ -353,6 +353,54 @@ def test_boundary_rates_and_equilibrium(oracle):
     assert np.count_nonzero(br) > 0
 
 
+
+@pytest.mark.parametrize("dims", [(4, 3, 3), (5, 4, 2)])
+def test_boundary_rates_balance_the_bound_face_sink(oracle, dims):
+    """What the mathematics fixes about the boundary rates, independent of
+    their formula: a bound face (cpu/InitFuncs.cpp:1019-1024, 1068-1112) is a
+    Dirichlet contact with the boundary value b, so a field that equals b
+    everywhere has zero diffusive rate, D (A c)_i + BR_i = 0, in every element
+    and every diffusing species (the operator's diagonal sink -D Face_C / V0 c
+    cancels the source D Face_C b / V0); and the rate of a uniform field c is
+    BR (1 - c / b), linear in c and zero in elements without bound faces.
+    Checked for the oracle's boundary rates and the product's, through the
+    oracle's own step with every reaction switched off."""
+    nx, ny, nz = dims
+    g = S.box_grid(W.tiny_box_spec(nx=nx, ny=ny, nz=nz, jsr=False, ryr=0, lcc=0, sl=False), no_flux=False)
+    P = S.read_parameters(None).reals.copy()
+    P[S.R["D_CA"]], P[S.R["D_DYE"]], P[S.R["D_ATP"]], P[S.R["D_CMDN"]] = 0.22, 0.042, 0.14, 0.025
+    for k in ("K_ON_TRPN", "K_ON_DYE", "K_ON_ATP", "K_ON_CMDN", "K_OFF_TRPN", "K_OFF_DYE", "K_OFF_ATP",
+              "K_OFF_CMDN", "A_P", "V_REFILL"):
+        P[S.R[k]] = 0.0
+    n = g.n_ele
+    aij, nij = S.assemble_operator(g)
+    b = np.array([0.7, 310.0, 11.0, 0.0, 0.0, 3.3])  # boundary values of Ca, ATP, CaM, -, -, Fluo
+    diff = np.array([P[S.R[k]] for k in ("D_CA", "D_ATP", "D_CMDN", "D_TRPN", "D_CSQN", "D_DYE")])
+    br_o = oracle.boundary_rates(n, g.ele1, g.face_type, g.face_c, g.V0, diff, b)
+    br_p = S.boundary_rates(g, P, b)
+    bound = np.zeros(n, bool)
+    bound[g.ele1[g.face_type == 1]] = True
+    assert bound.any() and not bound.all()
+    dt = 1e-7
+    for br in (br_o, br_p):
+        G = grid_dict(g, aij, nij, br)
+        G["ryr_ele"] = G["ryr_jsr"] = G["lcc_ele"] = np.zeros(0, np.int32)
+        G["ryr_nb"] = np.zeros(0, np.int32)
+        for scale in (1.0, 0.25):
+            X = np.repeat(b * scale, n)
+            F = Fields(X, np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0),
+                       [0.0, 0.0])
+            oracle.steps(G, P, F, 1, dt, gating=False)
+            rate = (F.X - X) / dt
+            for s in (0, 1, 2, 5):
+                r = rate[s * n:(s + 1) * n]
+                B = br[s * n:(s + 1) * n]
+                assert np.all(B[~bound] == 0) and np.all(B[bound] > 0), s
+                # zero at c = b, BR (1 - c/b) below it: to round-off of the diagonal sum
+                tol = 1e-9 * np.abs(B).max()
+                np.testing.assert_allclose(r, B * (1 - scale), rtol=0, atol=tol)
+
+
 # ----------------------------------------------------------- states records
 def test_states_record_hand_case():
     """Two RyRs (one open) and two LCCs (one open) with chosen concentrations:
