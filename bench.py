@@ -270,57 +270,15 @@ def bench_adjoint_batch(n_real=256, n_steps=256, n_points=64, n_samples=8, max_b
     by one gated run of all of them, frozen for the loss; a 64-point line
     scan along x through the cluster centre sampled n_samples times; targets
     from theta moved by +-10%.  Times the batched loss alone and the loss with
-    every member's gradient (checkpoint interval ceil(sqrt(n_steps)), at most
-    max_batch members stepped at once)."""
-    import srs_amd as S
+    every member's gradient (the library's checkpoint plan, at most max_batch
+    members stepped at once)."""
     from srs_amd import workloads as W
 
     t0 = time.perf_counter()
-    pr = W.prepare_dyad(h=0.03)
-    g = pr.grid
-    p = pr.params.copy()
-    ctx = S.Context(0)
-    mid = ctx.add_mesh_grid(g, pr.aij, pr.nij, perm=W.kuhn_block_order(g))
-    for r in range(n_real):
-        ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=38376427, stream=r)
-    ctx.finalize()
-    for r in range(n_real):
-        rs, ls, lvs, R = W.initial_channels(pr, seed=r)
-        ctx.state_set(r, pr.X0)
-        ctx.channels_set(r, rs, ls, lvs, R)
-        ctx.globals_set(r, 0.0, p.reals[S.R["DEFAULT_CA_NSR"]])
+    ctx, sid, scheds, samples, tgt, p, info = W.dyad_adjoint_batch(n_real, n_steps, n_points, n_samples, max_batch,
+                                                                   log=log)
     t_setup = time.perf_counter() - t0
-    log(f"adjoint batch: {n_real} realizations of {g.n_ele} elements set up in {t_setup:.1f} s")
-    t0 = time.perf_counter()
-    scheds = ctx.record_schedules(n_steps)
-    t_rec = time.perf_counter() - t0
-    for r in range(n_real):
-        rs, ls, lvs, R = W.initial_channels(pr, seed=r)
-        ctx.channels_set(r, scheds[r][0], ls, lvs, R)
-    cen = g.centroids()
-    c0 = np.array([1.0, 1.0, 0.5])
-    d = S.read_scan_parameters(None)
-    d.axis = 0
-    rc = cen[g.ryr_ele].mean(axis=0) - c0
-    d.offset1, d.offset2 = float(rc[1]), float(rc[2])
-    d.x_min, d.x_max = -0.96, 0.96
-    d.res_x_f = (d.x_max - d.x_min) / (n_points + 0.5)  # the point count truncates (x_max - x_min) / res
-    d.res_x_ca = 0.01
-    d.domain[0], d.domain[1], d.domain[2] = 2.0, 2.0, 1.0
-    d.pad_min, d.pad_max, d.pad_res = -2.5, 2.5, 0.1
-    sid, nf, _ = ctx.add_scan(mid, cen - c0, d)
-    samples = np.linspace(n_steps // n_samples, n_steps, n_samples).astype(np.int64)
-    idx = S.adjoint_theta()
-    sgn = np.where(np.random.default_rng(7).random(len(idx)) < 0.5, -1.0, 1.0)
-    Pt = p.reals.copy()
-    Pt[idx] *= 1 + 0.1 * sgn
-    for r in range(n_real):
-        ctx.params_set(r, Pt)
-    zero = np.zeros((n_real, n_samples, nf))
-    _, tgt = ctx.loss_grad_batch(range(n_real), sid, scheds, samples, zero, grad=False, scans=True,
-                                 max_batch=max_batch)
-    for r in range(n_real):
-        ctx.params_set(r, p.reals)
+    idx = __import__("srs_amd").adjoint_theta()
     t0 = time.perf_counter()
     Lf = ctx.loss_grad_batch(range(n_real), sid, scheds, samples, tgt, grad=False, max_batch=max_batch)
     t_fwd = time.perf_counter() - t0
@@ -328,18 +286,17 @@ def bench_adjoint_batch(n_real=256, n_steps=256, n_points=64, n_samples=8, max_b
     t0 = time.perf_counter()
     L, grad = ctx.loss_grad_batch(range(n_real), sid, scheds, samples, tgt, max_batch=max_batch)
     t_grad = time.perf_counter() - t0
-    log(f"adjoint batch: loss and gradients in {t_grad:.2f} s")
+    plan = ctx.adjoint_plan()
+    log(f"adjoint batch: loss and gradients in {t_grad:.2f} s, plan {plan}")
     ctx.close()
-    ckpt = int(np.ceil(np.sqrt(n_steps)))
-    elem_steps = float(n_real) * g.n_ele * n_steps
-    dt = float(p.reals[S.R["T_STEP"]])
-    full_steps = int(round(20.0 / dt))
-    return dict(workload=f"dyad_box geometry at h=30 nm ({g.n_ele} elements), {n_real} realizations with their own "
-                         f"recorded RyR trajectories, {n_steps} steps, {nf}-point line scan x {n_samples} samples, "
-                         f"checkpoint every {ckpt} steps, at most {max_batch} realizations at once, "
-                         f"{len(idx)} parameters",
-                n_real=n_real, n_elements=g.n_ele, n_steps=n_steps, forward_s=t_fwd, loss_and_gradient_s=t_grad,
-                schedule_record_s=t_rec, setup_s=t_setup,
+    n_ele = info["n_ele"]
+    elem_steps = float(n_real) * n_ele * n_steps
+    full_steps = int(round(20.0 / info["dt"]))
+    return dict(workload=f"dyad_box geometry at h=30 nm ({n_ele} elements), {n_real} realizations with their own "
+                         f"recorded RyR trajectories, {n_steps} steps, {info['n_f']}-point line scan x {n_samples} "
+                         f"samples, at most {max_batch} realizations at once, {len(idx)} parameters",
+                n_real=n_real, n_elements=n_ele, n_steps=n_steps, forward_s=t_fwd, loss_and_gradient_s=t_grad,
+                setup_s=t_setup, checkpoint_plan=plan,
                 ps_per_element_step_forward=1e12 * t_fwd / elem_steps,
                 ps_per_element_step_loss_and_gradient=1e12 * t_grad / elem_steps,
                 full_20ms_steps=full_steps,

@@ -277,6 +277,9 @@ int srs_ctx_create(int device, srs_ctx** out);
                                     step's gating (and, with SRS_STEP_PROTOCOL, the loop body) inside the
                                     next step's sweep launch (register budget 0 or 6; 0 with the loop
                                     body); 0: a gating launch after every sweep */
+#define SRS_TUNE_ADJ_LEVELS 11   /* adjoint checkpoint levels (srs_linescan_loss_grad*): 0 (default) the
+                                    fewest whose memory fits the sub-batch, 1..6 forced; each level below the
+                                    first is one more forward pass and needs about (L+1) n^(1/(L+1)) states */
 int srs_ctx_tune(srs_ctx* ctx, int32_t key, int32_t value);
 
 /* Replaces FreeDomainStruct / FreeGlobalStruct (gpu/RunSimulation.cu:874-901). */
@@ -494,10 +497,16 @@ int srs_linescan_loss_grad(srs_ctx* ctx, int32_t real, int32_t scan_id, int64_t 
  *   schedules  the members' n_steps x n_ryr(member) schedules, concatenated in
  *              real_ids order;
  *   targets    [n_real][n_samples][n_f];
- *   ckpt       checkpoint interval, 0 = ceil(sqrt(n_steps));
+ *   ckpt       length of a recomputed segment (every state of which is
+ *              kept), 0 = chosen with the checkpoint levels: n_steps^(1/(L+1))
+ *              for L levels (ceil(sqrt(n_steps)) for one level);
  *   max_batch  most members stepped at once, 0 = as many as device memory
- *              holds (checkpoints, one segment of states and the adjoint take
- *              about 48 B x elements x (n_steps/ckpt + ckpt + 2) each);
+ *              holds.  A member takes about 48 B x elements x (states kept + 2):
+ *              with one level n_steps/ckpt + ckpt, with L levels about
+ *              (L+1) n_steps^(1/(L+1)).  Unless SRS_TUNE_ADJ_LEVELS forces it,
+ *              the call takes the fewest levels with which max_batch members
+ *              (one, if max_batch is 0) fit; the plan changes memory and time,
+ *              never a bit of the result;
  *   losses     [n_real];  grads [n_real][SRS_N_THETA] or NULL;
  *   scans      NULL or [n_real][n_samples][n_f]: the simulated dF/F0.
  * Results for a member equal srs_linescan_loss_grad on that member alone, bit
@@ -506,6 +515,12 @@ int srs_linescan_loss_grad_batch(srs_ctx* ctx, int32_t n_real, const int32_t* re
                                  int64_t n_steps, const int32_t* schedules, int32_t n_samples,
                                  const int64_t* sample_steps, const double* targets, int32_t ckpt, int32_t max_batch,
                                  double* losses, double* grads, double* scans);
+
+/* The checkpoint plan of the context's last srs_linescan_loss_grad* call:
+ * levels, the strides of its levels (strides[levels-1] is the recomputed
+ * segment's length; 6 entries, unused ones 0) and the members stepped per
+ * sub-batch.  No reference counterpart. */
+int srs_adjoint_last_plan(srs_ctx* ctx, int32_t* levels, int64_t* strides, int32_t* sub_batch);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,7 @@ TUNE_GROUP = 7
 TUNE_LDS = 8
 TUNE_DROP_FINISHED = 9
 TUNE_GATED_SWEEP = 10
+TUNE_ADJ_LEVELS = 11
 
 
 def _p(a, t):
@@ -343,7 +344,7 @@ class Context:
         results."""
         keys = dict(tile=TUNE_TILE, nt_geometry=TUNE_NT_GEOMETRY, nt_state=TUNE_NT_STATE, plane_pad=TUNE_PLANE_PAD,
                     compact=TUNE_COMPACT, occupancy=TUNE_OCCUPANCY, group=TUNE_GROUP, lds=TUNE_LDS,
-                    drop_finished=TUNE_DROP_FINISHED, gated_sweep=TUNE_GATED_SWEEP)
+                    drop_finished=TUNE_DROP_FINISHED, gated_sweep=TUNE_GATED_SWEEP, adj_levels=TUNE_ADJ_LEVELS)
         for k, v in kw.items():
             check(lib().srs_ctx_tune(self._h, keys[k], int(v)))
         return self
@@ -590,6 +591,14 @@ class Context:
                                                  _f64(sc) if scans else None))
         out = (losses,) + ((g,) if grad else ()) + ((sc,) if scans else ())
         return out if len(out) > 1 else losses
+
+    def adjoint_plan(self) -> dict:
+        """Checkpoint plan of the last loss/gradient call: levels, their
+        strides (the last is the recomputed segment's length), sub-batch."""
+        lv, sb = C.c_int32(), C.c_int32()
+        st = (C.c_int64 * 6)()
+        check(lib().srs_adjoint_last_plan(self._h, C.byref(lv), st, C.byref(sb)))
+        return dict(levels=lv.value, strides=[int(x) for x in st[:lv.value]], sub_batch=sb.value)
 
     def time_sweep(self, reps: int):
         a, b = C.c_double(), C.c_double()

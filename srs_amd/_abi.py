@@ -137,6 +137,7 @@ _PROTOS = {
     "srs_linescan_loss_grad_batch": (C.c_int, [C.c_void_p, C.c_int32, i32p, C.c_int32, C.c_int64, i32p, C.c_int32,
                                                C.POINTER(C.c_int64), f64p, C.c_int32, C.c_int32, f64p, f64p, f64p]),
     "srs_record_schedules": (C.c_int, [C.c_void_p, C.c_int64, i32p]),
+    "srs_adjoint_last_plan": (C.c_int, [C.c_void_p, i32p, C.POINTER(C.c_int64), i32p]),
     "srs_explicit_step_prepare": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32]),
     "srs_checkpoint_size": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_size_t)]),
     "srs_checkpoint_save": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),

@@ -3,6 +3,8 @@
 // batches).  No reference counterpart.
 #include "dev_common.h"
 
+#include <functional>
+
 namespace srs_dev {
 
 // Reverse mode of the explicit step with the channel trajectory frozen
@@ -459,6 +461,42 @@ int srs_record_schedules(srs_ctx* c, int64_t n_steps, int32_t* schedules) {
   return record_run(c, n_steps, schedules);
 }
 
+// Checkpoint plan of the adjoint: L levels of stored states with strides
+// K[0] > K[1] > ... > K[L-1], the last being the length of a recomputed
+// segment whose every state is kept.  The forward run keeps the state every
+// K[0] steps; going backward, each level-l segment is recomputed from its
+// checkpoint keeping the states every K[l+1] steps, down to the segments of
+// the last level.  A member holds slots[0] + ... + slots[L-1] + K[L-1]
+// states, about (L + 1) n^(1/(L+1)) for equal ratios, and every level below
+// the first costs one more forward pass.  Each recomputation replays the same
+// kernels from the same state, so the plan changes memory and time, never a
+// bit of the losses or gradients.
+constexpr int ADJ_MAX_LEVELS = 6;
+struct AdjPlan {
+  int L = 1;
+  int64_t K[ADJ_MAX_LEVELS] = {};
+  int64_t slots[ADJ_MAX_LEVELS] = {};
+  int64_t kept() const {
+    int64_t s = 0;
+    for (int l = 0; l < L; ++l) s += slots[l];
+    return s;
+  }
+};
+// leaf: the recomputed segment's length, 0 = (n_steps)^(1/(L+1)) rounded up.
+static AdjPlan adj_plan(int64_t n_steps, int64_t leaf, int L) {
+  AdjPlan p;
+  p.L = L;
+  const double n = (double)std::max<int64_t>(n_steps, 1);
+  if (leaf <= 0) leaf = (int64_t)std::ceil(std::pow(n, 1.0 / (L + 1)) - 1e-9);
+  leaf = std::max<int64_t>(1, std::min<int64_t>(leaf, std::max<int64_t>(n_steps, 1)));
+  p.K[L - 1] = leaf;
+  const int64_t r = L > 1 ? std::max<int64_t>(2, (int64_t)std::ceil(std::pow(n / leaf, 1.0 / L) - 1e-9)) : 0;
+  for (int l = L - 2; l >= 0; --l) p.K[l] = p.K[l + 1] * r;
+  p.slots[0] = (std::max<int64_t>(n_steps, 1) + p.K[0] - 1) / p.K[0];
+  for (int l = 1; l < L; ++l) p.slots[l] = (p.K[l - 1] + p.K[l] - 1) / p.K[l];
+  return p;
+}
+
 // Line-scan loss and adjoint gradient of the realizations `ids` (all on the
 // scan's mesh, group size 1; arguments already checked).  Members run in
 // sub-batches of at most `max_batch` that step together; every other
@@ -475,8 +513,6 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
   const int nf = S.n_f, nca = S.n_ca, nout = nca + 2 * nf;
   const int n_mem = (int)ids.size(), n_real = (int)c->reals.size();
   const bool want_grad = grads != nullptr;
-  const int64_t n_ck = (n_steps + ckpt - 1) / ckpt;
-  const int64_t n_ck_kept = want_grad ? n_ck : 1;  // a loss alone keeps only the start state
   const int nblk = (hm.n + ADJ_BS - 1) / ADJ_BS;
   std::vector<size_t> soff(n_mem + 1, 0);
   size_t max_sched = 0;
@@ -485,20 +521,44 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
     soff[m + 1] = soff[m] + k;
     max_sched = std::max(max_sched, k);
   }
-  // device bytes per member of a sub-batch
-  const size_t per = 8 * n6 * (size_t)(n_ck_kept + (want_grad ? ckpt + 2 : 0)) + (want_grad ? 64 * np : 0) +
-                     8 * (size_t)n_samples * nout + 8 * (size_t)S.n_blocks * nf + 8 * (size_t)nblk * ADJ_NT +
-                     8 * (size_t)n_samples * nf + 4 * max_sched + 64;
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
   size_t mem_free = 0, mem_total = 0;
   HIPCHK(hipMemGetInfo(&mem_free, &mem_total));
   const size_t budget = mem_free / 10 * 9;
+  // device bytes per member of a sub-batch under a plan (a loss alone keeps
+  // only the start state)
+  auto per_member = [&](const AdjPlan& p) {
+    return 8 * n6 * (size_t)(want_grad ? p.kept() + p.K[p.L - 1] + 2 : 1) + (want_grad ? 64 * np : 0) +
+           8 * (size_t)n_samples * nout + 8 * (size_t)S.n_blocks * nf + 8 * (size_t)nblk * ADJ_NT +
+           8 * (size_t)n_samples * nf + 4 * max_sched + 64;
+  };
+  // the plan: the tuned number of levels, or the fewest levels (least
+  // recomputation) with which the wanted sub-batch fits
+  const int want = max_batch > 0 ? std::min(max_batch, n_mem) : 1;
+  AdjPlan plan = adj_plan(n_steps, ckpt, std::max(1, c->adj_levels));
+  if (want_grad && c->adj_levels == 0) {
+    for (int L = 1; L <= ADJ_MAX_LEVELS; ++L) {
+      const AdjPlan p = adj_plan(n_steps, L == 1 ? ckpt : 0, L);
+      if (L == 1 || budget / per_member(p) > budget / per_member(plan)) plan = p;
+      if (budget / per_member(p) >= (size_t)want) {
+        plan = p;
+        break;
+      }
+    }
+  }
+  const size_t per = per_member(plan);
   int bs = max_batch > 0 ? std::min(max_batch, n_mem) : n_mem;
   bs = (int)std::min<size_t>((size_t)bs, budget / per);
   if (bs < 1)
-    return srs::fail(SRS_E_ARG, "checkpoint plan needs %.3g GB per realization, %.3g GB free; raise or lower the "
-                     "checkpoint interval (about sqrt(n_steps) needs least)", per / 1e9, mem_free / 1e9);
+    return srs::fail(SRS_E_ARG, "checkpoint plan (%d level(s), segments of %lld steps) needs %.3g GB per "
+                     "realization, %.3g GB free", plan.L, (long long)plan.K[plan.L - 1], per / 1e9, mem_free / 1e9);
+  const int64_t K0 = plan.K[0];
+  const int64_t n_ck_kept = want_grad ? plan.slots[0] : 1;
+  const int64_t leaf = plan.K[plan.L - 1];
+  c->adj_last_levels = plan.L;
+  for (int l = 0; l < ADJ_MAX_LEVELS; ++l) c->adj_last_strides[l] = l < plan.L ? plan.K[l] : 0;
+  c->adj_last_batch = bs;
 
   // everything restored at the end
   std::vector<RealLive> live0(n_real);
@@ -514,11 +574,13 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
   DevMem M;
   double *ck = nullptr, *samp = nullptr, *part = nullptr, *dg = nullptr, *gpart = nullptr;
   double *lam0 = nullptr, *lam1 = nullptr, *seg = nullptr;
+  std::vector<double*> ckl(plan.L, nullptr);  // checkpoint levels; ckl[0] = ck, the forward's
   d4 *w0 = nullptr, *w1 = nullptr;
   int32_t *d_sched = nullptr, *d_rid = nullptr;
   int2* d_sub_tiles = nullptr;
   const int nt_mesh = (hm.n + c->tile - 1) / c->tile;
   HIPCHK(M.alloc(&ck, (size_t)n_ck_kept * bs * n6));
+  ckl[0] = ck;
   HIPCHK(M.alloc(&d_sub_tiles, (size_t)bs * nt_mesh));
   HIPCHK(M.alloc(&samp, (size_t)n_samples * bs * nout));
   HIPCHK(M.alloc(&part, (size_t)bs * S.n_blocks * nf));
@@ -529,7 +591,8 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
     HIPCHK(M.alloc(&lam1, (size_t)bs * n6));
     HIPCHK(M.alloc(&w0, (size_t)bs * np));
     HIPCHK(M.alloc(&w1, (size_t)bs * np));
-    HIPCHK(M.alloc(&seg, (size_t)ckpt * bs * n6));
+    HIPCHK(M.alloc(&seg, (size_t)leaf * bs * n6));
+    for (int l = 1; l < plan.L; ++l) HIPCHK(M.alloc(&ckl[l], (size_t)plan.slots[l] * bs * n6));
     HIPCHK(M.alloc(&gpart, (size_t)bs * nblk * ADJ_NT));
     HIPCHK(M.alloc(&dg, (size_t)bs * n_samples * nf));
   }
@@ -566,6 +629,7 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
       lv[q] = live0[q];
       lv[q].done = 1;
     }
+    std::vector<RealDesc> dsc_sub(nb);  // the members' descriptors with their schedules, own state buffers
     for (int y = 0; y < nb; ++y) {
       const int r = ids[m0 + y];
       lv[r].done = 0;
@@ -573,7 +637,8 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
       lv[r].step = 0;
       lv[r].sched_base = 0;
       HostReal& hr = c->reals[r];
-      RealDesc dsc = hr.desc;
+      RealDesc& dsc = dsc_sub[y];
+      dsc = hr.desc;
       dsc.sched = d_sched + (size_t)y * max_sched;
       dsc.n_sched = n_steps;
       HIPCHK(hipMemcpy(d_sched + (size_t)y * max_sched, schedules + soff[m0 + y],
@@ -588,7 +653,7 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
       hipLaunchKernelGGL(k_scan_final, dim3(1, nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, 0, d_rid,
                          S.n_blocks, nf, nca, part, S.d_pad, S.d_near_f, S.d_near_ca, S.desc.f0, S.desc.c0, out, nout);
     };
-    auto ck_at = [&](int64_t k) { return ck + (size_t)k * nb * n6; };
+    auto ck_at = [&](int l, int64_t k) { return ckl[l] + (size_t)k * nb * n6; };
 
     // forward, with checkpoints and scan samples; a single realization runs
     // the steps between two of those through the gated sweep (one launch per
@@ -598,12 +663,12 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
     int cur = lv[ids[m0]].cur;
     int ks = 0;
     for (int64_t n = 0; n < n_steps && e == hipSuccess;) {
-      if (n % ckpt == 0 && n / ckpt < n_ck_kept)
+      if (n % K0 == 0 && n / K0 < n_ck_kept)
         hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid,
-                           ck_at(n / ckpt), 1);
+                           ck_at(0, n / K0), 1);
       int64_t m = 1;
       if (gated) {
-        int64_t next = std::min<int64_t>(n_steps, (n / ckpt + 1) * ckpt);
+        int64_t next = std::min<int64_t>(n_steps, (n / K0 + 1) * K0);
         if (ks < n_samples && sample_steps[ks] > n) next = std::min<int64_t>(next, sample_steps[ks]);
         m = std::max<int64_t>(1, next - n);
         launch_gated_block(c, m, cur, gmode);
@@ -654,21 +719,20 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
                            dg + (size_t)ks * nf, n_samples * nf, lam);
         --ks;
       }
-      for (int64_t s0 = (n_ck - 1) * ckpt; s0 >= 0 && e == hipSuccess; s0 -= ckpt) {
-        const int64_t s1 = std::min<int64_t>(n_steps, s0 + ckpt);
-        // recompute the segment from its checkpoint, the sweep writing each
-        // state into its slot of the segment
+      // a segment of the last level: recompute it from its checkpoint, the
+      // sweep writing each state into its slot of the segment, then step
+      // backward through it
+      auto leaf_back = [&](int64_t s0, int64_t s1, const double* ckq) -> hipError_t {
         hipLaunchKernelGGL(k_seg_start, dim3(nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, s0);
-        e = hipMemcpyAsync(seg, ck_at(s0 / ckpt), (size_t)nb * n6 * sizeof(double), hipMemcpyDeviceToDevice,
-                           c->stream);
-        for (int64_t n = s0; n < s1 - 1 && e == hipSuccess; ++n) {
+        hipError_t e2 = hipMemcpyAsync(seg, ckq, (size_t)nb * n6 * sizeof(double), hipMemcpyDeviceToDevice,
+                                       c->stream);
+        for (int64_t n = s0; n < s1 - 1 && e2 == hipSuccess; ++n) {
           hipLaunchKernelGGL(k_seg_ptrs, dim3((nb + 63) / 64), dim3(64), 0, c->stream, c->d_desc, c->d_live, d_rid,
                              seg, s0, nb);
           launch_step(c, gmode);
-          e = hipGetLastError();
+          e2 = hipGetLastError();
         }
-        // backward through the segment
-        for (int64_t n = s1 - 1; n >= s0 && e == hipSuccess; --n) {
+        for (int64_t n = s1 - 1; n >= s0 && e2 == hipSuccess; --n) {
           const double* Xn = seg + (size_t)(n - s0) * nb * n6;
           if (n == s1 - 1)
             hipLaunchKernelGGL(k_adj_w, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, d_rid, Xn, lam, wcur);
@@ -682,10 +746,46 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
                              lout, gpart, nx);
           std::swap(lam, lout);
           std::swap(wcur, wnext);
-          e = hipGetLastError();
+          e2 = hipGetLastError();
         }
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-      }
+        if (e2 == hipSuccess) e2 = hipStreamSynchronize(c->stream);
+        return e2;
+      };
+      // the level-l segments of [a, b), last first; the states at a + q K[l]
+      // are in slot q of level l
+      std::function<hipError_t(int, int64_t, int64_t)> back = [&](int l, int64_t a, int64_t b) -> hipError_t {
+        const int64_t K = plan.K[l];
+        hipError_t e2 = hipSuccess;
+        for (int64_t q = (b - a + K - 1) / K - 1; q >= 0 && e2 == hipSuccess; --q) {
+          const int64_t s0 = a + q * K, s1 = std::min<int64_t>(b, s0 + K);
+          if (l == plan.L - 1) {
+            e2 = leaf_back(s0, s1, ck_at(l, q));
+            continue;
+          }
+          // recompute [s0, s1) in the members' own state buffers (the last
+          // level's recomputation pointed them into the segment), keeping
+          // the next level's checkpoints
+          e2 = hipStreamSynchronize(c->stream);
+          for (int y = 0; y < nb && e2 == hipSuccess; ++y)
+            e2 = hipMemcpy(c->d_desc + ids[m0 + y], &dsc_sub[y], sizeof(RealDesc), hipMemcpyHostToDevice);
+          if (e2 != hipSuccess) break;
+          hipLaunchKernelGGL(k_seg_start, dim3(nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, s0);
+          hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid,
+                             (double*)ck_at(l, q), 0);
+          const int64_t K1 = plan.K[l + 1];
+          for (int64_t n = s0; n < s1 && e2 == hipSuccess; ++n) {
+            if ((n - s0) % K1 == 0)
+              hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid,
+                                 ck_at(l + 1, (n - s0) / K1), 1);
+            launch_step(c, gmode);
+            e2 = hipGetLastError();
+            if (e2 == hipSuccess && (n - s0) % 4096 == 4095) e2 = hipStreamSynchronize(c->stream);
+          }
+          if (e2 == hipSuccess) e2 = back(l + 1, s0, s1);
+        }
+        return e2;
+      };
+      if (e == hipSuccess) e = back(0, 0, n_steps);
       if (e == hipSuccess) {
         gp.resize((size_t)nb * nblk * ADJ_NT);
         e = hipMemcpy(gp.data(), gpart, gp.size() * sizeof(double), hipMemcpyDeviceToHost);
@@ -706,7 +806,7 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
       HIPCHK(hipMemcpy(c->d_desc + ids[m0 + y], &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemcpy(c->d_live, live0.data(), n_real * sizeof(RealLive), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, ck_at(0), 0);
+    hipLaunchKernelGGL(k_batch_state, g_cp, dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, ck_at(0, 0), 0);
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   return SRS_OK;
@@ -725,14 +825,16 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
 }
 
 static int loss_grad_check(srs_ctx* c, int32_t scan_id, int64_t n_steps, int32_t n_samples,
-                           const int64_t* sample_steps, const double* target, int32_t ckpt, const void* loss) {
+                           const int64_t* sample_steps, const double* target, int32_t ckpt, const void* loss,
+                           bool auto_ckpt = false) {
   if (!c) return srs::fail(SRS_E_ARG, "null context");
   if (!c->finalized) return srs::fail(SRS_E_STATE, "context not finalized");
   if (c->group != 1) return srs::fail(SRS_E_ARG, "the adjoint needs realization groups of size 1");
   if (scan_id < 0 || scan_id >= (int)c->scans.size()) return srs::fail(SRS_E_ARG, "bad scan id %d", scan_id);
   if (c->scans[scan_id].n_f <= 0) return srs::fail(SRS_E_ARG, "the scan has no fluorescence points");
   if (n_steps <= 0 || n_samples <= 0 || !sample_steps || !target || !loss) return srs::fail(SRS_E_ARG, "bad arguments");
-  if (ckpt <= 0 || ckpt > 65536) return srs::fail(SRS_E_ARG, "checkpoint interval must be 1..65536");
+  if ((ckpt <= 0 && !(auto_ckpt && ckpt == 0)) || ckpt > 65536)
+    return srs::fail(SRS_E_ARG, "checkpoint interval must be 1..65536%s", auto_ckpt ? " or 0" : "");
   for (int k = 0; k < n_samples; ++k)
     if (sample_steps[k] < 1 || sample_steps[k] > n_steps || (k && sample_steps[k] <= sample_steps[k - 1]))
       return srs::fail(SRS_E_ARG, "sample steps must increase within 1..n_steps");
@@ -758,8 +860,7 @@ int srs_linescan_loss_grad_batch(srs_ctx* c, int32_t n_real, const int32_t* real
   int rc = c ? SRS_OK : srs::fail(SRS_E_ARG, "null context");
   if (rc) return rc;
   if (n_real <= 0 || !real_ids) return srs::fail(SRS_E_ARG, "no realizations");
-  if (ckpt == 0) ckpt = (int32_t)std::max<double>(1.0, std::ceil(std::sqrt((double)std::max<int64_t>(n_steps, 1))));
-  if ((rc = loss_grad_check(c, scan_id, n_steps, n_samples, sample_steps, targets, ckpt, losses))) return rc;
+  if ((rc = loss_grad_check(c, scan_id, n_steps, n_samples, sample_steps, targets, ckpt, losses, true))) return rc;
   std::vector<int32_t> ids(real_ids, real_ids + n_real);
   std::vector<char> seen(c->reals.size(), 0);
   bool any_ryr = false;
@@ -773,6 +874,16 @@ int srs_linescan_loss_grad_batch(srs_ctx* c, int32_t n_real, const int32_t* real
   if (any_ryr && !schedules) return srs::fail(SRS_E_ARG, "null schedules");
   return loss_grad_run(c, ids, scan_id, n_steps, schedules, n_samples, sample_steps, targets, ckpt, max_batch, losses,
                        grads, scans);
+}
+
+int srs_adjoint_last_plan(srs_ctx* c, int32_t* levels, int64_t* strides, int32_t* sub_batch) {
+  srs::clear_error();
+  if (!c) return srs::fail(SRS_E_ARG, "null context");
+  if (levels) *levels = c->adj_last_levels;
+  if (strides)
+    for (int l = 0; l < ADJ_MAX_LEVELS; ++l) strides[l] = c->adj_last_strides[l];
+  if (sub_batch) *sub_batch = c->adj_last_batch;
+  return SRS_OK;
 }
 
 

@@ -297,6 +297,9 @@ struct srs_ctx {
   int drop_finished = 1; // sweep without the tiles of groups that have all finished
   int gated = 1;         // fold each step's gating into the next step's sweep (k_sweep_gated) where it applies
   int gpu_layout = 0;     // states records in the GPU program's layout (Ca_cyt at each RyR appended)
+  int adj_levels = 0;     // adjoint checkpoint levels, 0: the fewest that fit the sub-batch
+  int adj_last_levels = 0, adj_last_batch = 0;  // the plan of the last loss/gradient call
+  int64_t adj_last_strides[6] = {};
   bool compact_all = false;
   bool layout_fixed = false;
   hipStream_t stream = nullptr;

@@ -1139,6 +1139,10 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
     case SRS_TUNE_DROP_FINISHED:
       c->drop_finished = value != 0;
       return c->finalized ? tiles_full(c) : SRS_OK;
+    case SRS_TUNE_ADJ_LEVELS:
+      if (value < 0 || value > 6) return srs::fail(SRS_E_ARG, "adjoint checkpoint levels must be 0..6");
+      c->adj_levels = value;
+      return SRS_OK;
     case SRS_TUNE_GROUP:
       if (value != 1 && value != 2 && value != 4) return srs::fail(SRS_E_ARG, "group must be 1, 2 or 4");
       if (c->finalized) return srs::fail(SRS_E_STATE, "group must be set before srs_ctx_finalize");

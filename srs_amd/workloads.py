@@ -419,3 +419,65 @@ def spark_adjoint_problem(t_sim: float = 5.0, n_points: int = 64, n_samples: int
     prob = LinescanProblem(ctx, sid, schedule, samples, tgt, p, pr.X0, ckpt=ckpt)
     return prob, dict(n_steps=n_steps, n_ele=g.n_ele, n_f=nf, theta_true=Pt[idx], loss_at_target=L0,
                       n_openings=int(schedule.sum()))
+
+
+def dyad_adjoint_batch(n_real: int = 256, n_steps: int = 256, n_points: int = 64, n_samples: int = 8,
+                       max_batch: int = 0, h: float = 0.03, seed: int = 7, perturb: float = 0.1, log=None):
+    """The batched fitting workload: n_real realizations on the dyad-box
+    geometry at h = 30 nm (927,816 elements), each with its own RyR trajectory
+    (Philox stream r) recorded by one gated run of all of them and then
+    frozen; a line scan of n_points along x through the cluster centre sampled
+    n_samples times; each member's target made by its own frozen-channel run
+    with theta moved by +-perturb (signs drawn from `seed`).  Returns the
+    context, the scan id, the schedules, the sample steps, the targets, the
+    parameters and a dict of sizes."""
+    from . import R, adjoint_theta, read_scan_parameters
+    from . import Context
+
+    pr = prepare_dyad(h=h)
+    g = pr.grid
+    p = pr.params.copy()
+    ctx = Context(0)
+    mid = ctx.add_mesh_grid(g, pr.aij, pr.nij, perm=kuhn_block_order(g))
+    for r in range(n_real):
+        ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=38376427, stream=r)
+    ctx.finalize()
+    for r in range(n_real):
+        rs, ls, lvs, Rr = initial_channels(pr, seed=r)
+        ctx.state_set(r, pr.X0)
+        ctx.channels_set(r, rs, ls, lvs, Rr)
+        ctx.globals_set(r, 0.0, p.reals[R["DEFAULT_CA_NSR"]])
+    if log:
+        log(f"  {n_real} realizations of {g.n_ele} elements set up; recording {n_steps} steps of schedules")
+    scheds = ctx.record_schedules(n_steps)
+    for r in range(n_real):
+        rs, ls, lvs, Rr = initial_channels(pr, seed=r)
+        ctx.channels_set(r, scheds[r][0], ls, lvs, Rr)
+    cen = g.centroids()
+    c0 = np.array([1.0, 1.0, 0.5])  # the linescan program assumes a centred mesh
+    d = read_scan_parameters(None)
+    d.axis = 0
+    rc = cen[g.ryr_ele].mean(axis=0) - c0
+    d.offset1, d.offset2 = float(rc[1]), float(rc[2])
+    d.x_min, d.x_max = -0.96, 0.96
+    d.res_x_f = (d.x_max - d.x_min) / (n_points + 0.5)  # the point count truncates (x_max - x_min) / res
+    d.res_x_ca = 0.01
+    d.domain[0], d.domain[1], d.domain[2] = 2.0, 2.0, 1.0
+    d.pad_min, d.pad_max, d.pad_res = -2.5, 2.5, 0.1
+    sid, nf, _ = ctx.add_scan(mid, cen - c0, d)
+    samples = np.linspace(n_steps // n_samples, n_steps, n_samples).astype(np.int64)
+    idx = adjoint_theta()
+    sgn = np.where(np.random.default_rng(seed).random(len(idx)) < 0.5, -1.0, 1.0)
+    Pt = p.reals.copy()
+    Pt[idx] *= 1 + perturb * sgn
+    for r in range(n_real):
+        ctx.params_set(r, Pt)
+    if log:
+        log("  targets")
+    zero = np.zeros((n_real, n_samples, nf))
+    _, tgt = ctx.loss_grad_batch(range(n_real), sid, scheds, samples, zero, grad=False, scans=True,
+                                 max_batch=max_batch)
+    for r in range(n_real):
+        ctx.params_set(r, p.reals)
+    return ctx, sid, scheds, samples, tgt, p, dict(n_ele=g.n_ele, n_f=nf, dt=float(p.reals[R["T_STEP"]]),
+                                                   theta_true=Pt[idx])

@@ -551,3 +551,82 @@ def test_boundary_rates_follow_d_in_step_and_gradient():
         fd = (Ls[0] - Ls[1]) / (2 * eps * th)
         assert abs(gr[k] - fd) <= 1e-5 * abs(fd), (name, gr[k], fd)
     ctx.close()
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_checkpoint_levels_do_not_change_the_batched_gradient(levels):
+    """Two- and three-level checkpoint plans (each level's segments
+    recomputed from its checkpoints, the last level's with every state kept)
+    give the one-level plan's losses and gradients bit for bit, in one
+    sub-batch and in sub-batches of two; the plan in use is reported."""
+    pr, members, samples = _batch_setup()
+    tg, _ = _targets(pr, members, samples)
+    ctx, sid, nf, _ = _batch_ctx(pr, members)
+    n = len(members)
+    scheds = [m[2] for m in members]
+    ctx.tune(adj_levels=1)
+    L1, g1 = ctx.loss_grad_batch(range(n), sid, scheds, samples, tg, ckpt=23)
+    assert ctx.adjoint_plan() == dict(levels=1, strides=[23], sub_batch=n)
+    ctx.tune(adj_levels=levels)
+    for ck, mb in ((0, 0), (5, 2)):
+        L2, g2 = ctx.loss_grad_batch(range(n), sid, scheds, samples, tg, ckpt=ck, max_batch=mb)
+        plan = ctx.adjoint_plan()
+        assert plan["levels"] == levels and plan["sub_batch"] == (mb or n)
+        assert plan["strides"] == sorted(plan["strides"], reverse=True) and plan["strides"][0] < B_STEPS
+        if ck:
+            assert plan["strides"][-1] == ck
+        assert np.array_equal(L2, L1) and np.array_equal(g2, g1)
+    ctx.close()
+
+
+@pytest.mark.timeout(1100)
+def test_dyad30_batch_gradients_at_full_batch_match_central_differences():
+    """The batched fitting workload at its batch size: 256 members on the
+    h = 30 nm dyad geometry (927,816 elements each), own recorded
+    trajectories, 2,000 steps, a 64-point line scan x 8 samples, with the
+    two-level checkpoint plan (outer checkpoints, inner checkpoints per outer
+    segment, recomputed segments of every state).  The gradients of four
+    sampled members against central differences of their losses (loss-only
+    calls with only that member's theta moved), |theta g - theta g_fd| <=
+    1e-5 relative (components three orders below the largest held to 1e-5 of
+    the largest); losses vanish at each member's generating parameters."""
+    n_real, n_steps = 256, 2000
+    print("dyad30 batch: setup", flush=True)
+    ctx, sid, scheds, samples, tgt, p, info = W.dyad_adjoint_batch(n_real, n_steps, log=print)
+    assert info["n_ele"] == 927816 and info["n_f"] == 64
+    idx = S.adjoint_theta()
+    ctx.tune(adj_levels=2)
+    print("dyad30 batch: losses and gradients of all members", flush=True)
+    L, g = ctx.loss_grad_batch(range(n_real), sid, scheds, samples, tgt)
+    plan = ctx.adjoint_plan()
+    print(f"  plan {plan}", flush=True)
+    assert plan["levels"] == 2 and np.all(L > 0) and np.all(np.isfinite(g))
+    picked = [0, 85, 170, 255]
+    th = p.reals[idx].copy()
+    eps = 1e-5
+    # a loss-only call starts from each member's current state, which every call restores
+    Pt = p.reals.copy()
+    Pt[idx] = info["theta_true"]
+    for m in picked:
+        ctx.params_set(m, Pt)
+    Lt = ctx.loss_grad_batch(picked, sid, [scheds[m] for m in picked], samples, tgt[picked], grad=False)
+    assert np.all(Lt == 0.0)
+    fd = np.zeros((len(picked), len(idx)))
+    for k in range(len(idx)):
+        Ls = []
+        for sgn in (1, -1):
+            P = p.reals.copy()
+            P[idx[k]] *= 1 + sgn * eps
+            for m in picked:
+                ctx.params_set(m, P)
+            Ls.append(ctx.loss_grad_batch(picked, sid, [scheds[m] for m in picked], samples, tgt[picked], grad=False))
+        fd[:, k] = (Ls[0] - Ls[1]) / (2 * eps * th[k])
+        print(f"  theta {k}: member 0 adjoint {th[k] * g[0, k]:.6e}, central difference {th[k] * fd[0, k]:.6e}",
+              flush=True)
+    ctx.close()
+    for q, m in enumerate(picked):
+        sens, sens_fd = th * g[m], th * fd[q]
+        scale = np.abs(sens_fd).max()
+        assert scale > 0
+        err = np.abs(sens - sens_fd) / np.maximum(np.abs(sens_fd), 1e-3 * scale)
+        assert err.max() <= 1e-5, (m, err, sens, sens_fd)
