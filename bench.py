@@ -73,7 +73,7 @@ def profiled_traffic(path):
     return None
 
 
-def setup_plain(pr, order="block", tune=None):
+def setup_plain(pr, order="block", tune=None, ext=None):
     import srs_amd as S
     from srs_amd.workloads import initial_channels, kuhn_block_order
 
@@ -86,6 +86,8 @@ def setup_plain(pr, order="block", tune=None):
         mid = ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij, order=S.ORDER_RCM if order == "rcm" else S.ORDER_AS_GIVEN)
     g = pr.grid
     ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, pr.params, seed=38376427, stream=0)
+    if ext:
+        ctx.extensions(0, **ext)
     ctx.finalize()
     rs, ls, lvs, R = initial_channels(pr)
     ctx.state_set(0, pr.X0)
@@ -95,10 +97,10 @@ def setup_plain(pr, order="block", tune=None):
     return ctx
 
 
-def bench_plain(pr, steps, warmup, order, tune):
+def bench_plain(pr, steps, warmup, order, tune, ext=None):
     import srs_amd as S
 
-    ctx = setup_plain(pr, order, tune)
+    ctx = setup_plain(pr, order, tune, ext)
     ctx.explicit_step(warmup, S.STEP_GRAPH)
     ctx.prepare(steps, S.STEP_GRAPH)  # graphs captured outside the timed region
     ctx.sync()
@@ -384,6 +386,8 @@ def main():
     ap.add_argument("--adjoint-batch", type=int, default=256,
                     help="realizations in the batched adjoint leg on the h=30 nm dyad geometry (0: skip)")
     ap.add_argument("--adjoint-batch-steps", type=int, default=256)
+    ap.add_argument("--config-physics", type=int, default=1,
+                    help="1: also time the plain workload with the configuration's jSR diffusion and SR sites")
     ap.add_argument("--fidelity", type=int, default=0,
                     help="realizations in a spark-fidelity ensemble run to completion (0: skip)")
     ap.add_argument("--order", choices=("block", "rcm", "given"), default="block",
@@ -458,6 +462,20 @@ def main():
                      "bytes_per_element": BYTES_PER_ELEMENT,
                      "live_bytes_per_launch": live_bytes(pr.grid.domain, pr.grid.tropc)},
     }
+    if args.config_physics:
+        # the same workload with the configuration's jSR diffusion (0.35 um^2/ms)
+        # and 47 uM SR sites, which the reference cannot express: dt from the
+        # per-domain bound
+        prc = W.with_config_physics(pr)
+        cwall_g, _, csweep_ms, _ = bench_plain(prc, args.steps, args.warmup, args.order, tune, W.CONFIG_SR_SITES)
+        cdt = float(prc.params.reals[1])
+        out["config_physics"] = {
+            "value": args.steps * cdt / cwall_g, "unit": "spark-ms simulated/s", "dt_ms": cdt,
+            "ms_per_step": 1e3 * cwall_g / args.steps, "kernel_ms": csweep_ms,
+            "physics": "Ca D 0.22 um^2/ms in the cytosol and 0.35 in the jSR, 47 uM SR sites (K_D 0.6 uM) as a "
+                       "rapid buffer; model extensions (the reference uses D_Ca in both domains and has no SR "
+                       "sites), so no reference timing exists for it"}
+        log(f"config physics: {1e3 * cwall_g / args.steps:.4f} ms/step at dt {cdt:.4e} ms")
     if args.cpu_budget > 0:
         n, cwall, kind = bench_cpu(pr, args.cpu_budget)
         out["cpu_baseline"] = {"value": n * dt / cwall, "unit": "spark-ms simulated/s", "cores": 1, "kind": kind,

@@ -240,6 +240,28 @@ int srs_run_cru3d(const char* param_path, const char* mesh_base, const char* out
  * the order moves data, never changes results. */
 int srs_morton_order(int32_t n_ele, const int32_t* ele_nodes, const double* node_xyz, int32_t* perm);
 
+/* Model extensions the reference cannot express (no counterpart there).
+ * All are off by default, and a run with them off is the reference's model
+ * bit for bit.
+ *   jsr_diffusion  1: Ca diffuses with D_CAJSR (reals[SRS_R_D_CA_JSR]) in jSR
+ *                  elements.  The reference parses D_CAJSR but assembles the
+ *                  operator with D = 1 (cpu/InitFuncs.cpp:1081) and scales Ca
+ *                  by D_CA in both domains (cpu/RunSimulation.cpp:62); jSR and
+ *                  cytosol are joined by no-flux faces only, so a per-domain
+ *                  coefficient is a per-row scale.
+ *   b_sr_sites     SR Ca-binding sites in the cytosol (uM), a rapid buffer of
+ *   k_d_sr_sites   dissociation constant k_d_sr_sites (uM) in the form of the
+ *                  sarcolemmal rapid buffer (cpu/RunSimulation.cpp:15-22):
+ *                  beta += b K / (K + c)^2 in every cytosol element.  0: none.
+ * srs_read_extensions reads the XML symbols flag_jsr_diffusion ("true"),
+ * B_TOT_SRSITES and K_D_SRSITES from a parameter file (the reference ignores
+ * unknown symbols; defaults 0, 0 and 0.6 uM). */
+typedef struct srs_extensions {
+  int32_t jsr_diffusion;
+  double b_sr_sites, k_d_sr_sites;
+} srs_extensions;
+int srs_read_extensions(const char* path, srs_extensions* out);
+
 /* ======================================================= device layer */
 
 typedef struct srs_ctx srs_ctx;
@@ -281,6 +303,12 @@ int srs_ctx_create(int device, srs_ctx** out);
                                     fewest whose memory fits the sub-batch, 1..6 forced; each level below the
                                     first is one more forward pass and needs about (L+1) n^(1/(L+1)) states */
 int srs_ctx_tune(srs_ctx* ctx, int32_t key, int32_t value);
+
+/* The model extensions of one realization (srs_extensions; no reference
+ * counterpart), at any time after srs_ctx_add_realization.  The adjoint
+ * (srs_linescan_loss_grad*) covers the reference model only and refuses a
+ * realization with an extension on. */
+int srs_ctx_extensions(srs_ctx* ctx, int32_t real, const srs_extensions* ext);
 
 /* Replaces FreeDomainStruct / FreeGlobalStruct (gpu/RunSimulation.cu:874-901). */
 int srs_ctx_destroy(srs_ctx* ctx);

@@ -300,6 +300,14 @@ def run_cru3d(param: str, mesh: str, out_dir: str | None, rank: int = 0, start: 
     return secs.value
 
 
+def read_extensions(path: str | None = None) -> dict:
+    """The model extensions of a parameter file (srs_read_extensions):
+    jsr_diffusion, b_sr_sites (uM), k_d_sr_sites (uM)."""
+    e = _abi.SrsExtensions()
+    check(lib().srs_read_extensions(path.encode() if path else None, C.byref(e)))
+    return dict(jsr_diffusion=e.jsr_diffusion, b_sr_sites=e.b_sr_sites, k_d_sr_sites=e.k_d_sr_sites)
+
+
 def read_scan_parameters(path: str | None = None) -> _abi.SrsScanDesc:
     """The linescan program's ReadParameters (linescan/main.cpp:353-508)."""
     d = _abi.SrsScanDesc()
@@ -411,6 +419,12 @@ class Context:
     def record_layout(self, gpu_layout: bool):
         """States records with Ca_cyt at each RyR (the GPU program's layout); before finalize."""
         check(lib().srs_ctx_record_layout(self._h, int(gpu_layout)))
+
+    def extensions(self, r: int, jsr_diffusion: int = 0, b_sr_sites: float = 0.0, k_d_sr_sites: float = 0.6):
+        """Model extensions of realization r (srs_ctx_extensions): Ca diffusion
+        with D_CAJSR in jSR elements, SR-site rapid buffer in the cytosol."""
+        e = _abi.SrsExtensions(int(jsr_diffusion), float(b_sr_sites), float(k_d_sr_sites))
+        check(lib().srs_ctx_extensions(self._h, r, C.byref(e)))
 
     def finalize(self, record_capacity: int = 0):
         check(lib().srs_ctx_finalize(self._h, record_capacity))

@@ -76,6 +76,10 @@ class SrsChanDesc(C.Structure):
     ]
 
 
+class SrsExtensions(C.Structure):
+    _fields_ = [("jsr_diffusion", C.c_int32), ("b_sr_sites", C.c_double), ("k_d_sr_sites", C.c_double)]
+
+
 class SrsLoopState(C.Structure):
     _fields_ = [
         ("t", C.c_double), ("dt_last", C.c_double), ("step", C.c_int64), ("done", C.c_int32),
@@ -104,6 +108,8 @@ _PROTOS = {
     "srs_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "srs_ctx_destroy": (C.c_int, [C.c_void_p]),
     "srs_ctx_record_layout": (C.c_int, [C.c_void_p, C.c_int32]),
+    "srs_ctx_extensions": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(SrsExtensions)]),
+    "srs_read_extensions": (C.c_int, [C.c_char_p, C.POINTER(SrsExtensions)]),
     "srs_ctx_tune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "srs_ctx_add_mesh": (C.c_int, [C.c_void_p, C.POINTER(SrsMeshDesc), i32p]),
     "srs_ctx_add_realization": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(SrsChanDesc), i32p, f64p,

@@ -848,6 +848,8 @@ int srs_linescan_loss_grad(srs_ctx* c, int32_t r, int32_t scan_id, int64_t n_ste
   if (rc) return rc;
   if ((rc = loss_grad_check(c, scan_id, n_steps, n_samples, sample_steps, target, ckpt, loss))) return rc;
   if (c->reals[r].mesh != c->scans[scan_id].mesh) return srs::fail(SRS_E_ARG, "realization is not on the scan's mesh");
+  if (c->reals[r].ext.jsr_diffusion || c->reals[r].ext.b_sr_sites != 0.0)
+    return srs::fail(SRS_E_ARG, "realization %d has a model extension on; the adjoint covers the reference model", r);
   if (!c->reals[r].ryr_ele.empty() && !schedule) return srs::fail(SRS_E_ARG, "null schedule");
   return loss_grad_run(c, {r}, scan_id, n_steps, schedule, n_samples, sample_steps, target, ckpt, 1, loss, grad,
                        nullptr);
@@ -869,6 +871,8 @@ int srs_linescan_loss_grad_batch(srs_ctx* c, int32_t n_real, const int32_t* real
     if (seen[r]++) return srs::fail(SRS_E_ARG, "realization %d listed twice", r);
     if (c->reals[r].mesh != c->scans[scan_id].mesh)
       return srs::fail(SRS_E_ARG, "realization %d is not on the scan's mesh", r);
+    if (c->reals[r].ext.jsr_diffusion || c->reals[r].ext.b_sr_sites != 0.0)
+      return srs::fail(SRS_E_ARG, "realization %d has a model extension on; the adjoint covers the reference model", r);
     any_ryr |= !c->reals[r].ryr_ele.empty();
   }
   if (any_ryr && !schedules) return srs::fail(SRS_E_ARG, "null schedules");

@@ -87,6 +87,10 @@ struct RealDesc {
   float* rec;
   int32_t rec_cap, rec_len, ryr_out, gpu_layout;
   uint64_t key, stream;
+  // model extensions (srs_extensions): Ca diffusion with D_CAJSR in jSR
+  // elements, SR-site rapid buffer (b_sr uM, dissociation constant k_sr uM)
+  int32_t jsr_diff;
+  double b_sr, k_sr;
   double P[SRS_N_REAL];
   int32_t I[SRS_N_INT];
 };
@@ -260,6 +264,7 @@ struct HostMesh {
 struct HostReal {
   int32_t mesh = 0;
   double* d_br_own = nullptr;  // boundary rates rescaled to this realization's D (srs_params_set)
+  srs_extensions ext{0, 0.0, 0.6};
   std::vector<int32_t> ryr_ele, ryr_jsr, ryr_nb, lcc_ele;  // dev numbering
   std::vector<int32_t> celem, cstart, centry;
   int32_t ints[SRS_N_INT];

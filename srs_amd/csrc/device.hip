@@ -143,8 +143,10 @@ __device__ __forceinline__ void step_elem(const RealDesc& d, const RealLive& l, 
     br_at = br[2 * np + j];
     br_cm = br[3 * np + j];
   }
-  // cpu/RunSimulation.cpp:61-71 (Ca uses D_CA in both domains)
-  double rca = P[SRS_R_D_CA] * (c_ca * a0 + m1.x * a1 + m2.x * a2 + m3.x * a3 + m4.x * a4) + br_ca;
+  // cpu/RunSimulation.cpp:61-71 (Ca uses D_CA in both domains; with the jSR
+  // diffusion extension, D_CAJSR in jSR elements)
+  const double d_ca = (live_cs && d.jsr_diff) ? P[SRS_R_D_CA_JSR] : P[SRS_R_D_CA];
+  double rca = d_ca * (c_ca * a0 + m1.x * a1 + m2.x * a2 + m3.x * a3 + m4.x * a4) + br_ca;
   double rdy = P[SRS_R_D_DYE] * (c_dy * a0 + m1.w * a1 + m2.w * a2 + m3.w * a3 + m4.w * a4) + br_dy;
   double rat = P[SRS_R_D_ATP] * (c_at * a0 + m1.y * a1 + m2.y * a2 + m3.y * a3 + m4.y * a4) + br_at;
   double rcm = P[SRS_R_D_CMDN] * (c_cm * a0 + m1.z * a1 + m2.z * a2 + m3.z * a3 + m4.z * a4) + br_cm;
@@ -204,12 +206,18 @@ __device__ __forceinline__ void step_elem(const RealDesc& d, const RealLive& l, 
     }
   }
 
-  if (f & F_SL) {  // :15-22; B == 1 exactly where the TT area is zero
-    const double ttj = gp(d.tt)[j], v0j = gp(d.V0)[j];
-    const double btothi = ttj * P[SRS_R_B_TOT_SL_HI] / (v0j * (1e-15));
-    const double btotlo = ttj * P[SRS_R_B_TOT_SL_LO] / (v0j * (1e-15));
-    const double kh = P[SRS_R_K_D_SL_HI], kl = P[SRS_R_K_D_SL_LO];
-    const double B = 1 + (btothi * kh / ((kh + c_ca) * (kh + c_ca))) + (btotlo * kl / ((kl + c_ca) * (kl + c_ca)));
+  // SR-site rapid buffer of the cytosol (extension; b_sr == 0 in the reference model)
+  const bool sr = !live_cs && d.b_sr != 0.0;
+  if ((f & F_SL) || sr) {
+    double B = 1.0;
+    if (f & F_SL) {  // :15-22; B == 1 exactly where the TT area is zero
+      const double ttj = gp(d.tt)[j], v0j = gp(d.V0)[j];
+      const double btothi = ttj * P[SRS_R_B_TOT_SL_HI] / (v0j * (1e-15));
+      const double btotlo = ttj * P[SRS_R_B_TOT_SL_LO] / (v0j * (1e-15));
+      const double kh = P[SRS_R_K_D_SL_HI], kl = P[SRS_R_K_D_SL_LO];
+      B = 1 + (btothi * kh / ((kh + c_ca) * (kh + c_ca))) + (btotlo * kl / ((kl + c_ca) * (kl + c_ca)));
+    }
+    if (sr) B = B + d.b_sr * d.k_sr / ((d.k_sr + c_ca) * (d.k_sr + c_ca));
     rca /= B;
   }
 
@@ -1161,6 +1169,25 @@ int srs_ctx_record_layout(srs_ctx* c, int32_t gpu_layout) {
   return SRS_OK;
 }
 
+int srs_ctx_extensions(srs_ctx* c, int32_t r, const srs_extensions* e) {
+  srs::clear_error();
+  if (!c || !e) return srs::fail(SRS_E_ARG, "null argument");
+  if (r < 0 || r >= (int32_t)c->reals.size()) return srs::fail(SRS_E_ARG, "realization %d out of range", r);
+  if (e->jsr_diffusion != 0 && e->jsr_diffusion != 1) return srs::fail(SRS_E_ARG, "jsr_diffusion must be 0 or 1");
+  if (!(e->b_sr_sites >= 0.0) || (e->b_sr_sites > 0.0 && !(e->k_d_sr_sites > 0.0)))
+    return srs::fail(SRS_E_ARG, "SR sites need b >= 0 and, when b > 0, a dissociation constant > 0");
+  HostReal& hr = c->reals[r];
+  hr.ext = *e;
+  if (!c->finalized) return SRS_OK;
+  hr.desc.jsr_diff = e->jsr_diffusion;
+  hr.desc.b_sr = e->b_sr_sites;
+  hr.desc.k_sr = e->k_d_sr_sites;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->d_desc + r, &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice));
+  return SRS_OK;
+}
+
 int srs_ctx_destroy(srs_ctx* c) {
   if (!c) return SRS_OK;
   (void)hipSetDevice(c->device);
@@ -1431,6 +1458,9 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
     }
     d.key = hr.key;
     d.stream = hr.stream;
+    d.jsr_diff = hr.ext.jsr_diffusion;
+    d.b_sr = hr.ext.b_sr_sites;
+    d.k_sr = hr.ext.k_d_sr_sites;
     memcpy(d.P, hr.reals, sizeof(d.P));
     memcpy(d.I, hr.ints, sizeof(d.I));
     hr.desc = d;

@@ -292,6 +292,40 @@ extern "C" int srs_read_parameters(const char* path, int32_t ints[SRS_N_INT], do
   return SRS_OK;
 }
 
+// ------------------------------------------------------------ model extensions
+extern "C" int srs_read_extensions(const char* path, srs_extensions* e) {
+  srs::clear_error();
+  if (!e) return srs::fail(SRS_E_ARG, "null output");
+  e->jsr_diffusion = 0;
+  e->b_sr_sites = 0.0;
+  e->k_d_sr_sites = 0.6;
+  if (!path) return SRS_OK;
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return srs::fail(SRS_E_IO, "cannot open parameter file %s", path);
+  std::stringstream ss;
+  ss << in.rdbuf();
+  const std::string src = ss.str();
+  Parser ps(src);
+  Elem root;
+  if (!ps.skip_misc() || !ps.element(root)) return srs::fail(SRS_E_IO, "%s: XML error: %s", path, ps.err.c_str());
+  std::vector<const Elem*> params;
+  collect(root, params);
+  for (const Elem* pe : params) {
+    const std::string* symbol = nullptr;
+    const std::string* value = nullptr;
+    for (const Elem& k : pe->kids) {
+      if (k.name == "symbol") symbol = &k.text;
+      else if (k.name == "value") value = &k.text;
+    }
+    if (!symbol || !value) continue;
+    const char* v = value->c_str();
+    if (*symbol == "flag_jsr_diffusion") e->jsr_diffusion = strcmp(v, "true") == 0 ? 1 : 0;
+    else if (*symbol == "B_TOT_SRSITES") e->b_sr_sites = atof(v);
+    else if (*symbol == "K_D_SRSITES") e->k_d_sr_sites = atof(v);
+  }
+  return SRS_OK;
+}
+
 // --------------------------------------------------------- linescan parameters
 namespace {
 

@@ -146,6 +146,8 @@ extern "C" int srs_run_cru3d(const char* param_path, const char* mesh_base, cons
   char title[SRS_TITLE_LEN];
   int rc = srs_read_parameters(param_path, I, P, title);
   if (rc) return rc;
+  srs_extensions ext;
+  if ((rc = srs_read_extensions(param_path, &ext))) return rc;
   if (I[SRS_I_INITIALIZE_FROM_FILE]) return srs::fail(SRS_E_ARG, "initial states from file are not supported");
 
   // Output directory, cpu/main.cpp:30-133
@@ -211,6 +213,7 @@ extern "C" int srs_run_cru3d(const char* param_path, const char* mesh_base, cons
     seeds[k] = (uint64_t)(uint32_t)seed;
     int32_t rid;
     if ((rc = srs_ctx_add_realization(G.ctx, mesh_id, &cd, I, P, seeds[k], (uint64_t)run, &rid))) return rc;
+    if ((rc = srs_ctx_extensions(G.ctx, rid, &ext))) return rc;
   }
   const double dt = P[SRS_R_T_STEP];
   const double iv = P[SRS_R_OUTPUT_STATES_INTERVAL];

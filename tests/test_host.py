@@ -124,6 +124,31 @@ def test_every_shipped_parameter_file():
         assert p.title == title[: len(p.title)] and len(p.title) == min(len(title), 255), f
 
 
+def test_model_extensions_are_off_in_every_shipped_parameter_file(tmp_path):
+    """The extension symbols (flag_jsr_diffusion, B_TOT_SRSITES, K_D_SRSITES)
+    appear in no shipped file, so every shipped file runs the reference
+    model; a file that sets them is read with the parameter reader's value
+    rules, and the reference's reader ignores them (unknown symbols)."""
+    files = sorted(glob.glob(os.path.join(REF_PARAM, "*.xml")) + glob.glob(os.path.join(REF_PARAM, "*", "*.xml")))
+    assert len(files) >= 100
+    off = dict(jsr_diffusion=0, b_sr_sites=0.0, k_d_sr_sites=0.6)
+    assert S.read_extensions(None) == off
+    for f in files:
+        assert S.read_extensions(f) == off, f
+    body = "".join(f"<parameter><symbol>{k}</symbol><value>{v}</value></parameter>" for k, v in [
+        ("D_CA", "0.22"), ("D_CAJSR", "0.35"), ("flag_jsr_diffusion", "true"), ("B_TOT_SRSITES", "47uM"),
+        ("K_D_SRSITES", "0.6")])
+    x = tmp_path / "ext.xml"
+    x.write_text(f'<?xml version="1.0"?>\n<simulation>{body}</simulation>\n')
+    assert S.read_extensions(str(x)) == dict(jsr_diffusion=1, b_sr_sites=47.0, k_d_sr_sites=0.6)
+    p = S.read_parameters(str(x))
+    assert p.reals[S.R["D_CA_JSR"]] == 0.35 and p.reals[S.R["D_CA"]] == 0.22
+    q = S.read_parameters(None)
+    q.setr(D_CA=0.22, D_CA_JSR=0.35)
+    np.testing.assert_array_equal(p.reals, q.reals)  # the extension symbols touch no reference parameter
+    np.testing.assert_array_equal(p.ints, q.ints)
+
+
 def test_parameter_value_quirks(tmp_path):
     """atoi/atof prefix parsing, 'true' as the only true flag, ns -> ms."""
     body = "".join(f"<parameter><symbol>{s}</symbol><value>{v}</value></parameter>" for s, v in [
