@@ -57,7 +57,40 @@ def build_ref() -> str | None:
     return REF_SO
 
 
+def build_bridge_check() -> str | None:
+    """oracle/_ref/bridge_check: the per-step bridge of INTEGRATION.md (its
+    first C block, extracted to _ref/srs_bridge.inc) in a driver that steps it
+    beside the reference's own ExplicitStep (cpu/RunSimulation.cpp compiled
+    where it lies), linked against srs_amd/libsrs_amd.so.  Needs the reference
+    tree and a built library; skipped otherwise (an existing build is kept)."""
+    import re
+
+    src = os.path.join(REF_CPU, "RunSimulation.cpp")
+    lib_dir = os.path.join(os.path.dirname(HERE), "srs_amd")
+    exe = os.path.join(REF_DIR, "bridge_check")
+    if not os.path.exists(src) or not os.path.exists(os.path.join(lib_dir, "libsrs_amd.so")):
+        print("reference tree or library absent; keeping existing bridge_check", flush=True)
+        return exe if os.path.exists(exe) else None
+    os.makedirs(REF_DIR, exist_ok=True)
+    with open(os.path.join(os.path.dirname(HERE), "INTEGRATION.md")) as f:
+        blocks = re.findall(r"```c\n(.*?)```", f.read(), re.S)
+    with open(os.path.join(REF_DIR, "srs_bridge.inc"), "w") as f:
+        f.write("#include <cstdio>\n#include <cstdlib>\n#include <cstring>\n" + blocks[0])
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    obj_rs = os.path.join(REF_DIR, "RunSimulation_exe.o")
+    obj_b = os.path.join(REF_DIR, "bridge_check.o")
+    _run(["g++", "-O3", "-ffunction-sections", "-fdata-sections", "-w", "-I", REF_CPU, "-c", src, "-o", obj_rs])
+    _run(["g++", "-O2", "-std=c++17", "-ffunction-sections", "-fdata-sections", "-I", REF_CPU, "-I", inc, "-I",
+          REF_DIR, "-c", os.path.join(HERE, "bridge_check.cpp"), "-o", obj_b])
+    _run(["g++", "-Wl,--gc-sections", "-o", exe, obj_b, obj_rs, "-L", lib_dir, "-lsrs_amd",
+          "-Wl,-rpath,$ORIGIN/../../srs_amd", "-lm"])
+    for o in (obj_rs, obj_b):
+        os.remove(o)
+    return exe
+
+
 if __name__ == "__main__":
     build_oracle()
     build_ref()
+    build_bridge_check()
     sys.exit(0)

@@ -1914,8 +1914,13 @@ int srs_records_read(srs_ctx* c, int32_t r, float* out, int32_t max_records, int
     HIPCHK(hipMemcpy(tail.data(), d.rec + (size_t)n * d.rec_len, tail.size() * sizeof(float), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(d.rec, tail.data(), tail.size() * sizeof(float), hipMemcpyHostToDevice));
   }
-  lv.rec_count = rest;
-  return live_rw(c, r, &lv, true);
+  // Only the record count changes: a finished realization whose records
+  // are drained stays finished, so the sweep's tile list and the captured
+  // graphs remain valid (a full live-state write would restore the tile
+  // list and drop them).
+  const int32_t cnt = rest;
+  HIPCHK(hipMemcpy((char*)(c->d_live + r) + offsetof(RealLive, rec_count), &cnt, sizeof cnt, hipMemcpyHostToDevice));
+  return SRS_OK;
 }
 
 int srs_ctx_sync(srs_ctx* c) {

@@ -929,3 +929,29 @@ def test_irregular_plane_spacing_gating_run(compact):
     assert len(np.unique(pr.aij.reshape(-1, 5), axis=0)) > g.n_ele // 4  # the rows really vary
     assert np.array_equal(ch[0], Fo.ryr_state) and np.array_equal(ch[1], Fo.lcc_state)
     assert rel_err(X, Fo.X) <= TOL
+
+
+def test_integration_bridge_runs_in_the_reference_step_loop(tmp_path):
+    """The per-step bridge of the integration notes (SrsInit / SrsExplicitStep
+    / SrsPullStates over the reference's own SimData and ParamStruct,
+    cpu/parameters.h) stepped beside the reference's own ExplicitStep
+    (cpu/RunSimulation.cpp:591, compiled from the reference source into
+    oracle/_ref/bridge_check), with the RyR states of every step set from
+    one random schedule before the call: the fields agree bit for bit after
+    200 steps."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "bridge_check")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/bridge_check is not built (needs the reference sources at build time)")
+    base = str(tmp_path / "mesh")
+    S.box_mesh_write(W.tiny_box_spec(nx=5, ny=5, tt=True, ryr=3), base)
+    g, _ = S.grid_read(base)
+    aij, _ = S.assemble_operator(g)
+    dt = W.stable_dt(aij, 0.25)
+    xml = str(tmp_path / "p.xml")
+    _write_params(xml, dict(t_step=repr(dt * 1e6), V_RYR="5e-5", paramset_title="bridge"))
+    out = subprocess.run([exe, xml, base, "200", "11"], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0, (out.returncode, out.stdout, out.stderr)
+    assert " 0 differ" in out.stdout
