@@ -299,6 +299,10 @@ int srs_ctx_create(int device, srs_ctx** out);
                                     step's gating (and, with SRS_STEP_PROTOCOL, the loop body) inside the
                                     next step's sweep launch (register budget 0 or 6; 0 with the loop
                                     body); 0: a gating launch after every sweep */
+#define SRS_TUNE_SUBSET_MB 12    /* 0 (default): every step sweeps all realizations; > 0: srs_explicit_step
+                                    steps runs of realization groups whose state takes at most this many MB
+                                    through a whole block of steps (128 with SRS_STEP_GRAPH) each, one after
+                                    another, so that a run's state stays in the last-level cache */
 #define SRS_TUNE_ADJ_LEVELS 11   /* adjoint checkpoint levels (srs_linescan_loss_grad*): 0 (default) the
                                     fewest whose memory fits the sub-batch, 1..6 forced; each level below the
                                     first is one more forward pass and needs about (L+1) n^(1/(L+1)) states */

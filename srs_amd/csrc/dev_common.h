@@ -322,6 +322,16 @@ struct srs_ctx {
   // enough of them, the sweep runs over a list without them (d_tiles_live).
   std::vector<int2> tiles_all;
   int2* d_tiles_all = nullptr;
+  std::vector<int2> tiles_cur;    // host copy of the tile list the sweep runs over (d_tiles)
+  // Realization subsets (SRS_TUNE_SUBSET_MB): consecutive groups of the tile
+  // list whose state takes at most subset_mb MB, each stepped through a
+  // whole block of steps before the next, so that its state stays in the
+  // last-level cache between steps.
+  struct Subset { int t0, nt, r0, nr; };
+  int subset_mb = 0;
+  std::vector<Subset> subsets;
+  bool subsets_valid = false;
+  int32_t* d_sub_rids = nullptr;  // realization ids of the subsets, concatenated
   int2* d_tiles_live = nullptr;
   bool tiles_compacted = false;
   int32_t* d_groups = nullptr;  // [n_groups][group] realization ids, -1 pads
@@ -376,6 +386,8 @@ void launch_gated_block(srs_ctx* c, int64_t m, int cur, uint32_t gmode);
 int drop_graphs(srs_ctx* c);
 int tiles_full(srs_ctx* c);
 int tiles_compact(srs_ctx* c);
+int plan_subsets(srs_ctx* c);
+void launch_subset_block(srs_ctx* c, int64_t m, uint32_t gmode);
 int live_rw(srs_ctx* c, int32_t r, RealLive* lv, bool write);
 int slot_copy(srs_ctx* c, const RealDesc& d, double* dev, double* host, bool to_dev);
 

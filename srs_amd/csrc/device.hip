@@ -664,6 +664,11 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate(const RealDesc* __restric
                                                        uint32_t mode) {
   gate_workgroup<GATE_THREADS>(D, L, blockIdx.x, mode);
 }
+// The gating of the realizations of a list (one subset of the context).
+__global__ __launch_bounds__(GATE_THREADS) void k_gate_list(const RealDesc* __restrict__ D, RealLive* __restrict__ L,
+                                                            const int32_t* __restrict__ rids, uint32_t mode) {
+  gate_workgroup<GATE_THREADS>(D, L, rids[blockIdx.x], mode);
+}
 #endif
 
 // One step of a single-realization context with the previous step's gating
@@ -944,6 +949,76 @@ void launch_sweep(srs_ctx* c) {
                      c->d_tiles, c->n_tiles, (const int*)c->d_groups, single);
 }
 
+// m steps of every subset, one subset after another: the sweep over the
+// subset's run of tiles and the gating of its realizations, m times.
+// Realizations are independent and each one's Philox counters follow its own
+// step count, so the order changes no result.
+void launch_subset_block(srs_ctx* c, int64_t m, uint32_t gmode) {
+  for (const srs_ctx::Subset& sb : c->subsets)
+    for (int64_t k = 0; k < m; ++k) {
+      if (sb.nt > 0)
+        hipLaunchKernelGGL(pick_sweep(c), dim3((sb.nt + NXCD - 1) / NXCD * NXCD), dim3(c->tile), 0, c->stream,
+                           c->d_desc, c->d_live, c->d_tiles + sb.t0, sb.nt, (const int*)c->d_groups, 0);
+      hipLaunchKernelGGL(k_gate_list, dim3((unsigned)sb.nr), dim3(GATE_THREADS), 0, c->stream, c->d_desc, c->d_live,
+                         (const int32_t*)c->d_sub_rids + sb.r0, gmode);
+    }
+}
+
+// Splits the current tile list into subsets: runs of whole groups (in list
+// order) whose realizations' state buffers (two of 48 B per element each)
+// take at most subset_mb MB; realizations whose groups have no tile left
+// (all finished) are gated with the last subset, as k_gate would.
+int plan_subsets(srs_ctx* c) {
+  c->subsets.clear();
+  c->subsets_valid = true;
+  if (c->subset_mb <= 0 || c->reals.size() < 2) return SRS_OK;
+  const size_t budget = (size_t)c->subset_mb << 20;
+  const int KG = c->group;
+  std::vector<int32_t> rids;
+  std::vector<char> listed(c->reals.size(), 0);
+  srs_ctx::Subset cur{0, 0, 0, 0};
+  size_t bytes = 0;
+  const int n = (int)c->tiles_cur.size();
+  for (int t = 0; t < n;) {
+    const int g = c->tiles_cur[t].x;
+    int t1 = t;
+    while (t1 < n && c->tiles_cur[t1].x == g) ++t1;
+    size_t gb = 0;
+    for (int s = 0; s < KG; ++s) {
+      const int r = c->groups[(size_t)KG * g + s];
+      if (r >= 0) gb += 2 * 48 * (size_t)c->meshes[c->reals[r].mesh]->np;
+    }
+    if (cur.nt > 0 && bytes + gb > budget) {
+      c->subsets.push_back(cur);
+      cur = srs_ctx::Subset{t, 0, (int)rids.size(), 0};
+      bytes = 0;
+    }
+    for (int s = 0; s < KG; ++s) {
+      const int r = c->groups[(size_t)KG * g + s];
+      if (r >= 0 && !listed[r]) {
+        listed[r] = 1;
+        rids.push_back(r);
+        cur.nr++;
+      }
+    }
+    cur.nt += t1 - t;
+    bytes += gb;
+    t = t1;
+  }
+  for (size_t r = 0; r < c->reals.size(); ++r)
+    if (!listed[r]) {
+      rids.push_back((int32_t)r);
+      cur.nr++;
+    }
+  if (cur.nt > 0 || cur.nr > 0) c->subsets.push_back(cur);
+  if (c->subsets.size() < 2) {  // one subset is the whole context: the plain step is the same launches
+    c->subsets.clear();
+    return SRS_OK;
+  }
+  HIPCHK(hipMemcpy(c->d_sub_rids, rids.data(), rids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  return SRS_OK;
+}
+
 int launch_step(srs_ctx* c, uint32_t gmode) {
   launch_sweep(c);
   hipLaunchKernelGGL(k_gate, dim3((unsigned)c->reals.size()), dim3(GATE_THREADS), 0, c->stream, c->d_desc,
@@ -985,6 +1060,7 @@ void set_tiles(srs_ctx* c, int2* d, int n) {
   c->d_tiles = d;
   c->n_tiles = n;
   c->grid_sweep = (n + NXCD - 1) / NXCD * NXCD;
+  c->subsets_valid = false;
 }
 
 // The full tile list again, before the host changes live state (a finished
@@ -994,6 +1070,7 @@ int tiles_full(srs_ctx* c) {
   int rc = drop_graphs(c);
   if (rc) return rc;
   set_tiles(c, c->d_tiles_all, (int)c->tiles_all.size());
+  c->tiles_cur = c->tiles_all;
   c->tiles_compacted = false;
   return SRS_OK;
 }
@@ -1026,6 +1103,7 @@ int tiles_compact(srs_ctx* c) {
   if (rc) return rc;
   if (!t.empty()) HIPCHK(hipMemcpy(c->d_tiles_live, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice));
   set_tiles(c, c->d_tiles_live, (int)t.size());
+  c->tiles_cur = t;
   c->tiles_compacted = true;
   return SRS_OK;
 }
@@ -1147,6 +1225,11 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
     case SRS_TUNE_DROP_FINISHED:
       c->drop_finished = value != 0;
       return c->finalized ? tiles_full(c) : SRS_OK;
+    case SRS_TUNE_SUBSET_MB:
+      if (value < 0 || value > (1 << 20)) return srs::fail(SRS_E_ARG, "subset size must be 0..2^20 MB");
+      c->subset_mb = value;
+      c->subsets_valid = false;
+      return c->finalized ? drop_graphs(c) : SRS_OK;
     case SRS_TUNE_ADJ_LEVELS:
       if (value < 0 || value > 6) return srs::fail(SRS_E_ARG, "adjoint checkpoint levels must be 0..6");
       c->adj_levels = value;
@@ -1478,7 +1561,9 @@ int srs_ctx_finalize(srs_ctx* c, int32_t record_capacity) {
   if ((rc = dev_upload(c, &c->d_tiles, tiles))) return rc;
   if ((rc = dev_upload(c, &c->d_groups, groups))) return rc;
   c->tiles_all = tiles;
+  c->tiles_cur = tiles;
   c->d_tiles_all = c->d_tiles;
+  if (c->reals.size() > 1 && (rc = dev_alloc(c, &c->d_sub_rids, c->reals.size()))) return rc;
   c->groups = groups;
   if (c->reals.size() > 1 && (rc = dev_alloc(c, &c->d_tiles_live, tiles.size()))) return rc;
   // meshes' channel flags were updated by add_realization; refresh them
@@ -1707,7 +1792,7 @@ int srs_checkpoint_load(srs_ctx* c, int32_t r, const void* buf, size_t bytes) {
 // The captured block of `block` steps in gate mode `gmode` (made on first use
 // and cached; tiles_compact and tiles_full drop the cache).
 // Graph-cache key bits of the gated blocks (host only, never passed to a kernel).
-constexpr uint32_t G_KEY_GATED = 1u << 8, G_KEY_CUR1 = 1u << 9;
+constexpr uint32_t G_KEY_GATED = 1u << 8, G_KEY_CUR1 = 1u << 9, G_KEY_SUBSETS = 1u << 10;
 
 static int graph_cached(srs_ctx* c, int64_t block, uint32_t key_mode, const std::function<void()>& capture,
                         hipGraphExec_t* out) {
@@ -1728,6 +1813,8 @@ static int graph_cached(srs_ctx* c, int64_t block, uint32_t key_mode, const std:
 }
 
 static int graph_for(srs_ctx* c, int64_t block, uint32_t gmode, hipGraphExec_t* out) {
+  if (!c->subsets.empty())
+    return graph_cached(c, block, gmode | G_KEY_SUBSETS, [&] { launch_subset_block(c, block, gmode); }, out);
   return graph_cached(c, block, gmode, [&] { for (int64_t k = 0; k < block; ++k) launch_step(c, gmode); }, out);
 }
 
@@ -1767,6 +1854,7 @@ int srs_explicit_step_prepare(srs_ctx* c, int64_t n_steps, uint32_t flags) {
   const uint32_t gmode = G_AFTER_SWEEP | ((flags & SRS_STEP_PROTOCOL) ? G_PROTOCOL : 0u);
   int rc = tiles_compact(c);
   if (rc) return rc;
+  if (!c->subsets_valid && (rc = plan_subsets(c))) return rc;
   hipGraphExec_t ge;
   bool gated;
   int cur;
@@ -1794,6 +1882,7 @@ int srs_explicit_step(srs_ctx* c, int64_t n_steps, uint32_t flags) {
   const uint32_t gmode = G_AFTER_SWEEP | ((flags & SRS_STEP_PROTOCOL) ? G_PROTOCOL : 0u);
   int rc = tiles_compact(c);
   if (rc) return rc;
+  if (!c->subsets_valid && (rc = plan_subsets(c))) return rc;
   bool gated = false;
   int cur = 0;
   if (n_steps > 0 && (rc = gated_start(c, flags, &gated, &cur))) return rc;
@@ -1817,6 +1906,8 @@ int srs_explicit_step(srs_ctx* c, int64_t n_steps, uint32_t flags) {
   }
   if (gated) {
     if (done < n_steps) launch_gated_block(c, n_steps - done, cur, gmode);
+  } else if (!c->subsets.empty()) {
+    if (done < n_steps) launch_subset_block(c, n_steps - done, gmode);
   } else {
     for (; done < n_steps; ++done) launch_step(c, gmode);
   }

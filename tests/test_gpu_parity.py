@@ -955,3 +955,53 @@ def test_integration_bridge_runs_in_the_reference_step_loop(tmp_path):
     print(out.stdout, out.stderr, flush=True)
     assert out.returncode == 0, (out.returncode, out.stdout, out.stderr)
     assert " 0 differ" in out.stdout
+
+
+@pytest.mark.parametrize("group", [1, 2])
+def test_realization_subsets_equal_all_at_once(group):
+    """SRS_TUNE_SUBSET_MB: runs of realization groups stepped through whole
+    blocks of steps one after another (each run's state small enough to stay
+    in the last-level cache) give every realization the fields, channel
+    states, loop state and records of the all-at-once step, with protocol
+    runs that finish at different times (the tile list shrinks, the subsets
+    follow), graph blocks and single launches."""
+    preps = [_tiny(tt=True), W.prepare(W.tiny_box_spec(nx=5, ny=4), W.spark_params())]
+    dt = preps[0].params.reals[1]
+    n_real = 40
+    fields, params, prs = [], [], []
+    for k in range(n_real):
+        pr = preps[k % 2]
+        p = fast_gating(pr.params)
+        p.setr(T_FINAL=(150 + 9 * k) * dt, OUTPUT_STATES_INTERVAL=13 * dt)
+        p.seti(FLAG_RYR_OUT=1)
+        prs.append(pr)
+        params.append(p)
+        fields.append(initial_fields(pr, ryr_open=(k % pr.grid.n_ryr,), seed=400 + k))
+    runs = []
+    for mb in (0, 1):
+        ctx = S.Context(0)
+        ctx.tune(group=group)
+        mids = [ctx.add_mesh_grid(pr.grid, pr.aij, pr.nij) for pr in preps]
+        for k in range(n_real):
+            g = prs[k].grid
+            ctx.add_realization(mids[k % 2], g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, params[k], seed=90 + k,
+                                stream=k)
+        ctx.finalize(128)
+        ctx.tune(subset_mb=mb)
+        for k, F in enumerate(fields):
+            ctx.state_set(k, F.X)
+            ctx.channels_set(k, F.ryr_state, F.lcc_state, F.lccv_state, F.R)
+            ctx.globals_set(k, F.gv[0], F.gv[1])
+        for n in (200, 7, 150, 130):
+            ctx.explicit_step(n, S.STEP_PROTOCOL | S.STEP_GRAPH)
+        ctx.explicit_step(5, S.STEP_PROTOCOL)
+        runs.append([(ctx.state_get(k), ctx.channels_get(k), ctx.loop_get(k), ctx.records(k)) for k in range(n_real)])
+        ctx.close()
+    steps = set()
+    for a, b in zip(*runs):
+        assert np.array_equal(a[0], b[0])
+        assert all(np.array_equal(u, v) for u, v in zip(a[1], b[1]))
+        assert a[2] == b[2]
+        np.testing.assert_array_equal(a[3], b[3])
+        steps.add(a[2]["step"])
+    assert len(steps) > 10  # the runs ended at different steps
