@@ -505,6 +505,7 @@ def test_boundary_rates_follow_d_in_step_and_gradient():
     p = W.spark_params()
     dt = W.stable_dt(aij, 0.25)
     p.setr(T_STEP=dt, INIT_RYR_HOLD=dt, V_RYR=20 * p.reals[S.R["V_RYR"]])
+    p.seti(INIT_RYR=-1)
     X0, bnd = S.equilibrium_state(g, p.reals)
     bnd = bnd * np.array([3.0, 1.0, 1.0, 1.0, 1.0, 1.2])  # Ca and Fluo flow in through the bound faces
     n = 400
