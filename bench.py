@@ -44,6 +44,14 @@ def log(*a):
 BYTES_PER_ELEMENT = 161
 
 
+def host_cores():
+    """Cores this process may run on (what nproc prints)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def algorithmic_bytes(n_ele):
     """HBM bytes one explicit step must move per element: 6 FP64 species read
     and written (96 B), the ELL row of 4 int32 + 5 FP64 (56 B) and 9 B of
@@ -307,6 +315,36 @@ def bench_adjoint_batch(n_real=256, n_steps=256, n_points=64, n_samples=8, max_b
                 mean_grad_norm=float(np.linalg.norm(grad.mean(axis=0) * p.reals[idx])))
 
 
+def bench_adjoint_full(n_real=8, t_sim=20.0, n_points=64, n_samples=50):
+    """The batched fitting workload at its full length: n_real members of the
+    h = 30 nm dyad geometry, t_sim ms (392,418 steps at 20 ms), each with its
+    own recorded trajectory, loss = squared dF/F0 of a 64-point scan sampled
+    n_samples times (zero target), loss and all gradients in one call with
+    the library's checkpoint plan.  Measures the whole call; the cost of
+    more members is that of as many more sub-batches."""
+    from srs_amd import workloads as W
+
+    dt = float(W.prepare_dyad(h=0.03).params.reals[1])
+    n_steps = int(round(t_sim / dt))
+    t0 = time.perf_counter()
+    ctx, sid, scheds, samples, tgt, p, info = W.dyad_adjoint_batch(n_real, n_steps, n_points, n_samples, 0, log=log,
+                                                                   zero_target=True)
+    t_setup = time.perf_counter() - t0
+    log(f"adjoint full: {n_real} members x {n_steps} steps set up in {t_setup:.1f} s")
+    t0 = time.perf_counter()
+    L, grad = ctx.loss_grad_batch(range(n_real), sid, scheds, samples, tgt)
+    t_grad = time.perf_counter() - t0
+    plan = ctx.adjoint_plan()
+    ctx.close()
+    elem_steps = float(n_real) * info["n_ele"] * n_steps
+    log(f"adjoint full: loss and gradients in {t_grad:.1f} s, plan {plan}")
+    return dict(workload=f"dyad_box geometry at h=30 nm ({info['n_ele']} elements), {n_real} realizations, "
+                         f"{t_sim} ms = {n_steps} steps, {info['n_f']}-point line scan x {n_samples} samples",
+                n_real=n_real, n_steps=n_steps, loss_and_gradient_s=t_grad, setup_s=t_setup, checkpoint_plan=plan,
+                ps_per_element_step_loss_and_gradient=1e12 * t_grad / elem_steps,
+                s_for_256_members=t_grad * 256 / n_real, finite=bool(np.all(np.isfinite(grad))))
+
+
 def bench_fidelity(n_real=512, t_final=20.0, chunk=2000, group=4):
     """Spark-fidelity ensemble with the reference's Fidelity protocol
     (param/walker2014/Fidelity.xml: flag_fidelity, a run ends once 4 RyRs are
@@ -388,6 +426,8 @@ def main():
     ap.add_argument("--adjoint-batch-steps", type=int, default=256)
     ap.add_argument("--config-physics", type=int, default=1,
                     help="1: also time the plain workload with the configuration's jSR diffusion and SR sites")
+    ap.add_argument("--adjoint-full", type=int, default=0,
+                    help="members of a full-length (20 ms) batched loss-and-gradient run (0: skip; minutes)")
     ap.add_argument("--fidelity", type=int, default=0,
                     help="realizations in a spark-fidelity ensemble run to completion (0: skip)")
     ap.add_argument("--order", choices=("block", "rcm", "given"), default="block",
@@ -479,7 +519,10 @@ def main():
     if args.cpu_budget > 0:
         n, cwall, kind = bench_cpu(pr, args.cpu_budget)
         out["cpu_baseline"] = {"value": n * dt / cwall, "unit": "spark-ms simulated/s", "cores": 1, "kind": kind,
-                               "sample": f"{n} ExplicitStep calls with gating on the same {N}-element mesh"}
+                               "host_cores": host_cores(),
+                               "sample": f"{n} ExplicitStep calls with gating on the same {N}-element mesh "
+                                         f"(the reference's step is single-threaded; one core of the host's "
+                                         f"{host_cores()})"}
         log(f"cpu {kind}: {n} steps in {cwall:.2f} s")
     if args.batch > 0:
         b = bench_batched(args.batch, args.batch_steps, args.batch_steps, args.order,
@@ -515,6 +558,8 @@ def main():
         log(f"adjoint: {a['n_steps']} steps, forward {a['forward_s']:.2f} s, loss+gradient {a['loss_and_gradient_s']:.2f} s")
     if args.adjoint_batch > 0:
         out["adjoint_batch"] = bench_adjoint_batch(args.adjoint_batch, args.adjoint_batch_steps)
+    if args.adjoint_full > 0:
+        out["adjoint_full"] = bench_adjoint_full(args.adjoint_full)
     if args.fidelity > 0:
         out["fidelity"] = bench_fidelity(args.fidelity)
     if trials:

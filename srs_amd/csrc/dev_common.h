@@ -302,6 +302,7 @@ struct srs_ctx {
   int lds = 1;           // stage each tile's quads in LDS for the neighbour gathers (groups of 1)
   int drop_finished = 1; // sweep without the tiles of groups that have all finished
   int gated = 1;         // fold each step's gating into the next step's sweep (k_sweep_gated) where it applies
+  int alt_order = 0;      // single-realization gated sweep: tiles in reverse order on alternate steps
   int gpu_layout = 0;     // states records in the GPU program's layout (Ca_cyt at each RyR appended)
   int adj_levels = 0;     // adjoint checkpoint levels, 0: the fewest that fit the sub-batch
   int adj_last_levels = 0, adj_last_batch = 0;  // the plan of the last loss/gradient call

@@ -685,11 +685,17 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate_list(const RealDesc* __re
 // again.  A block starts only outside the ffwd phase, which the loop body
 // can leave but not enter, so every step of it flips the live index.
 // gmode 0 skips the gating (the first step of a block).
+// curx: the live state index in bit 0; bit 1 set sweeps the tiles in the
+// reverse order when the index is 1 (alternate steps), so that a step first
+// reads the tiles the step before wrote last, while they may still be in the
+// last-level cache.  The order changes no result.
 template <int BS, bool NTG, bool NTS, bool COMPACT, bool LEAN, bool PROTO>
 __device__ __forceinline__ void sweep_gated(const RealDesc* __restrict__ D, RealLive* __restrict__ L, int n_tiles,
-                                            int cur, uint32_t gmode) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
+                                            int curx, uint32_t gmode) {
+  const int cur = curx & 1;
+  int b = xcd_tile(blockIdx.x, gridDim.x);
   if (b > n_tiles) return;  // b = 0: the gating workgroup; b = 1..n_tiles: tile b - 1
+  if (b > 0 && curx == 3) b = n_tiles + 1 - b;
   const RealDesc& d = D[0];
   const RealLive& l = L[0];
   const int skip = l.done | l.ffwd;
@@ -1033,7 +1039,8 @@ void launch_gated(srs_ctx* c, int cur, uint32_t gmode, bool proto) {
   const GatedFn f = c->tile == 128 ? pick_gated_bs<128>(c, proto)
                     : c->tile == 512 ? pick_gated_bs<512>(c, proto) : pick_gated_bs<256>(c, proto);
   const int grid = (c->n_tiles + 1 + NXCD - 1) / NXCD * NXCD;
-  hipLaunchKernelGGL(f, dim3(grid), dim3(c->tile), 0, c->stream, c->d_desc, c->d_live, c->n_tiles, cur, gmode);
+  hipLaunchKernelGGL(f, dim3(grid), dim3(c->tile), 0, c->stream, c->d_desc, c->d_live, c->n_tiles,
+                     cur | (c->alt_order ? 2 : 0), gmode);
 }
 
 // m steps through k_sweep_gated: the first without gating (nothing is
@@ -1225,6 +1232,9 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
     case SRS_TUNE_DROP_FINISHED:
       c->drop_finished = value != 0;
       return c->finalized ? tiles_full(c) : SRS_OK;
+    case SRS_TUNE_ALT_ORDER:
+      c->alt_order = value != 0;
+      return c->finalized ? drop_graphs(c) : SRS_OK;
     case SRS_TUNE_SUBSET_MB:
       if (value < 0 || value > (1 << 20)) return srs::fail(SRS_E_ARG, "subset size must be 0..2^20 MB");
       c->subset_mb = value;

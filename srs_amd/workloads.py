@@ -450,13 +450,16 @@ def spark_adjoint_problem(t_sim: float = 5.0, n_points: int = 64, n_samples: int
 
 
 def dyad_adjoint_batch(n_real: int = 256, n_steps: int = 256, n_points: int = 64, n_samples: int = 8,
-                       max_batch: int = 0, h: float = 0.03, seed: int = 7, perturb: float = 0.1, log=None):
+                       max_batch: int = 0, h: float = 0.03, seed: int = 7, perturb: float = 0.1, log=None,
+                       zero_target: bool = False):
     """The batched fitting workload: n_real realizations on the dyad-box
     geometry at h = 30 nm (927,816 elements), each with its own RyR trajectory
     (Philox stream r) recorded by one gated run of all of them and then
     frozen; a line scan of n_points along x through the cluster centre sampled
     n_samples times; each member's target made by its own frozen-channel run
-    with theta moved by +-perturb (signs drawn from `seed`).  Returns the
+    with theta moved by +-perturb (signs drawn from `seed`), or zero
+    (zero_target: the loss is then the squared dF/F0, and no forward run is
+    spent on targets).  Returns the
     context, the scan id, the schedules, the sample steps, the targets, the
     parameters and a dict of sizes."""
     from . import R, adjoint_theta, read_scan_parameters
@@ -498,14 +501,17 @@ def dyad_adjoint_batch(n_real: int = 256, n_steps: int = 256, n_points: int = 64
     sgn = np.where(np.random.default_rng(seed).random(len(idx)) < 0.5, -1.0, 1.0)
     Pt = p.reals.copy()
     Pt[idx] *= 1 + perturb * sgn
-    for r in range(n_real):
-        ctx.params_set(r, Pt)
-    if log:
-        log("  targets")
     zero = np.zeros((n_real, n_samples, nf))
-    _, tgt = ctx.loss_grad_batch(range(n_real), sid, scheds, samples, zero, grad=False, scans=True,
-                                 max_batch=max_batch)
-    for r in range(n_real):
-        ctx.params_set(r, p.reals)
+    if zero_target:
+        tgt = zero
+    else:
+        for r in range(n_real):
+            ctx.params_set(r, Pt)
+        if log:
+            log("  targets")
+        _, tgt = ctx.loss_grad_batch(range(n_real), sid, scheds, samples, zero, grad=False, scans=True,
+                                     max_batch=max_batch)
+        for r in range(n_real):
+            ctx.params_set(r, p.reals)
     return ctx, sid, scheds, samples, tgt, p, dict(n_ele=g.n_ele, n_f=nf, dt=float(p.reals[R["T_STEP"]]),
                                                    theta_true=Pt[idx])

@@ -754,7 +754,8 @@ def test_finished_realizations_leave_the_sweep(group):
                                  dict(lds=1, tile=128, compact=0, nt_state=1), dict(lds=1, tile=512, nt_geometry=1),
                                  dict(lds=1, tile=512, occupancy=6), dict(lds=1, tile=256, occupancy=8),
                                  dict(tile=256, occupancy=6), dict(group=2, occupancy=6),
-                                 dict(tile=256, nt_geometry=0, lds=0)])
+                                 dict(tile=256, nt_geometry=0, lds=0), dict(lds=1, tile=512, occupancy=6, alt_order=1),
+                                 dict(lds=1, tile=128, alt_order=1)])
 def test_tunables_do_not_change_results(cfg):
     pr = _tiny(tt=True)
     p = fast_gating(pr.params)
@@ -767,18 +768,20 @@ def test_tunables_do_not_change_results(cfg):
         g = pr.grid
         ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=4, stream=0)
         ctx.finalize()
-        ctx.tune(**{k: v for k, v in tune.items() if k in ("nt_geometry", "nt_state", "occupancy", "lds")})
+        ctx.tune(**{k: v for k, v in tune.items() if k in ("nt_geometry", "nt_state", "occupancy", "lds", "alt_order")})
         ctx.state_set(0, F.X)
         ctx.channels_set(0, F.ryr_state, F.lcc_state, F.lccv_state, F.R)
         ctx.globals_set(0, F.gv[0], F.gv[1])
         ctx.explicit_step(200, S.STEP_GRAPH)
+        ctx.explicit_step(13, S.STEP_PROTOCOL)
         outs.append((ctx.state_get(0), ctx.channels_get(0)[0]))
         ctx.close()
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
 
-@pytest.mark.parametrize("cfg", [dict(tile=512), dict(tile=256, occupancy=6), dict(tile=128, nt_state=1, nt_geometry=0)])
+@pytest.mark.parametrize("cfg", [dict(tile=512), dict(tile=256, occupancy=6), dict(tile=128, nt_state=1, nt_geometry=0),
+                                 dict(tile=128, occupancy=6, alt_order=1)])
 def test_gated_sweep_equals_separate_gating(cfg):
     """Each step's gating run inside the next step's sweep launch
     (k_sweep_gated) against a gating launch after every sweep: identical
