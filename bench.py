@@ -345,6 +345,50 @@ def bench_adjoint_full(n_real=8, t_sim=20.0, n_points=64, n_samples=50):
                 s_for_256_members=t_grad * 256 / n_real, finite=bool(np.all(np.isfinite(grad))))
 
 
+def bench_size_scan(steps=400):
+    """Cost per element-step against working-set size: the plain path on the
+    dyad box at h = 40/30/25/20 nm (tuned sweep, block order), the ensemble
+    with 8..256 members stepped together, and the 2048-member ensemble with
+    realization subsets of 0/64/128/192 MB stepped block by block."""
+    import srs_amd as S
+    from srs_amd import workloads as W
+
+    out = []
+    tune = dict(tile=512, compact=1, nt_geometry=1, nt_state=0, occupancy=6, lds=1)
+    for h in (0.04, 0.03, 0.025, 0.02):
+        pr = W.prepare_dyad(h=h)
+        ctx = setup_plain(pr, "block", tune)
+        ctx.explicit_step(50, S.STEP_GRAPH)
+        ctx.prepare(steps, S.STEP_GRAPH)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.explicit_step(steps, S.STEP_GRAPH)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        ctx.close()
+        n = pr.grid.n_ele
+        out.append(dict(workload=f"dyad h={h}", n_elements=n, state_mb=96 * n / 1e6, us_per_step=1e6 * wall / steps,
+                        ps_per_element_step=1e12 * wall / steps / n))
+        log(json.dumps(out[-1]))
+    btune = dict(tile=256, compact=1, group=1, nt_geometry=0, nt_state=1, occupancy=6, lds=1)
+    for k, mb in ((8, 0), (16, 0), (64, 0), (256, 0), (2048, 0), (2048, 64), (2048, 128), (2048, 192)):
+        ctx, _, n_ele, _ = W.ensemble_context(k, tune=btune)
+        ctx.tune(subset_mb=mb)
+        m = 128
+        ctx.explicit_step(16, S.STEP_GRAPH)
+        ctx.prepare(m, S.STEP_GRAPH)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.explicit_step(m, S.STEP_GRAPH)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        ctx.close()
+        out.append(dict(workload=f"ensemble {k} members, subsets {mb} MB", n_elements=n_ele, state_mb=96 * n_ele / 1e6,
+                        us_per_step=1e6 * wall / m, ps_per_element_step=1e12 * wall / m / n_ele))
+        log(json.dumps(out[-1]))
+    return out
+
+
 def bench_fidelity(n_real=512, t_final=20.0, chunk=2000, group=4):
     """Spark-fidelity ensemble with the reference's Fidelity protocol
     (param/walker2014/Fidelity.xml: flag_fidelity, a run ends once 4 RyRs are
@@ -426,6 +470,8 @@ def main():
     ap.add_argument("--adjoint-batch-steps", type=int, default=256)
     ap.add_argument("--config-physics", type=int, default=1,
                     help="1: also time the plain workload with the configuration's jSR diffusion and SR sites")
+    ap.add_argument("--size-scan", action="store_true",
+                    help="also scan the sweep's cost per element-step against working-set size (minutes)")
     ap.add_argument("--adjoint-full", type=int, default=0,
                     help="members of a full-length (20 ms) batched loss-and-gradient run (0: skip; minutes)")
     ap.add_argument("--fidelity", type=int, default=0,
@@ -558,6 +604,8 @@ def main():
         log(f"adjoint: {a['n_steps']} steps, forward {a['forward_s']:.2f} s, loss+gradient {a['loss_and_gradient_s']:.2f} s")
     if args.adjoint_batch > 0:
         out["adjoint_batch"] = bench_adjoint_batch(args.adjoint_batch, args.adjoint_batch_steps)
+    if args.size_scan:
+        out["size_scan"] = bench_size_scan()
     if args.adjoint_full > 0:
         out["adjoint_full"] = bench_adjoint_full(args.adjoint_full)
     if args.fidelity > 0:

@@ -306,6 +306,8 @@ int srs_ctx_create(int device, srs_ctx** out);
 #define SRS_TUNE_ALT_ORDER 13    /* 1: the single-realization step (SRS_TUNE_GATED_SWEEP) sweeps its tiles
                                     in reverse order on alternate steps, reading first what the step before
                                     wrote last; 0 (default): one order */
+#define SRS_TUNE_ADJ_WAVES 14    /* adjoint reverse step's register budget: 0 (default) the compiler's, 4: at
+                                    most 128 VGPRs so that four wavefronts share each SIMD */
 #define SRS_TUNE_ADJ_LEVELS 11   /* adjoint checkpoint levels (srs_linescan_loss_grad*): 0 (default) the
                                     fewest whose memory fits the sub-batch, 1..6 forced; each level below the
                                     first is one more forward pass and needs about (L+1) n^(1/(L+1)) states */

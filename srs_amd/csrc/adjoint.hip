@@ -95,11 +95,10 @@ struct AdjNext {
 
 // Pass 2: lambda_n and the theta-gradient partials of one step.
 template <bool COMPACT, bool SCAN>
-__global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
-                                                     const int32_t* __restrict__ rid, const double* __restrict__ X,
-                                                     const double* __restrict__ lam, const d4* __restrict__ w,
-                                                     int64_t step, double* __restrict__ lam_out,
-                                                     double* __restrict__ gpart, AdjNext nx) {
+__device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                                         const int32_t* __restrict__ rid, const double* __restrict__ X,
+                                         const double* __restrict__ lam, const d4* __restrict__ w, int64_t step,
+                                         double* __restrict__ lam_out, double* __restrict__ gpart, AdjNext nx) {
   __shared__ double red[ADJ_BS / 64][ADJ_NT];
   const int r = rid[blockIdx.y];
   const RealDesc& d = D[r];
@@ -273,6 +272,24 @@ __global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict_
     for (int q = 0; q < ADJ_BS / 64; ++q) s += red[q][threadIdx.x];
     gpart[(size_t)blockIdx.x * ADJ_NT + threadIdx.x] = gp_old + s;
   }
+}
+
+// The compiler's register budget (144 VGPRs: three wavefronts per SIMD), or
+// at most 128 (four) with a few spills.
+template <bool COMPACT, bool SCAN>
+__global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
+                                                     const int32_t* __restrict__ rid, const double* __restrict__ X,
+                                                     const double* __restrict__ lam, const d4* __restrict__ w,
+                                                     int64_t step, double* __restrict__ lam_out,
+                                                     double* __restrict__ gpart, AdjNext nx) {
+  adj_step<COMPACT, SCAN>(D, L, rid, X, lam, w, step, lam_out, gpart, nx);
+}
+template <bool COMPACT, bool SCAN>
+__global__ __launch_bounds__(ADJ_BS) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_step_w4(
+    const RealDesc* __restrict__ D, const RealLive* __restrict__ L, const int32_t* __restrict__ rid,
+    const double* __restrict__ X, const double* __restrict__ lam, const d4* __restrict__ w, int64_t step,
+    double* __restrict__ lam_out, double* __restrict__ gpart, AdjNext nx) {
+  adj_step<COMPACT, SCAN>(D, L, rid, X, lam, w, step, lam_out, gpart, nx);
 }
 
 // The loss gradient of one scan sample in the Fluo adjoint:
@@ -740,8 +757,11 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
           AdjNext nx{n > s0 ? Xn - nb * n6 : nullptr, wnext, S.d_cen, scan ? dg + (size_t)ks * nf : nullptr,
                      S.geo, nf, n_samples * nf};
           if (scan) --ks;
-          auto kern = c->compact_all ? (scan ? k_adj_step<true, true> : k_adj_step<true, false>)
-                                     : (scan ? k_adj_step<false, true> : k_adj_step<false, false>);
+          auto kern = c->adj_waves == 4
+                          ? (c->compact_all ? (scan ? k_adj_step_w4<true, true> : k_adj_step_w4<true, false>)
+                                            : (scan ? k_adj_step_w4<false, true> : k_adj_step_w4<false, false>))
+                          : (c->compact_all ? (scan ? k_adj_step<true, true> : k_adj_step<true, false>)
+                                            : (scan ? k_adj_step<false, true> : k_adj_step<false, false>));
           hipLaunchKernelGGL(kern, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, d_rid, Xn, lam, wcur, n,
                              lout, gpart, nx);
           std::swap(lam, lout);

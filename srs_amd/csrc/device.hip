@@ -1232,6 +1232,10 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
     case SRS_TUNE_DROP_FINISHED:
       c->drop_finished = value != 0;
       return c->finalized ? tiles_full(c) : SRS_OK;
+    case SRS_TUNE_ADJ_WAVES:
+      if (value != 0 && value != 4) return srs::fail(SRS_E_ARG, "adjoint register budget must be 0 or 4");
+      c->adj_waves = value;
+      return SRS_OK;
     case SRS_TUNE_ALT_ORDER:
       c->alt_order = value != 0;
       return c->finalized ? drop_graphs(c) : SRS_OK;

@@ -631,3 +631,22 @@ def test_dyad30_batch_gradients_at_full_batch_match_central_differences():
         assert scale > 0
         err = np.abs(sens - sens_fd) / np.maximum(np.abs(sens_fd), 1e-3 * scale)
         assert err.max() <= 1e-5, (m, err, sens, sens_fd)
+
+
+def test_adjoint_register_budget_does_not_change_the_gradient():
+    """SRS_TUNE_ADJ_WAVES 4 (the reverse step limited to 128 VGPRs) gives
+    the compiler's-budget losses and gradients bit for bit, batched and
+    single."""
+    pr, members, samples = _batch_setup()
+    tg, _ = _targets(pr, members, samples)
+    ctx, sid, nf, _ = _batch_ctx(pr, members)
+    n = len(members)
+    scheds = [m[2] for m in members]
+    out = []
+    for waves in (0, 4):
+        ctx.tune(adj_waves=waves)
+        out.append(ctx.loss_grad_batch(range(n), sid, scheds, samples, tg, ckpt=31))
+        out.append(ctx.loss_grad(2, sid, scheds[2], samples, tg[2], 64))
+    ctx.close()
+    assert np.array_equal(out[0][0], out[2][0]) and np.array_equal(out[0][1], out[2][1])
+    assert out[1][0] == out[3][0] and np.array_equal(out[1][1], out[3][1])
