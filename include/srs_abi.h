@@ -435,6 +435,15 @@ int srs_channel_gating(srs_ctx* ctx);
  * states of realization `real` overwritten before step k by
  * schedule[k*n_ryr + i] (a recorded open/closed schedule). */
 int srs_explicit_step_replay(srs_ctx* ctx, int32_t real, int64_t n_steps, const int32_t* schedule);
+/* srs_explicit_step_replay without any device allocation, for callers that
+ * cannot allocate (code under graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The schedule is staged
+ * in a fixed device buffer of 2,097,152 RyR states (8 MiB) that each GPU holds
+ * from its first srs_ctx_create until the process ends; n_steps x n_ryr above
+ * that is SRS_E_ARG.  Every call on one GPU shares that buffer (it keeps the
+ * last call's schedule): calls that overlap on one GPU, from several threads
+ * or contexts, race.  Results equal srs_explicit_step_replay's bit for bit. */
+int srs_explicit_step_replay_static(srs_ctx* ctx, int32_t real, int64_t n_steps, const int32_t* schedule);
 
 /* Ca at the channel elements (for the states record on the host). */
 int srs_channel_ca(srs_ctx* ctx, int32_t real, double* ca_ryr_cyt, double* ca_ryr_jsr, double* ca_lcc);

@@ -126,6 +126,7 @@ _PROTOS = {
     "srs_explicit_step": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32]),
     "srs_channel_gating": (C.c_int, [C.c_void_p]),
     "srs_explicit_step_replay": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, i32p]),
+    "srs_explicit_step_replay_static": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, i32p]),
     "srs_channel_ca": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p, f64p]),
     "srs_records_read": (C.c_int, [C.c_void_p, C.c_int32, f32p, C.c_int32, i32p, i32p]),
     "srs_ctx_sync": (C.c_int, [C.c_void_p]),

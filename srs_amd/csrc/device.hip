@@ -28,6 +28,7 @@
 #endif
 
 #include <functional>
+#include <mutex>
 
 namespace srs_dev {
 
@@ -1169,6 +1170,36 @@ int live_rw(srs_ctx* c, int32_t r, RealLive* lv, bool write) {
 }  // namespace srs_dev
 
 #if SRS_UNIT == 0
+// Replay schedules of srs_explicit_step_replay_static: file-scope device
+// memory that every GPU the library runs on holds until the process ends
+// (8 MiB: 2,097,152 RyR states, steps x RyRs).  In host code `scratch` names
+// a host-side stand-in; the device address of each GPU's copy comes from
+// hipGetSymbolAddress with that GPU current and is the only form used.
+constexpr size_t kStaticSchedInts = (size_t)1 << 21;
+static __device__ __align__(256) int32_t scratch[kStaticSchedInts];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current (srs_ctx_create sets it).
+int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return srs::fail(SRS_E_ARG, "device %d beyond %d", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return SRS_OK;
+  void* p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = (int32_t*)p;
+  return SRS_OK;
+}
+
+int32_t* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
+
 extern "C" {
 
 int srs_ctx_create(int device, srs_ctx** out) {
@@ -1183,6 +1214,8 @@ int srs_ctx_create(int device, srs_ctx** out) {
   if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return srs::fail(SRS_E_NODEV, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
   HIPCHK(hipSetDevice(device));
+  int rc = scratch_setup(device);
+  if (rc) return rc;
   auto c = new srs_ctx();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1945,22 +1978,12 @@ int srs_channel_gating(srs_ctx* c) {
   return SRS_OK;
 }
 
-int srs_explicit_step_replay(srs_ctx* c, int32_t r, int64_t n_steps, const int32_t* schedule) {
-  int rc = check_real(c, r);
-  if (rc) return rc;
-  if (c->reals.size() != 1) return srs::fail(SRS_E_ARG, "replay needs a single-realization context");
+// n_steps replayed steps with the schedule already in device memory at ds
+// (null when the realization has no RyRs).
+static int replay_from(srs_ctx* c, int32_t r, int64_t n_steps, const int32_t* schedule, int32_t* ds) {
   HostReal& hr = c->reals[r];
   const int nr = (int)hr.ryr_ele.size();
-  if (n_steps <= 0) return SRS_OK;
-  if (!schedule && nr) return srs::fail(SRS_E_ARG, "null schedule");
-  HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  int32_t* ds = nullptr;
-  if (nr) {
-    HIPCHK(hipMalloc(&ds, (size_t)n_steps * nr * sizeof(int32_t)));
-    HIPCHK(hipMemcpy(ds, schedule, (size_t)n_steps * nr * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(hr.desc.ryr_state, schedule, nr * sizeof(int32_t), hipMemcpyHostToDevice));
-  }
+  if (nr) HIPCHK(hipMemcpy(hr.desc.ryr_state, schedule, nr * sizeof(int32_t), hipMemcpyHostToDevice));
   RealDesc d = hr.desc;
   d.sched = ds;
   d.n_sched = n_steps;
@@ -1974,9 +1997,52 @@ int srs_explicit_step_replay(srs_ctx* c, int32_t r, int64_t n_steps, const int32
   hipError_t e = hipGetLastError();
   (void)hipStreamSynchronize(c->stream);
   HIPCHK(hipMemcpy(c->d_desc + r, &hr.desc, sizeof(hr.desc), hipMemcpyHostToDevice));
-  if (ds) (void)hipFree(ds);
   if (e != hipSuccess) return srs::fail(SRS_E_HIP, "replay launch: %s", hipGetErrorString(e));
   return SRS_OK;
+}
+
+static int replay_check(srs_ctx* c, int32_t r, int64_t n_steps, const int32_t* schedule) {
+  int rc = check_real(c, r);
+  if (rc) return rc;
+  if (c->reals.size() != 1) return srs::fail(SRS_E_ARG, "replay needs a single-realization context");
+  if (n_steps < 0) return srs::fail(SRS_E_ARG, "negative step count");
+  if (!schedule && n_steps > 0 && !c->reals[r].ryr_ele.empty()) return srs::fail(SRS_E_ARG, "null schedule");
+  return SRS_OK;
+}
+
+int srs_explicit_step_replay(srs_ctx* c, int32_t r, int64_t n_steps, const int32_t* schedule) {
+  int rc = replay_check(c, r, n_steps, schedule);
+  if (rc || n_steps == 0) return rc;
+  const int nr = (int)c->reals[r].ryr_ele.size();
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int32_t* ds = nullptr;
+  if (nr) {
+    HIPCHK(hipMalloc(&ds, (size_t)n_steps * nr * sizeof(int32_t)));
+    hipError_t e = hipMemcpy(ds, schedule, (size_t)n_steps * nr * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(ds);
+      return srs::fail(SRS_E_HIP, "replay schedule copy: %s", hipGetErrorString(e));
+    }
+  }
+  rc = replay_from(c, r, n_steps, schedule, ds);
+  if (ds) (void)hipFree(ds);
+  return rc;
+}
+
+int srs_explicit_step_replay_static(srs_ctx* c, int32_t r, int64_t n_steps, const int32_t* schedule) {
+  int rc = replay_check(c, r, n_steps, schedule);
+  if (rc || n_steps == 0) return rc;
+  const size_t nr = c->reals[r].ryr_ele.size();
+  if (nr && (size_t)n_steps > kStaticSchedInts / nr)
+    return srs::fail(SRS_E_ARG, "replay of %lld steps x %zu RyRs exceeds the static schedule buffer of %zu RyR states",
+                     (long long)n_steps, nr, kStaticSchedInts);
+  int32_t* ds = scratch_ptr(c->device);
+  if (!ds) return srs::fail(SRS_E_STATE, "no static schedule buffer on device %d", c->device);
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (nr) HIPCHK(hipMemcpy(ds, schedule, (size_t)n_steps * nr * sizeof(int32_t), hipMemcpyHostToDevice));
+  return replay_from(c, r, n_steps, schedule, nr ? ds : nullptr);
 }
 
 int srs_channel_ca(srs_ctx* c, int32_t r, double* ca_ryr_cyt, double* ca_ryr_jsr, double* ca_lcc) {
