@@ -245,7 +245,7 @@ def bench_batched(n_real, steps, warmup, order, tune=None, n_tune=512):
                 plain_tunes=[ptunes[m] for m in range(3)], trials=trials)
 
 
-def bench_adjoint(t_sim=5.0, n_points=64, n_samples=50, ckpt=256):
+def bench_adjoint(t_sim=5.0, n_points=64, n_samples=50, ckpt=256, adj_waves=0):
     """Adjoint workload: the h = 40 nm spark box, t_sim ms with the RyR
     schedule frozen from one gated run, a 64-point line scan along x through
     the cluster centre sampled 50 times, target from theta moved by +-10%.
@@ -255,6 +255,7 @@ def bench_adjoint(t_sim=5.0, n_points=64, n_samples=50, ckpt=256):
 
     t0 = time.perf_counter()
     prob, info = W.spark_adjoint_problem(t_sim, n_points, n_samples, ckpt)
+    prob.ctx.tune(adj_waves=adj_waves)
     t_setup = time.perf_counter() - t0
     th = prob.theta()
     t0 = time.perf_counter()
@@ -274,7 +275,7 @@ def bench_adjoint(t_sim=5.0, n_points=64, n_samples=50, ckpt=256):
                 grad_norm=float(np.linalg.norm(grad * th)), loss_at_target_parameters=info["loss_at_target"])
 
 
-def bench_adjoint_batch(n_real=256, n_steps=256, n_points=64, n_samples=8, max_batch=128):
+def bench_adjoint_batch(n_real=256, n_steps=256, n_points=64, n_samples=8, max_batch=128, adj_waves=0):
     """Batched adjoint workload: n_real realizations on the dyad-box geometry
     at h = 30 nm, each with its own RyR trajectory (Philox stream r) recorded
     by one gated run of all of them, frozen for the loss; a 64-point line
@@ -287,6 +288,7 @@ def bench_adjoint_batch(n_real=256, n_steps=256, n_points=64, n_samples=8, max_b
     t0 = time.perf_counter()
     ctx, sid, scheds, samples, tgt, p, info = W.dyad_adjoint_batch(n_real, n_steps, n_points, n_samples, max_batch,
                                                                    log=log)
+    ctx.tune(adj_waves=adj_waves)
     t_setup = time.perf_counter() - t0
     idx = __import__("srs_amd").adjoint_theta()
     t0 = time.perf_counter()
@@ -315,13 +317,13 @@ def bench_adjoint_batch(n_real=256, n_steps=256, n_points=64, n_samples=8, max_b
                 mean_grad_norm=float(np.linalg.norm(grad.mean(axis=0) * p.reals[idx])))
 
 
-def bench_adjoint_full(n_real=8, t_sim=20.0, n_points=64, n_samples=50):
+def bench_adjoint_full(n_real=8, t_sim=20.0, n_points=64, n_samples=50, per_call=2, adj_waves=0):
     """The batched fitting workload at its full length: n_real members of the
     h = 30 nm dyad geometry, t_sim ms (392,418 steps at 20 ms), each with its
     own recorded trajectory, loss = squared dF/F0 of a 64-point scan sampled
     n_samples times (zero target), loss and all gradients in one call with
-    the library's checkpoint plan.  Measures the whole call; the cost of
-    more members is that of as many more sub-batches."""
+    the library's checkpoint plan, per_call members per call.  The cost of
+    more members is that of as many more calls."""
     from srs_amd import workloads as W
 
     dt = float(W.prepare_dyad(h=0.03).params.reals[1])
@@ -329,12 +331,23 @@ def bench_adjoint_full(n_real=8, t_sim=20.0, n_points=64, n_samples=50):
     t0 = time.perf_counter()
     ctx, sid, scheds, samples, tgt, p, info = W.dyad_adjoint_batch(n_real, n_steps, n_points, n_samples, 0, log=log,
                                                                    zero_target=True)
+    ctx.tune(adj_waves=adj_waves)
     t_setup = time.perf_counter() - t0
     log(f"adjoint full: {n_real} members x {n_steps} steps set up in {t_setup:.1f} s")
-    t0 = time.perf_counter()
-    L, grad = ctx.loss_grad_batch(range(n_real), sid, scheds, samples, tgt)
-    t_grad = time.perf_counter() - t0
-    plan = ctx.adjoint_plan()
+    # one call per `per_call` members (each member's result does not depend on
+    # which others share its call), a line after each
+    t_grad, grads, plans = 0.0, [], []
+    for m0 in range(0, n_real, per_call):
+        ids = list(range(m0, min(n_real, m0 + per_call)))
+        t0 = time.perf_counter()
+        _, g = ctx.loss_grad_batch(ids, sid, [scheds[m] for m in ids], samples, tgt[ids])
+        dt_call = time.perf_counter() - t0
+        t_grad += dt_call
+        grads.append(g)
+        plans.append(ctx.adjoint_plan())
+        log(f"adjoint full: members {ids[0]}-{ids[-1]} in {dt_call:.1f} s, plan {plans[-1]}")
+    grad = np.concatenate(grads)
+    plan = plans[0]
     ctx.close()
     elem_steps = float(n_real) * info["n_ele"] * n_steps
     log(f"adjoint full: loss and gradients in {t_grad:.1f} s, plan {plan}")
@@ -472,6 +485,8 @@ def main():
                     help="1: also time the plain workload with the configuration's jSR diffusion and SR sites")
     ap.add_argument("--size-scan", action="store_true",
                     help="also scan the sweep's cost per element-step against working-set size (minutes)")
+    ap.add_argument("--adj-waves", type=int, default=0, choices=(0, 4),
+                    help="register budget of the adjoint reverse step (SRS_TUNE_ADJ_WAVES) in the adjoint legs")
     ap.add_argument("--adjoint-full", type=int, default=0,
                     help="members of a full-length (20 ms) batched loss-and-gradient run (0: skip; minutes)")
     ap.add_argument("--fidelity", type=int, default=0,
@@ -599,15 +614,16 @@ def main():
             out["batched"]["speedup_vs_cpu_ensemble"] = brate / ce["value"]
             log(f"cpu ensemble ({ce['kind']}, {ce['cores']} processes): {ce['value']:.4f} spark-ms/s")
     if args.adjoint:
-        a = bench_adjoint()
+        a = bench_adjoint(adj_waves=args.adj_waves)
         out["adjoint"] = a
         log(f"adjoint: {a['n_steps']} steps, forward {a['forward_s']:.2f} s, loss+gradient {a['loss_and_gradient_s']:.2f} s")
     if args.adjoint_batch > 0:
-        out["adjoint_batch"] = bench_adjoint_batch(args.adjoint_batch, args.adjoint_batch_steps)
+        out["adjoint_batch"] = bench_adjoint_batch(args.adjoint_batch, args.adjoint_batch_steps,
+                                                   adj_waves=args.adj_waves)
     if args.size_scan:
         out["size_scan"] = bench_size_scan()
     if args.adjoint_full > 0:
-        out["adjoint_full"] = bench_adjoint_full(args.adjoint_full)
+        out["adjoint_full"] = bench_adjoint_full(args.adjoint_full, adj_waves=args.adj_waves)
     if args.fidelity > 0:
         out["fidelity"] = bench_fidelity(args.fidelity)
     if trials:
