@@ -47,7 +47,11 @@ __device__ __forceinline__ double scan_adj_sum(const double* __restrict__ cen, c
                                                const double* __restrict__ gcoef, size_t np, int j) {
   const double cx = cen[j], cy = cen[np + j], cz = cen[2 * np + j];
   double s = 0.0;
-  for (int i = 0; i < n_f; ++i) {
+  // only the points the cell can reach; the exact test below still decides,
+  // so the sum is the full loop's, term for term
+  int i_lo, i_hi;
+  scan_point_range(g, n_f, cx, cy, cz, &i_lo, &i_hi);
+  for (int i = i_lo; i <= i_hi; ++i) {
     double p[3] = {g.base[0], g.base[1], g.base[2]};
     p[g.axis] = g.x_min + i * g.res;
     const double rx = cx - p[0], ry = cy - p[1], rz = cz - p[2];

@@ -215,6 +215,27 @@ struct ScanGeo {
   double sx, sy, sz, alpha;
 };
 
+// The scan points a cell at (cx, cy, cz) can reach (distance < 1 um): none
+// (lo > hi) when its distance to the line is 1 um or more, else those whose
+// axis coordinate lies within sqrt(1 - t^2) of the cell's, t the distance to
+// the line, widened by one point each way and by 1e-9 on t^2 against the
+// rounding of the exact test, which the callers still apply.
+__device__ __forceinline__ void scan_point_range(const ScanGeo& g, int n_f, double cx, double cy, double cz, int* lo,
+                                                 int* hi) {
+  const double c[3] = {cx, cy, cz};
+  double t2 = 0.0;
+  for (int a = 0; a < 3; ++a)
+    if (a != g.axis) t2 += (c[a] - g.base[a]) * (c[a] - g.base[a]);
+  if (!(t2 < 1.0 + 1e-9)) {
+    *lo = n_f;
+    *hi = -1;
+    return;
+  }
+  const double half = sqrt(1.0 + 1e-9 - t2);
+  *lo = max(0, (int)floor((c[g.axis] - half - g.x_min) / g.res) - 1);
+  *hi = min(n_f - 1, (int)ceil((c[g.axis] + half - g.x_min) / g.res) + 1);
+}
+
 constexpr int SCAN_BS = 256;
 constexpr int SCAN_MAX_POINTS = 1024;
 constexpr int SCAN_CELLS_PER_THREAD = 4;

@@ -27,7 +27,28 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_blur(const RealDesc* __restric
     }
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // The points any cell of this wavefront can reach: outside them every
+  // lane's sum is zero, so the wavefront's partial is written as zero
+  // without the loop (the same bits as the full loop).
+  int lo = n_f, hi = -1;
+  for (int q = 0; q < SCAN_CELLS_PER_THREAD; ++q) {
+    const int j = (blockIdx.x * SCAN_CELLS_PER_THREAD + q) * SCAN_BS + threadIdx.x;
+    if (j < n) {
+      int a, b;
+      scan_point_range(g, n_f, cx[q], cy[q], cz[q], &a, &b);
+      lo = min(lo, a);
+      hi = max(hi, b);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o));
+    hi = max(hi, __shfl_xor(hi, o));
+  }
   for (int i = 0; i < n_f; ++i) {
+    if (i < lo || i > hi) {  // uniform across the wavefront
+      if (lane == 0) red[wave][i] = 0.0;
+      continue;
+    }
     double p[3] = {g.base[0], g.base[1], g.base[2]};
     p[g.axis] = g.x_min + i * g.res;
     double acc = 0.0;

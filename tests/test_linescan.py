@@ -184,3 +184,50 @@ def test_nearest_cell_brute_force(oracle):
             dist = np.sum((allc - p) ** 2, axis=1)
             assert out[i] == vals[int(np.argmin(dist))]
     assert np.any(ocaf == 4.0) and np.any(ocaf != 4.0)
+
+
+def test_scan_point_range_keeps_every_contributing_point():
+    """The forward blur and the adjoint's scan source visit only the points
+    a cell can reach (dev_common.h scan_point_range): those whose axis
+    coordinate lies within sqrt(1 + 1e-9 - t^2) (+- one point) of the
+    cell's, t the cell's distance to the line, and none when
+    t^2 >= 1 + 1e-9.  Restated here with the same double arithmetic: over
+    random cells near and far from lines on each axis (including cells at
+    distance ~1 um and beyond the line's ends), the visited range holds every
+    point that passes the full loop's 1 um test, so the sum is the full
+    loop's term for term."""
+    import math
+
+    rng = np.random.default_rng(5)
+    for axis in (0, 1, 2):
+        base = rng.uniform(-0.3, 0.3, 3)
+        x_min, res, n_f = -0.96, 0.0297, 64
+        cells = np.concatenate([rng.uniform(-2.0, 2.0, (4000, 3)),
+                                base + rng.normal(0, 1, (4000, 3)) / np.sqrt(3.0)])
+        # cells at distance 1 um from the line, give or take rounding
+        u = rng.normal(0, 1, (2000, 3))
+        u[:, axis] = 0
+        u /= np.linalg.norm(u, axis=1)[:, None]
+        ring = base + u * (1.0 + rng.uniform(-1e-15, 1e-15, (2000, 1)))
+        ring[:, axis] = rng.uniform(-1.5, 1.5, 2000)
+        cells = np.concatenate([cells, ring])
+        for c in cells:
+            p_axis = [x_min + i * res for i in range(n_f)]
+            full = []
+            for i in range(n_f):
+                p = list(base)
+                p[axis] = p_axis[i]
+                rx, ry, rz = c[0] - p[0], c[1] - p[1], c[2] - p[2]
+                if rx * rx + ry * ry + rz * rz < 1.0:
+                    full.append(i)
+            t2 = 0.0
+            for a in range(3):
+                if a != axis:
+                    t2 += (c[a] - base[a]) * (c[a] - base[a])
+            if not (t2 < 1.0 + 1e-9):
+                assert full == []
+                continue
+            half = math.sqrt(1.0 + 1e-9 - t2)
+            lo = max(0, int(math.floor((c[axis] - half - x_min) / res)) - 1)
+            hi = min(n_f - 1, int(math.ceil((c[axis] + half - x_min) / res)) + 1)
+            assert all(lo <= i <= hi for i in full), (c, full, lo, hi)
