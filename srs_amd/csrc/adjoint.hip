@@ -42,22 +42,24 @@ __device__ __forceinline__ SlBuf sl_buffer(const RealDesc& d, int j, double c, u
 }
 
 // The loss source of one scan sample at element j: sum_i gcoef_i w(c_j - p_i)
-// over points within 1 um (the blur's PSF, linescan/parameters.h:60-62).
+// over points within 1 um (the blur's PSF, linescan/parameters.h:60-62), with
+// the blur's own weights (the walk from the anchor at or below the first
+// point the cell can reach).
 __device__ __forceinline__ double scan_adj_sum(const double* __restrict__ cen, const ScanGeo& g, int n_f,
                                                const double* __restrict__ gcoef, size_t np, int j) {
   const double cx = cen[j], cy = cen[np + j], cz = cen[2 * np + j];
   double s = 0.0;
-  // only the points the cell can reach; the exact test below still decides,
-  // so the sum is the full loop's, term for term
   int i_lo, i_hi;
   scan_point_range(g, n_f, cx, cy, cz, &i_lo, &i_hi);
-  for (int i = i_lo; i <= i_hi; ++i) {
-    double p[3] = {g.base[0], g.base[1], g.base[2]};
-    p[g.axis] = g.x_min + i * g.res;
-    const double rx = cx - p[0], ry = cy - p[1], rz = cz - p[2];
-    if (rx * rx + ry * ry + rz * rz < 1.0)
-      s += gcoef[i] * (g.alpha * exp(-0.5 * ((rx * rx / (g.sx * g.sx)) + (ry * ry / (g.sy * g.sy)) +
-                                             (rz * rz / (g.sz * g.sz)))) / (g.sx * g.sy * g.sz));
+  const ScanCell c = scan_cell(g, cx, cy, cz);
+  const int i0 = scan_walk_start(g, i_lo);
+  double w = 0.0, r = 0.0;
+  for (int i = i0; i <= i_hi; ++i) {
+    const double pa = scan_pa(g, i);
+    if (scan_is_anchor(g, i, i0)) scan_anchor(g, c, pa, &w, &r);
+    if (i >= i_lo && scan_dist2(c, pa) < 1.0) s += gcoef[i] * w;
+    w *= r;
+    r *= g.q;
   }
   return s;
 }

@@ -208,12 +208,65 @@ __device__ __forceinline__ Geo load_geo(const RealDesc& d, const int j) {
 // Gaussian PSF w of linescan/parameters.h:60-62.  Mesh cells are summed on the
 // device, one partial sum per workgroup and point, then the partials in
 // workgroup order; the padding cells' part is a per-point constant.
+//
+// Along the line only the axis term of the PSF's exponent changes, by a
+// quadratic in the point index, so a cell's weights at consecutive points
+// follow w(i+1) = w(i) r(i), r(i+1) = r(i) q with q = exp(-res^2 / s_a^2): two
+// products per point instead of an exponential.  The walk restarts from the
+// closed form at every SCAN_ANCHOR-th point, which keeps the rounding of a
+// weight within a few tens of ulps; a weight is a function of the cell and
+// the point alone, so the blur and the adjoint's loss source use the same
+// numbers.  PSFs so narrow that an anchor could underflow while its segment
+// holds a weight above exp(-260), or a ratio overflow, within a cell's reach
+// take the closed form at every point (walk = 0).
 struct ScanGeo {
   double base[3];   // scan line point with the axis coordinate zero
   int axis;
   double x_min, res;
   double sx, sy, sz, alpha;
+  double inv[3];    // 1 / s^2 per coordinate (s = sx, sy, sz)
+  double kw;        // alpha / (sx sy sz)
+  double q;         // exp(-res^2 inv[axis])
+  int walk;         // 1: the recurrence above; 0: the closed form everywhere
 };
+
+constexpr int SCAN_ANCHOR = 8;
+
+// One cell's terms of the walk: its axis coordinate, the perpendicular
+// squares in the reference's summation order (distance^2 = (da^2 + u) + v,
+// da the axis difference) and the perpendicular part of the PSF exponent.
+struct ScanCell {
+  double ca, u, v, eperp;
+};
+__device__ __forceinline__ ScanCell scan_cell(const ScanGeo& g, double cx, double cy, double cz) {
+  const double r0 = cx - g.base[0], r1 = cy - g.base[1], r2 = cz - g.base[2];
+  const double t0 = r0 * r0, t1 = r1 * r1, t2 = r2 * r2;
+  ScanCell c;
+  if (g.axis == 0) { c.ca = cx; c.u = t1; c.v = t2; c.eperp = t1 * g.inv[1] + t2 * g.inv[2]; }
+  else if (g.axis == 1) { c.ca = cy; c.u = t0; c.v = t2; c.eperp = t0 * g.inv[0] + t2 * g.inv[2]; }
+  else { c.ca = cz; c.u = t0 + t1; c.v = 0.0; c.eperp = t0 * g.inv[0] + t1 * g.inv[1]; }
+  return c;
+}
+// Axis coordinate of scan point i (linescan/main.cpp:199-200).
+__device__ __forceinline__ double scan_pa(const ScanGeo& g, int i) { return g.x_min + i * g.res; }
+// The reference's distance^2 between the cell and point i, bit for bit.
+__device__ __forceinline__ double scan_dist2(const ScanCell& c, double pa) {
+  const double da = c.ca - pa;
+  return (da * da + c.u) + c.v;
+}
+// Weight at point i from the closed form and the ratio to point i + 1.
+__device__ __forceinline__ void scan_anchor(const ScanGeo& g, const ScanCell& c, double pa, double* w, double* r) {
+  const double da = c.ca - pa, ia = g.inv[g.axis];
+  *w = g.kw * exp(-0.5 * (da * da * ia + c.eperp));
+  *r = exp(g.res * (da - 0.5 * g.res) * ia);
+}
+// The first point of a walk that covers lo: its anchor.
+__device__ __forceinline__ int scan_walk_start(const ScanGeo& g, int lo) {
+  return g.walk ? lo - lo % SCAN_ANCHOR : lo;
+}
+__device__ __forceinline__ bool scan_is_anchor(const ScanGeo& g, int i, int i0) {
+  return !g.walk || i == i0 || i % SCAN_ANCHOR == 0;
+}
 
 // The scan points a cell at (cx, cy, cz) can reach (distance < 1 um): none
 // (lo > hi) when its distance to the line is 1 um or more, else those whose

@@ -31,7 +31,9 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_blur(const RealDesc* __restric
   // lane's sum is zero, so the wavefront's partial is written as zero
   // without the loop (the same bits as the full loop).
   int lo = n_f, hi = -1;
+  ScanCell sc[SCAN_CELLS_PER_THREAD];
   for (int q = 0; q < SCAN_CELLS_PER_THREAD; ++q) {
+    sc[q] = scan_cell(g, cx[q], cy[q], cz[q]);
     const int j = (blockIdx.x * SCAN_CELLS_PER_THREAD + q) * SCAN_BS + threadIdx.x;
     if (j < n) {
       int a, b;
@@ -44,23 +46,26 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_blur(const RealDesc* __restric
     lo = min(lo, __shfl_xor(lo, o));
     hi = max(hi, __shfl_xor(hi, o));
   }
-  for (int i = 0; i < n_f; ++i) {
-    if (i < lo || i > hi) {  // uniform across the wavefront
-      if (lane == 0) red[wave][i] = 0.0;
-      continue;
-    }
-    double p[3] = {g.base[0], g.base[1], g.base[2]};
-    p[g.axis] = g.x_min + i * g.res;
+  if (lane == 0) {
+    for (int i = 0; i < min(lo, n_f); ++i) red[wave][i] = 0.0;
+    for (int i = max(hi + 1, 0); i < n_f; ++i) red[wave][i] = 0.0;
+  }
+  // Each cell's weights by the walk from the anchor at or below lo (the
+  // cut-off test is the reference's exact one, so a weight that is never
+  // used may be anything).
+  const int i0 = scan_walk_start(g, lo);
+  double ww[SCAN_CELLS_PER_THREAD], wr[SCAN_CELLS_PER_THREAD];
+  for (int i = i0; i <= hi; ++i) {
+    const double pa = scan_pa(g, i);
+    const bool anchor = scan_is_anchor(g, i, i0);  // uniform across the wavefront
     double acc = 0.0;
     for (int q = 0; q < SCAN_CELLS_PER_THREAD; ++q) {
-      const double rx = cx[q] - p[0], ry = cy[q] - p[1], rz = cz[q] - p[2];
-      const double dist = rx * rx + ry * ry + rz * rz;
-      if (dist < 1.0) {
-        const double w = g.alpha * exp(-0.5 * ((rx * rx / (g.sx * g.sx)) + (ry * ry / (g.sy * g.sy)) +
-                                               (rz * rz / (g.sz * g.sz)))) / (g.sx * g.sy * g.sz);
-        acc += vf[q] * w;
-      }
+      if (anchor) scan_anchor(g, sc[q], pa, &ww[q], &wr[q]);
+      if (scan_dist2(sc[q], pa) < 1.0) acc += vf[q] * ww[q];
+      ww[q] *= wr[q];
+      wr[q] *= g.q;
     }
+    if (i < lo) continue;
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     if (lane == 0) red[wave][i] = acc;
   }
@@ -131,6 +136,21 @@ int srs_ctx_add_scan(srs_ctx* c, int32_t mesh_id, const double* centers, const s
   if (sd->axis == 2) { g.sx = std::sqrt(sd->var[2]); g.sy = std::sqrt(sd->var[0]); g.sz = std::sqrt(sd->var[1]); }
   else { g.sx = std::sqrt(sd->var[0]); g.sy = std::sqrt(sd->var[1]); g.sz = std::sqrt(sd->var[2]); }
   g.alpha = 1.0 / std::pow(2 * 3.14159, 1.5);  // the reference's pi, linescan/parameters.h:45-46
+  g.inv[0] = 1.0 / (g.sx * g.sx);
+  g.inv[1] = 1.0 / (g.sy * g.sy);
+  g.inv[2] = 1.0 / (g.sz * g.sz);
+  g.kw = g.alpha / (g.sx * g.sy * g.sz);
+  {
+    // The walk is safe when an anchor SCAN_ANCHOR - 1 points before a point
+    // of weight ~1 has an exponent below 100 (then a weight exp(-x) of the
+    // segment can be lost to the anchor's underflow only for x > 260) and
+    // the ratio stays finite over a cell's reach (|da| < 1 + (SCAN_ANCHOR +
+    // 1) res).
+    const double ia = g.inv[g.axis], h = g.res;
+    const double far = (SCAN_ANCHOR - 1) * h;
+    g.walk = 0.5 * far * far * ia < 100.0 && h * (1.0 + (SCAN_ANCHOR + 1) * h) * ia < 600.0;
+    g.q = std::exp(-h * h * ia);
+  }
 
   // padding cells, linescan/main.cpp:119-141 and AddPointGrid :268-306
   std::vector<std::array<double, 3>> pc;

@@ -8,7 +8,7 @@ import math
 import numpy as np
 
 import srs_amd as S
-from oracle.oracle import RNG_PHILOX, Fields, Oracle, grid_dict
+from oracle.oracle import RNG_GLIBC, RNG_PHILOX, Fields, Oracle, grid_dict
 from srs_amd import workloads as W
 
 
@@ -138,3 +138,94 @@ def oracle_loop(O: Oracle, pr, p: S.Params, F: Fields, key: int, stream: int, ma
             raise RuntimeError("oracle loop did not terminate")
     recs.append(O.record(t, G, F, P, ryr_out=ryr_out, gpu_layout=gpu_layout))
     return recs, t, step
+
+
+def spark_stats(states: np.ndarray, first: int):
+    """Statistics of one realization's RyR trajectory (rows = steps, columns =
+    RyRs): the step at which RyR `first` (open at the start) first closes
+    (rows + 1 if it never does), the number of RyR openings, the number of
+    distinct RyRs that were ever open, and the largest number open at once."""
+    st = states > 0
+    closed = np.nonzero(~st[:, first])[0]
+    t_close = int(closed[0]) if closed.size else st.shape[0] + 1
+    openings = int(np.sum(st[1:] & ~st[:-1]))
+    return t_close, openings, int(st.any(axis=0).sum()), int(st.sum(axis=1).max())
+
+
+# The spark-box distribution test's workload: the 56,250-element spark box,
+# its centre RyR (24 of the 7x7 cluster) open at t = 0, 1,500 steps, RyR
+# kinetics fast enough that the cluster recruits within that window.
+SPARK_DIST = dict(n_steps=1500, first=24, k_plus=1.0, k_minus=20.0, n_dev=1024, n_ref=128, key=38376427)
+
+
+def spark_dist_fields(pr, k):
+    """Initial fields of realization k: equilibrium state, RyR `first` open,
+    LCCs closed, next-transition thresholds log(u) from seed 1000 + k."""
+    g = pr.grid
+    rs = np.zeros(g.n_ryr, np.int32)
+    rs[SPARK_DIST["first"]] = 1
+    R = np.log(np.random.default_rng(1000 + k).random(g.n_ryr + g.n_lcc))
+    return Fields(pr.X0, rs, np.ones(g.n_lcc, np.int32), np.full(g.n_lcc, 2, np.int32), R,
+                  [0.0, pr.params.reals[S.R["DEFAULT_CA_NSR"]]])
+
+
+def spark_dist_check(dev, ref, q, n_steps, alpha=0.01):
+    """The distribution test's verdicts: two-sample KS and Welch t p-values of
+    each statistic (device vs reference), and the one-sample KS p-values of
+    both sets' first closing times against their exact law, a geometric
+    distribution of parameter q censored at n_steps (the per-integer
+    distance, with the continuous KS tail, which is conservative for a
+    discrete law).  Returns (names, p-values, all above alpha)."""
+    from scipy import stats
+
+    names, pv = [], []
+    for c, name in enumerate(("first closing", "openings", "RyRs recruited", "peak open")):
+        names += [f"{name} KS", f"{name} t"]
+        pv.append(stats.ks_2samp(dev[:, c], ref[:, c]).pvalue)
+        a, b = dev[:, c].astype(float), ref[:, c].astype(float)
+        pv.append(1.0 if a.var() == 0 and b.var() == 0 and a.mean() == b.mean() else
+                  stats.ttest_ind(a, b, equal_var=False).pvalue)
+    k = np.arange(1, n_steps)  # a closing is seen at rows 1 .. n_steps - 1
+    law = 1.0 - (1.0 - q) ** k
+    for nm, arr in (("device", dev), ("reference", ref)):
+        t = arr[:, 0]
+        emp = np.searchsorted(np.sort(t), k, side="right") / t.size
+        names.append(f"{nm} first closing law")
+        pv.append(float(stats.kstwo.sf(np.abs(emp - law).max(), t.size)))
+    return names, pv, min(pv) > alpha
+
+
+_SPARK = {}
+
+
+def oracle_spark_init(G, reals, X0):
+    """Pool initializer for oracle_glibc_spark: the spark box's step arrays
+    (grid_dict), parameter vector and equilibrium state, handed over once per
+    worker so the workers never load the HIP library."""
+    _SPARK.update(G=G, reals=reals, X0=X0)
+
+
+def oracle_glibc_spark(args):
+    """One spark-box run with the oracle's step and the reference's own
+    glibc rand() gating (a process-pool worker, after oracle_spark_init):
+    n_steps steps from the equilibrium state with RyR `first` open, srand(seed)
+    first; returns spark_stats of the RyR trajectory."""
+    import ctypes
+
+    seed, n_steps, first = args
+    G, P = _SPARK["G"], _SPARK["reals"]
+    nr, nl = len(G["ryr_ele"]), len(G["lcc_ele"])
+    rs = np.zeros(nr, np.int32)
+    rs[first] = 1
+    R0 = np.log(np.random.default_rng(seed).random(nr + nl))
+    F = Fields(_SPARK["X0"], rs, np.ones(nl, np.int32), np.full(nl, 2, np.int32), R0,
+               [0.0, P[S.R["DEFAULT_CA_NSR"]]])
+    ctypes.CDLL(None).srand(ctypes.c_uint(seed))
+    O = Oracle()
+    dt = float(P[S.R["T_STEP"]])
+    rows = np.zeros((n_steps, nr), np.int32)
+    rows[0] = F.ryr_state
+    for k in range(1, n_steps):
+        O.steps(G, P, F, 1, dt, gating=True, rng_kind=RNG_GLIBC)
+        rows[k] = F.ryr_state
+    return spark_stats(rows, first)

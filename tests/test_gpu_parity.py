@@ -338,12 +338,16 @@ def test_stochastic_gating_matches_reference_in_distribution():
     assert ref[:, 3].mean() > 0.5  # the cluster does recruit openings
 
 
-@pytest.mark.parametrize("axis", [0, 2])
-def test_linescan_matches_oracle(axis):
+@pytest.mark.parametrize("axis,psf", [(0, "default"), (2, "default"), (0, "narrow"), (2, "fine")])
+def test_linescan_matches_oracle(axis, psf):
     """The device linescan (nearest-cell Ca and CaF, PSF-blurred CaF with the
     padding) on the 56,250-element spark box against the oracle's restatement
     of linescan/main.cpp.  Nearest values are copies (bit-exact); the blurred
-    sum is summed in another order (rel <= 1e-12)."""
+    sum is summed in another order, and its weights walk along the line by
+    products between closed-form anchors (rel <= 1e-12).  "narrow": a PSF of
+    s = 10 nm, too narrow for the walk, so every weight is the closed form;
+    "fine": 5 nm points (200 per um) under the default PSF, the longest
+    walks."""
     pr = W.prepare_spark()
     g = pr.grid
     n = g.n_ele
@@ -362,6 +366,11 @@ def test_linescan_matches_oracle(axis):
     else:
         d.x_min, d.x_max, d.res_x_f, d.res_x_ca = -0.45, 0.45, 0.02, 0.005
         d.offset1, d.offset2 = 0.02, 0.01
+    if psf == "narrow":
+        for k in range(3):
+            d.var[k] = 1e-4
+    elif psf == "fine":
+        d.res_x_f = 0.005
     with S.Context(0) as ctx:
         mid = ctx.add_mesh_grid(g, pr.aij, pr.nij, perm=W.kuhn_block_order(g))
         ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, pr.params, seed=1, stream=0)
@@ -1008,3 +1017,49 @@ def test_realization_subsets_equal_all_at_once(group):
         np.testing.assert_array_equal(a[3], b[3])
         steps.add(a[2]["step"])
     assert len(steps) > 10  # the runs ended at different steps
+
+
+@pytest.mark.timeout(600)
+def test_spark_box_gating_matches_reference_in_distribution():
+    """Stochastic gating on the plain CPU workload's mesh: the 56,250-element
+    spark box with its 7x7 cluster, the centre RyR open at t = 0, 1,500 steps,
+    with RyR kinetics fast enough that the cluster recruits within that window
+    (k_open 1.0 [Ca]^2.1 /ms, k_close 20 /ms).  1,024 device realizations
+    (Philox, own streams, one batched context) against 128 runs of the
+    oracle step with the reference's own glibc rand() gating (bit-exact with
+    the reference's step): two-sample KS and Welch t tests, p > 0.01, on the
+    centre RyR's first closing time, the number of openings, the number of
+    RyRs recruited and the peak number open at once, and one-sample KS tests
+    of both sets' first closing times against their exact geometric law.
+    Seeds are fixed, so the outcome is reproducible.  tests/spark_box_power.py
+    plants a 1.5x k_open or k_close in the oracle's Philox restatement of
+    the device's gating and shows that this test then fails."""
+    import multiprocessing as mp
+
+    from tests.helpers import (SPARK_DIST, oracle_glibc_spark, oracle_spark_init, spark_dist_check,
+                               spark_dist_fields, spark_stats)
+
+    c = SPARK_DIST
+    pr = W.prepare_spark()
+    g = pr.grid
+    p = pr.params.copy().setr(RYR_K_PLUS=c["k_plus"], RYR_K_MINUS=c["k_minus"])
+    n_dev, n_ref, n_steps, first = c["n_dev"], c["n_ref"], c["n_steps"], c["first"]
+    fields = [spark_dist_fields(pr, k) for k in range(n_dev)]
+    print(f"spark box in distribution: {n_dev} device realizations", flush=True)
+    with gpu_context([pr] * n_dev, fields, params=[p] * n_dev, seeds=[c["key"]] * n_dev, share_mesh=True,
+                     order="block") as ctx:
+        scheds = ctx.record_schedules(n_steps)
+    dev = np.array([spark_stats(s, first) for s in scheds])
+    del scheds
+    print(f"spark box in distribution: {n_ref} oracle runs with glibc gating, 16 processes", flush=True)
+    seeds = [93843 + 38376427 + 83 * k for k in range(n_ref)]
+    init = (grid_dict(g, pr.aij, pr.nij), np.array(p.reals), pr.X0)
+    with mp.get_context("spawn").Pool(16, initializer=oracle_spark_init, initargs=init) as pool:
+        ref = np.array(pool.map(oracle_glibc_spark, [(s, n_steps, first) for s in seeds]))
+    q = 1.0 - np.exp(-c["k_minus"] * p.reals[S.R["T_STEP"]])
+    names, pv, ok = spark_dist_check(dev, ref, q, n_steps)
+    for nm, v in zip(names, pv):
+        print(f"  {nm}: p {v:.4f}", flush=True)
+    print(f"  means device {dev.mean(axis=0)}, reference {ref.mean(axis=0)}", flush=True)
+    assert ok, dict(zip(names, pv))
+    assert ref[:, 1].mean() > 1 and dev[:, 2].mean() > 1.5  # the cluster does recruit
