@@ -64,7 +64,9 @@ __device__ __forceinline__ double scan_adj_sum(const double* __restrict__ cen, c
   return s;
 }
 
-// w = (lambda_Ca / beta, lambda_ATP, lambda_CaM, lambda_Fluo) / V0 of element j.
+// w = (lambda_Ca / beta, lambda_ATP, lambda_CaM, lambda_Fluo) / V0 of element j
+// (c, the element's Ca, is read only where the SL buffer makes beta depend on
+// it; callers pass 0 elsewhere and skip the load).
 __device__ __forceinline__ d4 adj_weight(const RealDesc& d, int j, double c, uint8_t f, double la_c, double la_a,
                                          double la_m, double la_d) {
   const SlBuf sb = sl_buffer(d, j, c, f);
@@ -84,7 +86,8 @@ __global__ __launch_bounds__(ADJ_BS) void k_adj_w(const RealDesc* __restrict__ D
   lam += blockIdx.y * 6 * (size_t)d.np;
   w += blockIdx.y * (size_t)d.np;
   const d4 m = ((const d4*)lam)[j];
-  w[j] = adj_weight(d, j, X[4 * (size_t)j], d.flags[j], m.x, m.y, m.z, m.w);
+  const uint8_t f = d.flags[j];
+  w[j] = adj_weight(d, j, (f & F_SL) ? X[4 * (size_t)j] : 0.0, f, m.x, m.y, m.z, m.w);
 }
 
 // What the step kernel does after lambda_n: add the loss source of a scan
@@ -109,15 +112,18 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
   const int r = rid[blockIdx.y];
   const RealDesc& d = D[r];
   const RealLive& l = L[r];
-  const int j = blockIdx.x * ADJ_BS + threadIdx.x;
+  // Tiles in grid order: an XCD-aware order (consecutive tiles on one XCD)
+  // cuts this kernel's L2 misses by 40% but measured 6% slower.
+  const int tile = blockIdx.x;
+  const int j = tile * ADJ_BS + threadIdx.x;
   const size_t np = d.np;
   X += blockIdx.y * 6 * np;
   lam += blockIdx.y * 6 * np;
   lam_out += blockIdx.y * 6 * np;
   w += blockIdx.y * np;
   gpart += (size_t)blockIdx.y * gridDim.x * ADJ_NT;
-  // the block's running partials, loaded now so the final update waits on no load
-  const double gp_old = threadIdx.x < ADJ_NT ? gpart[(size_t)blockIdx.x * ADJ_NT + threadIdx.x] : 0.0;
+  // the tile's running partials, loaded now so the final update waits on no load
+  const double gp_old = threadIdx.x < ADJ_NT ? gpart[(size_t)tile * ADJ_NT + threadIdx.x] : 0.0;
   // RyR states during this step: row `step` of the frozen schedule
   const int32_t* ryr = d.n_ryr ? d.sched + step * d.n_ryr : nullptr;
   double gr[ADJ_NT];
@@ -261,22 +267,24 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     lam_out[5 * np + j] = lC;
     if (nx.Xprev) {
       const double* Xp = nx.Xprev + blockIdx.y * 6 * np;
-      nx.wout[blockIdx.y * np + j] = adj_weight(d, j, Xp[4 * (size_t)j], f, lc, la, lm, ld);
+      nx.wout[blockIdx.y * np + j] = adj_weight(d, j, (f & F_SL) ? Xp[4 * (size_t)j] : 0.0, f, lc, la, lm, ld);
     }
   }
-  // block reduction of the gradient partials, accumulated per block
+  // block reduction of the gradient partials, accumulated per block: DPP
+  // sums within each wavefront (a component that is zero in every lane, as
+  // the channel and SL terms are in most wavefronts, is not summed), then
+  // the four wavefronts in LDS
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < ADJ_NT; ++k) {
-    double v = gr[k];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane == 0) red[wave][k] = v;
+    const double v = __builtin_amdgcn_ballot_w64(gr[k] != 0.0) ? wave_sum_dpp(gr[k]) : 0.0;
+    if (lane == 63) red[wave][k] = v;
   }
   __syncthreads();
   if (threadIdx.x < ADJ_NT) {
     double s = 0.0;
     for (int q = 0; q < ADJ_BS / 64; ++q) s += red[q][threadIdx.x];
-    gpart[(size_t)blockIdx.x * ADJ_NT + threadIdx.x] = gp_old + s;
+    gpart[(size_t)tile * ADJ_NT + threadIdx.x] = gp_old + s;
   }
 }
 

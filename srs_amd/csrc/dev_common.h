@@ -138,6 +138,27 @@ __device__ __forceinline__ int xcd_tile(int bid, int grid) {
   return (bid % NXCD) * (grid / NXCD) + bid / NXCD;
 }
 
+// Sum of v over the 64 lanes of a wavefront, left in lane 63, by DPP lane
+// moves (no LDS traffic): sums within quads, then within each row of 16
+// lanes (row_shr 4 and 8 leave the row's sum in its lane 15), then row 0
+// into row 1 and rows 0-1 into rows 2-3 by row broadcasts.  Every lane must
+// be active.  A fixed order, so the result is deterministic.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_f64<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_f64<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
 // Loads of data read once per step (operator rows, flags) may bypass cache
 // allocation so that the state planes stay resident between steps.  Pointers
 // read from the descriptor are generic; casting them to the global address

@@ -66,8 +66,8 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_blur(const RealDesc* __restric
       wr[q] *= g.q;
     }
     if (i < lo) continue;
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane == 0) red[wave][i] = acc;
+    acc = wave_sum_dpp(acc);
+    if (lane == 63) red[wave][i] = acc;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < n_f; i += SCAN_BS) {
