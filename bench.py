@@ -101,7 +101,7 @@ def setup_plain(pr, order="block", tune=None, ext=None):
     ctx.state_set(0, pr.X0)
     ctx.channels_set(0, rs, ls, lvs, R)
     ctx.globals_set(0, 0.0, pr.params.reals[16])
-    ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state", "occupancy", "lds") if k in tune})
+    ctx.tune(**{k: tune[k] for k in ("nt_geometry", "nt_state", "occupancy", "lds", "gated_sweep") if k in tune})
     return ctx
 
 
@@ -541,8 +541,8 @@ def main():
     # The library folds each step's gating into the next step's sweep launch
     # (k_sweep_gated) for a single realization with the LDS sweep and a
     # register budget of 0 or 6; time_sweep then times that kernel.
-    eff = {"lds": 1, "occupancy": 0, **tune}
-    kernel = "k_sweep_gated" if eff["lds"] and eff["occupancy"] in (0, 6) else "k_sweep"
+    eff = {"lds": 1, "occupancy": 0, "gated_sweep": 1, **tune}
+    kernel = "k_sweep_gated" if eff["gated_sweep"] and eff["lds"] and eff["occupancy"] in (0, 6) else "k_sweep"
     nbytes = algorithmic_bytes(N)
     achieved = nbytes / (sweep_ms * 1e-3) / 1e9
     traffic = profiled_traffic(os.path.join(ROOT, "profiles", f"k_sweep_{args.workload}_traffic.md"))
