@@ -757,7 +757,22 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
         hipLaunchKernelGGL(k_seg_start, dim3(nb), dim3(256), 0, c->stream, c->d_desc, c->d_live, d_rid, s0);
         hipError_t e2 = hipMemcpyAsync(seg, ckq, (size_t)nb * n6 * sizeof(double), hipMemcpyDeviceToDevice,
                                        c->stream);
-        for (int64_t n = s0; n < s1 - 1 && e2 == hipSuccess; ++n) {
+        if (gated) {
+          // a single realization: one gated-sweep launch per step, from slot
+          // n - s0 into slot n - s0 + 1, whose workgroup 0 replays the step
+          // before; k_gate replays the last one (the same live state as
+          // below, and the same fields)
+          for (int64_t n = s0; n < s1 - 1 && e2 == hipSuccess; ++n) {
+            launch_gated(c, 0, n > s0 ? gmode : 0u, false, seg + (size_t)(n - s0) * n6,
+                         seg + (size_t)(n - s0 + 1) * n6);
+            e2 = hipGetLastError();
+          }
+          if (s1 - 1 > s0 && e2 == hipSuccess) {
+            launch_gate_single(c, gmode);
+            e2 = hipGetLastError();
+          }
+        }
+        for (int64_t n = s0; !gated && n < s1 - 1 && e2 == hipSuccess; ++n) {
           hipLaunchKernelGGL(k_seg_ptrs, dim3((nb + 63) / 64), dim3(64), 0, c->stream, c->d_desc, c->d_live, d_rid,
                              seg, s0, nb);
           launch_step(c, gmode);

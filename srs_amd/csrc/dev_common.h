@@ -480,6 +480,11 @@ int launch_step(srs_ctx* c, uint32_t gmode);
 // m steps of a single-realization context through the gated sweep, starting
 // from live state index cur, ending with the gating of the last step
 void launch_gated_block(srs_ctx* c, int64_t m, int cur, uint32_t gmode);
+// One step through the single-realization gated sweep; xo_at/xn_at (both or
+// neither) replace X[cur] -> X[cur ^ 1] as the step's state buffers.
+void launch_gated(srs_ctx* c, int cur, uint32_t gmode, bool proto, const double* xo_at = nullptr,
+                  double* xn_at = nullptr);
+void launch_gate_single(srs_ctx* c, uint32_t gmode);
 int drop_graphs(srs_ctx* c);
 int tiles_full(srs_ctx* c);
 int tiles_compact(srs_ctx* c);

@@ -692,7 +692,7 @@ __global__ __launch_bounds__(GATE_THREADS) void k_gate_list(const RealDesc* __re
 // last-level cache.  The order changes no result.
 template <int BS, bool NTG, bool NTS, bool COMPACT, bool LEAN, bool PROTO>
 __device__ __forceinline__ void sweep_gated(const RealDesc* __restrict__ D, RealLive* __restrict__ L, int n_tiles,
-                                            int curx, uint32_t gmode) {
+                                            int curx, uint32_t gmode, const double* xo_at, double* xn_at) {
   const int cur = curx & 1;
   int b = xcd_tile(blockIdx.x, gridDim.x);
   if (b > n_tiles) return;  // b = 0: the gating workgroup; b = 1..n_tiles: tile b - 1
@@ -700,7 +700,9 @@ __device__ __forceinline__ void sweep_gated(const RealDesc* __restrict__ D, Real
   const RealDesc& d = D[0];
   const RealLive& l = L[0];
   const int skip = l.done | l.ffwd;
-  const Bufs bf = bufs(d.X[0], d.X[1], cur);
+  // the state buffers: X[cur] -> X[cur ^ 1], or the caller's (xo_at, xn_at)
+  // (a replayed segment whose steps each have a slot of their own)
+  const Bufs bf = xo_at ? Bufs{gp(xo_at), gpw(xn_at)} : bufs(d.X[0], d.X[1], cur);
   scalars_here(fold(skip, bf.xo, bf.xn) ^ geo_fold<COMPACT>(d));
   if (b == 0) {
     if (gmode) gate_workgroup<BS, LEAN, PROTO>(D, L, 0, gmode);
@@ -723,19 +725,22 @@ __device__ __forceinline__ void sweep_gated(const RealDesc* __restrict__ D, Real
 // the RyR update (LEAN), which is what lets the kernel fit 80 VGPRs.
 template <int BS, bool NTG, bool NTS, bool COMPACT>
 __global__ __launch_bounds__(BS) void k_sweep_gated(const RealDesc* __restrict__ D, RealLive* __restrict__ L,
-                                                    int n_tiles, int cur, uint32_t gmode) {
-  sweep_gated<BS, NTG, NTS, COMPACT, false, false>(D, L, n_tiles, cur, gmode);
+                                                    int n_tiles, int cur, uint32_t gmode, const double* xo_at,
+                                                    double* xn_at) {
+  sweep_gated<BS, NTG, NTS, COMPACT, false, false>(D, L, n_tiles, cur, gmode, xo_at, xn_at);
 }
 template <int BS, bool NTG, bool NTS, bool COMPACT, int W>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_sweep_gated_w(
-    const RealDesc* __restrict__ D, RealLive* __restrict__ L, int n_tiles, int cur, uint32_t gmode) {
-  sweep_gated<BS, NTG, NTS, COMPACT, true, false>(D, L, n_tiles, cur, gmode);
+    const RealDesc* __restrict__ D, RealLive* __restrict__ L, int n_tiles, int cur, uint32_t gmode,
+    const double* xo_at, double* xn_at) {
+  sweep_gated<BS, NTG, NTS, COMPACT, true, false>(D, L, n_tiles, cur, gmode, xo_at, xn_at);
 }
 // with the loop body (SRS_STEP_PROTOCOL), compiler's register budget only
 template <int BS, bool NTG, bool NTS, bool COMPACT>
 __global__ __launch_bounds__(BS) void k_sweep_gated_p(const RealDesc* __restrict__ D, RealLive* __restrict__ L,
-                                                      int n_tiles, int cur, uint32_t gmode) {
-  sweep_gated<BS, NTG, NTS, COMPACT, false, true>(D, L, n_tiles, cur, gmode);
+                                                      int n_tiles, int cur, uint32_t gmode, const double* xo_at,
+                                                      double* xn_at) {
+  sweep_gated<BS, NTG, NTS, COMPACT, false, true>(D, L, n_tiles, cur, gmode, xo_at, xn_at);
 }
 
 // ------------------------------------------------------------ kernel choice
@@ -745,7 +750,7 @@ __global__ __launch_bounds__(BS) void k_sweep_gated_p(const RealDesc* __restrict
 // compiles in parallel.
 
 using SweepFn = void (*)(const RealDesc*, const RealLive*, const int2*, int, const int*, int);
-using GatedFn = void (*)(const RealDesc*, RealLive*, int, int, uint32_t);
+using GatedFn = void (*)(const RealDesc*, RealLive*, int, int, uint32_t, const double*, double*);
 SweepFn pick_sweep(const srs_ctx* c);
 template <int BS>
 GatedFn pick_gated_bs(const srs_ctx* c, bool proto);
@@ -1036,12 +1041,12 @@ int launch_step(srs_ctx* c, uint32_t gmode) {
 // One step through k_sweep_gated: tile b - 1 in workgroup b, workgroup 0
 // gates (gmode) and updates the channel elements.  proto picks the kernel
 // with the loop body.
-void launch_gated(srs_ctx* c, int cur, uint32_t gmode, bool proto) {
+void launch_gated(srs_ctx* c, int cur, uint32_t gmode, bool proto, const double* xo_at, double* xn_at) {
   const GatedFn f = c->tile == 128 ? pick_gated_bs<128>(c, proto)
                     : c->tile == 512 ? pick_gated_bs<512>(c, proto) : pick_gated_bs<256>(c, proto);
   const int grid = (c->n_tiles + 1 + NXCD - 1) / NXCD * NXCD;
   hipLaunchKernelGGL(f, dim3(grid), dim3(c->tile), 0, c->stream, c->d_desc, c->d_live, c->n_tiles,
-                     cur | (c->alt_order ? 2 : 0), gmode);
+                     cur | (c->alt_order ? 2 : 0), gmode, xo_at, xn_at);
 }
 
 // m steps through k_sweep_gated: the first without gating (nothing is
@@ -1049,6 +1054,11 @@ void launch_gated(srs_ctx* c, int cur, uint32_t gmode, bool proto) {
 // then k_gate for the last step, so that a block leaves no gating pending.
 // cur is the live state index when the block starts; gmode is the step's
 // gating mode (G_AFTER_SWEEP, with G_PROTOCOL for the loop body).
+// The gating of a single realization's last gated step (k_gate alone).
+void launch_gate_single(srs_ctx* c, uint32_t gmode) {
+  hipLaunchKernelGGL(k_gate, dim3(1), dim3(GATE_THREADS), 0, c->stream, c->d_desc, c->d_live, gmode);
+}
+
 void launch_gated_block(srs_ctx* c, int64_t m, int cur, uint32_t gmode) {
   const bool proto = (gmode & G_PROTOCOL) != 0;
   for (int64_t k = 0; k < m; ++k) launch_gated(c, cur ^ (int)(k & 1), k ? gmode : 0u, proto);
