@@ -245,7 +245,7 @@ def bench_batched(n_real, steps, warmup, order, tune=None, n_tune=512):
                 plain_tunes=[ptunes[m] for m in range(3)], trials=trials)
 
 
-def bench_adjoint(t_sim=5.0, n_points=64, n_samples=50, ckpt=256, adj_waves=0):
+def bench_adjoint(t_sim=5.0, n_points=64, n_samples=50, ckpt=256, adj_waves=0, tune=None):
     """Adjoint workload: the h = 40 nm spark box, t_sim ms with the RyR
     schedule frozen from one gated run, a 64-point line scan along x through
     the cluster centre sampled 50 times, target from theta moved by +-10%.
@@ -254,7 +254,7 @@ def bench_adjoint(t_sim=5.0, n_points=64, n_samples=50, ckpt=256, adj_waves=0):
     from srs_amd import workloads as W
 
     t0 = time.perf_counter()
-    prob, info = W.spark_adjoint_problem(t_sim, n_points, n_samples, ckpt)
+    prob, info = W.spark_adjoint_problem(t_sim, n_points, n_samples, ckpt, tune=tune)
     prob.ctx.tune(adj_waves=adj_waves)
     t_setup = time.perf_counter() - t0
     th = prob.theta()

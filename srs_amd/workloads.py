@@ -387,14 +387,15 @@ def kuhn_block_order(g: Grid, block: int = 64) -> np.ndarray:
 
 
 def spark_adjoint_problem(t_sim: float = 5.0, n_points: int = 64, n_samples: int = 50, ckpt: int = 256,
-                          seed: int = 7, perturb: float = 0.1):
+                          seed: int = 7, perturb: float = 0.1, tune=None):
     """The fitting workload on the h = 40 nm spark box: t_sim ms with the RyR
     trajectory frozen from one gated run (LCC states held), a line scan of
     n_points along x through the cluster centre sampled n_samples times, and a
     target made by the same frozen-channel run with every parameter of theta
-    moved by +-perturb (signs drawn from `seed`).  Returns the
-    LinescanProblem (its context holds the realization and the scan) and
-    a dict with the sizes and the loss at the generating parameters."""
+    moved by +-perturb (signs drawn from `seed`); `tune`: sweep tunables of
+    the context.  Returns the LinescanProblem (its context holds the
+    realization and the scan) and a dict with the sizes and the loss at the
+    generating parameters."""
     from . import R, adjoint_theta, dff0, read_scan_parameters
     from . import Context
     from .fit import LinescanProblem
@@ -405,6 +406,7 @@ def spark_adjoint_problem(t_sim: float = 5.0, n_points: int = 64, n_samples: int
     dt = float(p.reals[R["T_STEP"]])
     n_steps = int(round(t_sim / dt))
     ctx = Context(0)
+    ctx.tune(**(tune or {}))
     mid = ctx.add_mesh_grid(g, pr.aij, pr.nij, perm=kuhn_block_order(g))
     ctx.add_realization(mid, g.ryr_ele, g.ryr_jsr, g.ryr_nb, g.lcc_ele, p, seed=38376427, stream=0)
     ctx.finalize()
