@@ -653,3 +653,21 @@ def test_adjoint_register_budget_does_not_change_the_gradient():
     ctx.close()
     assert np.array_equal(out[0][0], out[2][0]) and np.array_equal(out[0][1], out[2][1])
     assert out[1][0] == out[3][0] and np.array_equal(out[1][1], out[3][1])
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_checkpoint_levels_do_not_change_the_single_gradient(levels):
+    """A single realization with a multi-level checkpoint plan (its leaf
+    segments recomputed through the gated sweep into the segment's slots, the
+    levels above through separate sweep and replay launches in the
+    realization's own buffers) gives the one-level plan's loss and gradient
+    bit for bit."""
+    prob, _, _ = _setup()
+    th = prob.theta()
+    L1, g1 = prob.loss_grad(th)
+    prob.ctx.tune(adj_levels=levels)
+    Ln, gn = prob.loss_grad(th)
+    assert prob.ctx.adjoint_plan()["levels"] == levels
+    prob.ctx.tune(adj_levels=0)
+    prob.ctx.close()
+    assert Ln == L1 and np.array_equal(gn, g1)
