@@ -135,7 +135,7 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     const Geo g = load_geo<false, COMPACT, -1>(d, j);
     const uint8_t f = g.f;
     const d4* Q = (const d4*)X;
-    const d4 m0 = Q[j], m1 = Q[g.n1], m2 = Q[g.n2], m3 = Q[g.n3], m4 = Q[g.n4];
+    const d4 m0 = Q[j];
     const d4 w0 = w[j], w1 = w[g.n1], w2 = w[g.n2], w3 = w[g.n3], w4 = w[g.n4];
     const d4 mu = ((const d4*)lam)[j];
     const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
@@ -158,16 +158,20 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     double ld = mu.w + dt * P[SRS_R_D_DYE] * V0 * aw_d;
     double lT = muT, lC = muC;
 
-    // forward Ca rate before the SL division (for the SL terms) and the
-    // diffusion sums (for the D gradients)
-    const double ac = c * g.a0 + m1.x * g.a1 + m2.x * g.a2 + m3.x * g.a3 + m4.x * g.a4;
-    const double ad = bD * g.a0 + m1.w * g.a1 + m2.w * g.a2 + m3.w * g.a3 + m4.w * g.a4;
+    // The D gradients sum u_j (A c)_j and mu_Fluo,j (A b_Fluo)_j over the mesh;
+    // V_i A_ij = V_j A_ji turns those sums into sum_j c_j V0_j (A w_Ca)_j and
+    // sum_j b_Fluo,j V0_j (A w_Fluo)_j, terms of the element alone.  So the
+    // neighbours' states are gathered only for an SL element, whose SL terms
+    // need its full Ca rate before the division (rca; unused elsewhere).
+    double ac = 0.0;
+    if (f & F_SL)
+      ac = c * g.a0 + Q[g.n1].x * g.a1 + Q[g.n2].x * g.a2 + Q[g.n3].x * g.a3 + Q[g.n4].x * g.a4;
     // boundary rates are D times a D-free plane: they enter the D gradients
     const double bu_c = (f & F_BR) ? br_unit(d.br_mesh[j], d.brd[0]) : 0.0;
     const double bu_d = (f & F_BR) ? br_unit(d.br_mesh[np + j], d.brd[1]) : 0.0;
     double rca = P[SRS_R_D_CA] * ac + ((f & F_BR) ? d.br[j] : 0.0);
-    gr[0] += dt * u * (ac + bu_c);       // D_CA
-    gr[1] += dt * mu.w * (ad + bu_d);    // D_DYE
+    gr[0] += dt * (c * V0 * aw_c + u * bu_c);       // D_CA
+    gr[1] += dt * (bD * V0 * aw_d + mu.w * bu_d);   // D_DYE
     if (!(f & F_JSR)) {
       if (f & F_TROPC) {
         const double kon = P[SRS_R_K_ON_TRPN], koff = P[SRS_R_K_OFF_TRPN], B = P[SRS_R_B_TOT_TRPN];
