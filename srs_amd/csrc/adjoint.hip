@@ -141,7 +141,10 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
     const bool live_cs = (f & F_JSR) != 0;
     const double bT = live_tr ? X[4 * np + j] : 0.0, bC = live_cs ? X[5 * np + j] : 0.0;
-    const double muT = lam[4 * np + j], muC = lam[5 * np + j];
+    // the TnC and CSQ adjoints change only where the buffer lives (troponin
+    // cytosol, jSR); both lambda buffers start at zero, so elsewhere they stay
+    // zero and those planes are neither read nor written
+    const double muT = live_tr ? lam[4 * np + j] : 0.0, muC = live_cs ? lam[5 * np + j] : 0.0;
     const double c = m0.x, bA = m0.y, bM = m0.z, bD = m0.w;
     const double V0 = d.V0[j];
     const SlBuf sb = sl_buffer(d, j, c, f);
@@ -267,8 +270,8 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     }
     if (SCAN) ld += V0 * scan_adj_sum(nx.cen, nx.g, nx.n_f, nx.gcoef + (size_t)blockIdx.y * nx.gstride, np, j);
     ((d4*)lam_out)[j] = d4{lc, la, lm, ld};
-    lam_out[4 * np + j] = lT;
-    lam_out[5 * np + j] = lC;
+    if (live_tr) lam_out[4 * np + j] = lT;
+    if (live_cs) lam_out[5 * np + j] = lC;
     if (nx.Xprev) {
       const double* Xp = nx.Xprev + blockIdx.y * 6 * np;
       nx.wout[blockIdx.y * np + j] = adj_weight(d, j, (f & F_SL) ? Xp[4 * (size_t)j] : 0.0, f, lc, la, lm, ld);
@@ -741,6 +744,7 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
     if (want_grad) {
       e = hipMemcpy(dg, gco.data(), gco.size() * sizeof(double), hipMemcpyHostToDevice);
       if (e == hipSuccess) e = hipMemsetAsync(lam0, 0, (size_t)nb * n6 * sizeof(double), c->stream);
+      if (e == hipSuccess) e = hipMemsetAsync(lam1, 0, (size_t)nb * n6 * sizeof(double), c->stream);
       if (e == hipSuccess) e = hipMemsetAsync(gpart, 0, (size_t)nb * nblk * ADJ_NT * sizeof(double), c->stream);
       double* lam = lam0;   // lambda_{n+1}
       double* lout = lam1;  // lambda_n
