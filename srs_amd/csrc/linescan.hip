@@ -86,10 +86,16 @@ __global__ void k_scan_final(const RealDesc* __restrict__ D, const RealLive* __r
   part += (size_t)blockIdx.y * n_blocks * n_f;
   out += (size_t)blockIdx.y * out_stride;
   const double* X = D[r].X[L[r].cur];
-  for (int i = threadIdx.x; i < n_f; i += blockDim.x) {
+  // a wavefront per point: each lane sums every 64th workgroup's partial,
+  // then the lanes' sums by DPP (a fixed order: deterministic)
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, n_waves = blockDim.x >> 6;
+  for (int i = wave; i < n_f; i += n_waves) {
     double s = 0.0;
-    for (int b = 0; b < n_blocks; ++b) s += part[(size_t)b * n_f + i];
-    out[n_ca + n_f + i] = s + pad[i];
+    for (int b = lane; b < n_blocks; b += 64) s += part[(size_t)b * n_f + i];
+    s = wave_sum_dpp(s);
+    if (lane == 63) out[n_ca + n_f + i] = s + pad[i];
+  }
+  for (int i = threadIdx.x; i < n_f; i += blockDim.x) {
     const int e = near_f[i];
     out[n_ca + i] = e >= 0 ? X[cq(D[r], e) + 3] : f0;
   }
