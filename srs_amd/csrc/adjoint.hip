@@ -43,8 +43,8 @@ __device__ __forceinline__ SlBuf sl_buffer(const RealDesc& d, int j, double c, u
 
 // The loss source of one scan sample at element j: sum_i gcoef_i w(c_j - p_i)
 // over points within 1 um (the blur's PSF, linescan/parameters.h:60-62), with
-// the blur's own weights (the walk from the anchor at or below the first
-// point the cell can reach).
+// the blur's own points and weights (the cell's run of points within the
+// cut-off; the walk from the anchor at or below its first point).
 __device__ __forceinline__ double scan_adj_sum(const double* __restrict__ cen, const ScanGeo& g, int n_f,
                                                const double* __restrict__ gcoef, size_t np, int j) {
   const double cx = cen[j], cy = cen[np + j], cz = cen[2 * np + j];
@@ -52,12 +52,13 @@ __device__ __forceinline__ double scan_adj_sum(const double* __restrict__ cen, c
   int i_lo, i_hi;
   scan_point_range(g, n_f, cx, cy, cz, &i_lo, &i_hi);
   const ScanCell c = scan_cell(g, cx, cy, cz);
+  scan_exact_range(g, c, &i_lo, &i_hi);
   const int i0 = scan_walk_start(g, i_lo);
   double w = 0.0, r = 0.0;
   for (int i = i0; i <= i_hi; ++i) {
     const double pa = scan_pa(g, i);
     if (scan_is_anchor(g, i, i0)) scan_anchor(g, c, pa, &w, &r);
-    if (i >= i_lo && scan_dist2(c, pa) < 1.0) s += gcoef[i] * w;
+    if (i >= i_lo) s += gcoef[i] * w;
     w *= r;
     r *= g.q;
   }

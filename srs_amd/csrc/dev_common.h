@@ -281,6 +281,16 @@ __device__ __forceinline__ void scan_anchor(const ScanGeo& g, const ScanCell& c,
   *w = g.kw * exp(-0.5 * (da * da * ia + c.eperp));
   *r = exp(g.res * (da - 0.5 * g.res) * ia);
 }
+// The points within the reference's exact cut-off of a cell, [lo, hi] (none
+// if lo > hi), inside a superset [lo, hi] (scan_point_range).  The point
+// coordinate x_min + i res rises with i and every rounding step is monotonic,
+// so the distance^2 falls and then rises along the line: the points that pass
+// are one run, found by testing its ends, and the callers' per-point test
+// becomes a range check (the same points, the same bits).
+__device__ __forceinline__ void scan_exact_range(const ScanGeo& g, const ScanCell& c, int* lo, int* hi) {
+  while (*lo <= *hi && !(scan_dist2(c, scan_pa(g, *lo)) < 1.0)) ++*lo;
+  while (*hi >= *lo && !(scan_dist2(c, scan_pa(g, *hi)) < 1.0)) --*hi;
+}
 // The first point of a walk that covers lo: its anchor.
 __device__ __forceinline__ int scan_walk_start(const ScanGeo& g, int lo) {
   return g.walk ? lo - lo % SCAN_ANCHOR : lo;

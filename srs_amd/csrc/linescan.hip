@@ -32,14 +32,19 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_blur(const RealDesc* __restric
   // without the loop (the same bits as the full loop).
   int lo = n_f, hi = -1;
   ScanCell sc[SCAN_CELLS_PER_THREAD];
+  int clo[SCAN_CELLS_PER_THREAD], chi[SCAN_CELLS_PER_THREAD];  // each cell's points within the cut-off
   for (int q = 0; q < SCAN_CELLS_PER_THREAD; ++q) {
     sc[q] = scan_cell(g, cx[q], cy[q], cz[q]);
     const int j = (blockIdx.x * SCAN_CELLS_PER_THREAD + q) * SCAN_BS + threadIdx.x;
+    clo[q] = n_f;
+    chi[q] = -1;
     if (j < n) {
-      int a, b;
-      scan_point_range(g, n_f, cx[q], cy[q], cz[q], &a, &b);
-      lo = min(lo, a);
-      hi = max(hi, b);
+      scan_point_range(g, n_f, cx[q], cy[q], cz[q], &clo[q], &chi[q]);
+      scan_exact_range(g, sc[q], &clo[q], &chi[q]);
+      if (clo[q] <= chi[q]) {
+        lo = min(lo, clo[q]);
+        hi = max(hi, chi[q]);
+      }
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -50,9 +55,9 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_blur(const RealDesc* __restric
     for (int i = 0; i < min(lo, n_f); ++i) red[wave][i] = 0.0;
     for (int i = max(hi + 1, 0); i < n_f; ++i) red[wave][i] = 0.0;
   }
-  // Each cell's weights by the walk from the anchor at or below lo (the
-  // cut-off test is the reference's exact one, so a weight that is never
-  // used may be anything).
+  // Each cell's weights by the walk from the anchor at or below lo (a weight
+  // outside the cell's run of points within the cut-off is never used, so it
+  // may be anything).
   const int i0 = scan_walk_start(g, lo);
   double ww[SCAN_CELLS_PER_THREAD], wr[SCAN_CELLS_PER_THREAD];
   for (int i = i0; i <= hi; ++i) {
@@ -61,7 +66,7 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_blur(const RealDesc* __restric
     double acc = 0.0;
     for (int q = 0; q < SCAN_CELLS_PER_THREAD; ++q) {
       if (anchor) scan_anchor(g, sc[q], pa, &ww[q], &wr[q]);
-      if (scan_dist2(sc[q], pa) < 1.0) acc += vf[q] * ww[q];
+      if (i >= clo[q] && i <= chi[q]) acc += vf[q] * ww[q];
       ww[q] *= wr[q];
       wr[q] *= g.q;
     }
