@@ -44,21 +44,38 @@ __device__ __forceinline__ SlBuf sl_buffer(const RealDesc& d, int j, double c, u
 // The loss source of one scan sample at element j: sum_i gcoef_i w(c_j - p_i)
 // over points within 1 um (the blur's PSF, linescan/parameters.h:60-62), with
 // the blur's own points and weights (the cell's run of points within the
-// cut-off; the walk from the anchor at or below its first point).
+// cut-off).  Every lane of the wavefront calls it (valid false: no element,
+// sum zero).  The loop runs over the points any lane can reach, the same for
+// the whole wavefront, so gcoef_i is one uniform load per point; each lane's
+// weights walk from anchors at multiples of the anchor spacing, so a lane's
+// weight at a point of its own run is the one its own walk would give (a
+// weight outside the run is never used, so it may be anything).
 __device__ __forceinline__ double scan_adj_sum(const double* __restrict__ cen, const ScanGeo& g, int n_f,
-                                               const double* __restrict__ gcoef, size_t np, int j) {
-  const double cx = cen[j], cy = cen[np + j], cz = cen[2 * np + j];
+                                               const double* __restrict__ gcoef, size_t np, int j, bool valid) {
+  int i_lo = n_f, i_hi = -1;
+  ScanCell c{0.0, 0.0, 0.0, 0.0};
+  if (valid) {
+    const double cx = cen[j], cy = cen[np + j], cz = cen[2 * np + j];
+    scan_point_range(g, n_f, cx, cy, cz, &i_lo, &i_hi);
+    c = scan_cell(g, cx, cy, cz);
+    scan_exact_range(g, c, &i_lo, &i_hi);
+  }
+  int lo = i_lo <= i_hi ? i_lo : n_f, hi = i_lo <= i_hi ? i_hi : -1;
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o));
+    hi = max(hi, __shfl_xor(hi, o));
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  hi = __builtin_amdgcn_readfirstlane(hi);
   double s = 0.0;
-  int i_lo, i_hi;
-  scan_point_range(g, n_f, cx, cy, cz, &i_lo, &i_hi);
-  const ScanCell c = scan_cell(g, cx, cy, cz);
-  scan_exact_range(g, c, &i_lo, &i_hi);
-  const int i0 = scan_walk_start(g, i_lo);
+  if (lo > hi) return s;
+  const int i0 = __builtin_amdgcn_readfirstlane(scan_walk_start(g, lo));
   double w = 0.0, r = 0.0;
-  for (int i = i0; i <= i_hi; ++i) {
+  for (int i = i0; i <= hi; ++i) {
     const double pa = scan_pa(g, i);
     if (scan_is_anchor(g, i, i0)) scan_anchor(g, c, pa, &w, &r);
-    if (i >= i_lo) s += gcoef[i] * w;
+    const double gi = gcoef[i];
+    if (i >= i_lo && i <= i_hi) s += gi * w;
     w *= r;
     r *= g.q;
   }
@@ -130,6 +147,9 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
   double gr[ADJ_NT];
 #pragma unroll
   for (int k = 0; k < ADJ_NT; ++k) gr[k] = 0.0;
+  // the scan sample's loss source (all lanes of the wavefront take part)
+  const double src =
+      SCAN ? scan_adj_sum(nx.cen, nx.g, nx.n_f, nx.gcoef + (size_t)blockIdx.y * nx.gstride, np, j, j < d.n) : 0.0;
   if (j < d.n) {
     const double* P = d.P;
     const double dt = P[SRS_R_T_STEP];
@@ -269,7 +289,7 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
       lc += dt * mu.x * (-rca * sb.dbeta_dc / b2);
       gr[12] += dt * mu.x * (-rca * sb.dbeta_dkh / b2);
     }
-    if (SCAN) ld += V0 * scan_adj_sum(nx.cen, nx.g, nx.n_f, nx.gcoef + (size_t)blockIdx.y * nx.gstride, np, j);
+    if (SCAN) ld += V0 * src;
     ((d4*)lam_out)[j] = d4{lc, la, lm, ld};
     if (live_tr) lam_out[4 * np + j] = lT;
     if (live_cs) lam_out[5 * np + j] = lC;
@@ -322,10 +342,11 @@ __global__ __launch_bounds__(ADJ_BS) void k_scan_adj(const RealDesc* __restrict_
                                                      double* __restrict__ lam) {
   const RealDesc& d = D[rid[blockIdx.y]];
   const int j = blockIdx.x * ADJ_BS + threadIdx.x;
-  if (j >= d.n) return;
   const size_t np = d.np;
+  const double s = scan_adj_sum(cen, g, n_f, gcoef + (size_t)blockIdx.y * gstride, np, j, j < d.n);
+  if (j >= d.n) return;
   lam += blockIdx.y * 6 * np;
-  lam[4 * (size_t)j + 3] += d.V0[j] * scan_adj_sum(cen, g, n_f, gcoef + (size_t)blockIdx.y * gstride, np, j);
+  lam[4 * (size_t)j + 3] += d.V0[j] * s;
 }
 
 // Copies the current state (X[cur]) of each realization of a table to slot y

@@ -246,6 +246,8 @@ struct ScanGeo {
   double x_min, res;
   double sx, sy, sz, alpha;
   double inv[3];    // 1 / s^2 per coordinate (s = sx, sy, sz)
+  double ia;        // inv[axis] (device code indexes no array by axis: the
+                    // struct then stays in scalar registers)
   double kw;        // alpha / (sx sy sz)
   double q;         // exp(-res^2 inv[axis])
   int walk;         // 1: the recurrence above; 0: the closed form everywhere
@@ -277,7 +279,7 @@ __device__ __forceinline__ double scan_dist2(const ScanCell& c, double pa) {
 }
 // Weight at point i from the closed form and the ratio to point i + 1.
 __device__ __forceinline__ void scan_anchor(const ScanGeo& g, const ScanCell& c, double pa, double* w, double* r) {
-  const double da = c.ca - pa, ia = g.inv[g.axis];
+  const double da = c.ca - pa, ia = g.ia;
   *w = g.kw * exp(-0.5 * (da * da * ia + c.eperp));
   *r = exp(g.res * (da - 0.5 * g.res) * ia);
 }
@@ -306,18 +308,19 @@ __device__ __forceinline__ bool scan_is_anchor(const ScanGeo& g, int i, int i0) 
 // rounding of the exact test, which the callers still apply.
 __device__ __forceinline__ void scan_point_range(const ScanGeo& g, int n_f, double cx, double cy, double cz, int* lo,
                                                  int* hi) {
-  const double c[3] = {cx, cy, cz};
+  const double ca = g.axis == 0 ? cx : g.axis == 1 ? cy : cz;
   double t2 = 0.0;
-  for (int a = 0; a < 3; ++a)
-    if (a != g.axis) t2 += (c[a] - g.base[a]) * (c[a] - g.base[a]);
+  if (g.axis != 0) t2 += (cx - g.base[0]) * (cx - g.base[0]);
+  if (g.axis != 1) t2 += (cy - g.base[1]) * (cy - g.base[1]);
+  if (g.axis != 2) t2 += (cz - g.base[2]) * (cz - g.base[2]);
   if (!(t2 < 1.0 + 1e-9)) {
     *lo = n_f;
     *hi = -1;
     return;
   }
   const double half = sqrt(1.0 + 1e-9 - t2);
-  *lo = max(0, (int)floor((c[g.axis] - half - g.x_min) / g.res) - 1);
-  *hi = min(n_f - 1, (int)ceil((c[g.axis] + half - g.x_min) / g.res) + 1);
+  *lo = max(0, (int)floor((ca - half - g.x_min) / g.res) - 1);
+  *hi = min(n_f - 1, (int)ceil((ca + half - g.x_min) / g.res) + 1);
 }
 
 constexpr int SCAN_BS = 256;

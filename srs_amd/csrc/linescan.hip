@@ -150,6 +150,7 @@ int srs_ctx_add_scan(srs_ctx* c, int32_t mesh_id, const double* centers, const s
   g.inv[0] = 1.0 / (g.sx * g.sx);
   g.inv[1] = 1.0 / (g.sy * g.sy);
   g.inv[2] = 1.0 / (g.sz * g.sz);
+  g.ia = g.inv[g.axis];
   g.kw = g.alpha / (g.sx * g.sy * g.sz);
   {
     // The walk is safe when an anchor SCAN_ANCHOR - 1 points before a point
