@@ -127,6 +127,7 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
                                          const double* __restrict__ lam, const d4* __restrict__ w, int64_t step,
                                          double* __restrict__ lam_out, double* __restrict__ gpart, AdjNext nx) {
   __shared__ double red[ADJ_BS / 64][ADJ_NT];
+  __shared__ d4 tw[ADJ_BS];  // the tile's own w: neighbours inside the tile read it from here
   const int r = rid[blockIdx.y];
   const RealDesc& d = D[r];
   const RealLive& l = L[r];
@@ -147,17 +148,31 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
   double gr[ADJ_NT];
 #pragma unroll
   for (int k = 0; k < ADJ_NT; ++k) gr[k] = 0.0;
+  // the element's operator row and own w, then the tile's w staged in LDS
+  const bool in = j < d.n;
+  Geo g{};
+  d4 w0 = {0.0, 0.0, 0.0, 0.0};
+  if (in) {
+    g = load_geo<false, COMPACT, -1>(d, j);
+    w0 = w[j];
+  }
+  tw[threadIdx.x] = w0;
+  __syncthreads();
   // the scan sample's loss source (all lanes of the wavefront take part)
   const double src =
-      SCAN ? scan_adj_sum(nx.cen, nx.g, nx.n_f, nx.gcoef + (size_t)blockIdx.y * nx.gstride, np, j, j < d.n) : 0.0;
-  if (j < d.n) {
+      SCAN ? scan_adj_sum(nx.cen, nx.g, nx.n_f, nx.gcoef + (size_t)blockIdx.y * nx.gstride, np, j, in) : 0.0;
+  if (in) {
     const double* P = d.P;
     const double dt = P[SRS_R_T_STEP];
-    const Geo g = load_geo<false, COMPACT, -1>(d, j);
     const uint8_t f = g.f;
     const d4* Q = (const d4*)X;
     const d4 m0 = Q[j];
-    const d4 w0 = w[j], w1 = w[g.n1], w2 = w[g.n2], w3 = w[g.n3], w4 = w[g.n4];
+    const int base = tile * ADJ_BS;
+    auto nw = [&](int nb) -> d4 {
+      const unsigned k = (unsigned)(nb - base);
+      return k < (unsigned)ADJ_BS ? tw[k] : w[nb];
+    };
+    const d4 w1 = nw(g.n1), w2 = nw(g.n2), w3 = nw(g.n3), w4 = nw(g.n4);
     const d4 mu = ((const d4*)lam)[j];
     const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
     const bool live_cs = (f & F_JSR) != 0;
@@ -316,8 +331,8 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
   }
 }
 
-// The compiler's register budget (144 VGPRs: three wavefronts per SIMD), or
-// at most 128 (four) with a few spills.
+// The compiler's register budget (126 VGPRs: four wavefronts per SIMD), or
+// at most 128 (four) asked for explicitly; the two compile alike.
 template <bool COMPACT, bool SCAN>
 __global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
                                                      const int32_t* __restrict__ rid, const double* __restrict__ X,
