@@ -92,40 +92,26 @@ __device__ __forceinline__ d4 adj_weight(const RealDesc& d, int j, double c, uin
   return d4{la_c / sb.beta * iv, la_a * iv, la_m * iv, la_d * iv};
 }
 
-// Pass 1 (the first step of a recomputed segment; later steps get w from the
-// step kernel): w = (lambda_Ca / beta, lambda_ATP, lambda_CaM, lambda_Fluo) / V0.
-__global__ __launch_bounds__(ADJ_BS) void k_adj_w(const RealDesc* __restrict__ D, const int32_t* __restrict__ rid,
-                                                  const double* __restrict__ X, const double* __restrict__ lam,
-                                                  d4* __restrict__ w) {
-  const RealDesc& d = D[rid[blockIdx.y]];
-  const int j = blockIdx.x * ADJ_BS + threadIdx.x;
-  if (j >= d.n) return;
-  X += blockIdx.y * 6 * (size_t)d.np;
-  lam += blockIdx.y * 6 * (size_t)d.np;
-  w += blockIdx.y * (size_t)d.np;
-  const d4 m = ((const d4*)lam)[j];
-  const uint8_t f = d.flags[j];
-  w[j] = adj_weight(d, j, (f & F_SL) ? X[4 * (size_t)j] : 0.0, f, m.x, m.y, m.z, m.w);
-}
-
 // What the step kernel does after lambda_n: add the loss source of a scan
-// sample taken after n steps (SCAN), and form w for step n - 1 from lambda_n
-// and that step's state (Xprev, when it is in the recomputed segment).
+// sample taken after n steps (SCAN).
 struct AdjNext {
-  const double* Xprev;  // state of step n - 1, or null
-  d4* wout;             // its w
   const double* cen;    // scan cell centres [3][np]
   const double* gcoef;  // the sample's loss coefficients per point
   ScanGeo g;
   int n_f, gstride;
 };
 
-// Pass 2: lambda_n and the theta-gradient partials of one step.
+// lambda_n and the theta-gradient partials of one step.  The transposed
+// diffusion gathers w = (lambda_Ca / beta, lambda_ATP, lambda_CaM,
+// lambda_Fluo) / V0 of lambda_{n+1} and this step's state: each element forms
+// its own and stages it in LDS for the tile, and a neighbour outside the tile
+// forms it from its lambda (the same operations, so the same bits as a w
+// written by the step before, with no w buffer in memory).
 template <bool COMPACT, bool SCAN>
 __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
                                          const int32_t* __restrict__ rid, const double* __restrict__ X,
-                                         const double* __restrict__ lam, const d4* __restrict__ w, int64_t step,
-                                         double* __restrict__ lam_out, double* __restrict__ gpart, AdjNext nx) {
+                                         const double* __restrict__ lam, int64_t step, double* __restrict__ lam_out,
+                                         double* __restrict__ gpart, AdjNext nx) {
   __shared__ double red[ADJ_BS / 64][ADJ_NT];
   __shared__ d4 tw[ADJ_BS];  // the tile's own w: neighbours inside the tile read it from here
   const int r = rid[blockIdx.y];
@@ -139,7 +125,6 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
   X += blockIdx.y * 6 * np;
   lam += blockIdx.y * 6 * np;
   lam_out += blockIdx.y * 6 * np;
-  w += blockIdx.y * np;
   gpart += (size_t)blockIdx.y * gridDim.x * ADJ_NT;
   // the tile's running partials, loaded now so the final update waits on no load
   const double gp_old = threadIdx.x < ADJ_NT ? gpart[(size_t)tile * ADJ_NT + threadIdx.x] : 0.0;
@@ -148,13 +133,23 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
   double gr[ADJ_NT];
 #pragma unroll
   for (int k = 0; k < ADJ_NT; ++k) gr[k] = 0.0;
-  // the element's operator row and own w, then the tile's w staged in LDS
+  // the element's operator row, state, lambda_{n+1} and w; the tile's w
+  // staged in LDS
   const bool in = j < d.n;
+  const d4* Q = (const d4*)X;
+  const d4* Lm = (const d4*)lam;
   Geo g{};
-  d4 w0 = {0.0, 0.0, 0.0, 0.0};
+  d4 m0 = {0.0, 0.0, 0.0, 0.0}, mu = {0.0, 0.0, 0.0, 0.0}, w0 = {0.0, 0.0, 0.0, 0.0};
+  double V0 = 1.0;
+  SlBuf sb{1.0, 0.0, 0.0};
   if (in) {
     g = load_geo<false, COMPACT, -1>(d, j);
-    w0 = w[j];
+    m0 = Q[j];
+    mu = Lm[j];
+    V0 = d.V0[j];
+    sb = sl_buffer(d, j, m0.x, g.f);
+    const double iv = 1.0 / V0;
+    w0 = d4{mu.x / sb.beta * iv, mu.y * iv, mu.z * iv, mu.w * iv};
   }
   tw[threadIdx.x] = w0;
   __syncthreads();
@@ -165,15 +160,15 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     const double* P = d.P;
     const double dt = P[SRS_R_T_STEP];
     const uint8_t f = g.f;
-    const d4* Q = (const d4*)X;
-    const d4 m0 = Q[j];
     const int base = tile * ADJ_BS;
     auto nw = [&](int nb) -> d4 {
       const unsigned k = (unsigned)(nb - base);
-      return k < (unsigned)ADJ_BS ? tw[k] : w[nb];
+      if (k < (unsigned)ADJ_BS) return tw[k];
+      const uint8_t fn = d.flags[nb];
+      const d4 ln = Lm[nb];
+      return adj_weight(d, nb, (fn & F_SL) ? Q[nb].x : 0.0, fn, ln.x, ln.y, ln.z, ln.w);
     };
     const d4 w1 = nw(g.n1), w2 = nw(g.n2), w3 = nw(g.n3), w4 = nw(g.n4);
-    const d4 mu = ((const d4*)lam)[j];
     const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
     const bool live_cs = (f & F_JSR) != 0;
     const double bT = live_tr ? X[4 * np + j] : 0.0, bC = live_cs ? X[5 * np + j] : 0.0;
@@ -182,8 +177,6 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     // zero and those planes are neither read nor written
     const double muT = live_tr ? lam[4 * np + j] : 0.0, muC = live_cs ? lam[5 * np + j] : 0.0;
     const double c = m0.x, bA = m0.y, bM = m0.z, bD = m0.w;
-    const double V0 = d.V0[j];
-    const SlBuf sb = sl_buffer(d, j, c, f);
     const double u = mu.x / sb.beta;
 
     // transposed diffusion: dt D_s V0 (A w_s)_j
@@ -308,10 +301,6 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     ((d4*)lam_out)[j] = d4{lc, la, lm, ld};
     if (live_tr) lam_out[4 * np + j] = lT;
     if (live_cs) lam_out[5 * np + j] = lC;
-    if (nx.Xprev) {
-      const double* Xp = nx.Xprev + blockIdx.y * 6 * np;
-      nx.wout[blockIdx.y * np + j] = adj_weight(d, j, (f & F_SL) ? Xp[4 * (size_t)j] : 0.0, f, lc, la, lm, ld);
-    }
   }
   // block reduction of the gradient partials, accumulated per block: DPP
   // sums within each wavefront (a component that is zero in every lane, as
@@ -336,17 +325,17 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
 template <bool COMPACT, bool SCAN>
 __global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
                                                      const int32_t* __restrict__ rid, const double* __restrict__ X,
-                                                     const double* __restrict__ lam, const d4* __restrict__ w,
-                                                     int64_t step, double* __restrict__ lam_out,
-                                                     double* __restrict__ gpart, AdjNext nx) {
-  adj_step<COMPACT, SCAN>(D, L, rid, X, lam, w, step, lam_out, gpart, nx);
+                                                     const double* __restrict__ lam, int64_t step,
+                                                     double* __restrict__ lam_out, double* __restrict__ gpart,
+                                                     AdjNext nx) {
+  adj_step<COMPACT, SCAN>(D, L, rid, X, lam, step, lam_out, gpart, nx);
 }
 template <bool COMPACT, bool SCAN>
 __global__ __launch_bounds__(ADJ_BS) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_step_w4(
     const RealDesc* __restrict__ D, const RealLive* __restrict__ L, const int32_t* __restrict__ rid,
-    const double* __restrict__ X, const double* __restrict__ lam, const d4* __restrict__ w, int64_t step,
-    double* __restrict__ lam_out, double* __restrict__ gpart, AdjNext nx) {
-  adj_step<COMPACT, SCAN>(D, L, rid, X, lam, w, step, lam_out, gpart, nx);
+    const double* __restrict__ X, const double* __restrict__ lam, int64_t step, double* __restrict__ lam_out,
+    double* __restrict__ gpart, AdjNext nx) {
+  adj_step<COMPACT, SCAN>(D, L, rid, X, lam, step, lam_out, gpart, nx);
 }
 
 // The loss gradient of one scan sample in the Fluo adjoint:
@@ -604,7 +593,7 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
   // device bytes per member of a sub-batch under a plan (a loss alone keeps
   // only the start state)
   auto per_member = [&](const AdjPlan& p) {
-    return 8 * n6 * (size_t)(want_grad ? p.kept() + p.K[p.L - 1] + 2 : 1) + (want_grad ? 64 * np : 0) +
+    return 8 * n6 * (size_t)(want_grad ? p.kept() + p.K[p.L - 1] + 2 : 1) +
            8 * (size_t)n_samples * nout + 8 * (size_t)S.n_blocks * nf + 8 * (size_t)nblk * ADJ_NT +
            8 * (size_t)n_samples * nf + 4 * max_sched + 64;
   };
@@ -650,7 +639,6 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
   double *ck = nullptr, *samp = nullptr, *part = nullptr, *dg = nullptr, *gpart = nullptr;
   double *lam0 = nullptr, *lam1 = nullptr, *seg = nullptr;
   std::vector<double*> ckl(plan.L, nullptr);  // checkpoint levels; ckl[0] = ck, the forward's
-  d4 *w0 = nullptr, *w1 = nullptr;
   int32_t *d_sched = nullptr, *d_rid = nullptr;
   int2* d_sub_tiles = nullptr;
   const int nt_mesh = (hm.n + c->tile - 1) / c->tile;
@@ -664,8 +652,6 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
   if (want_grad) {
     HIPCHK(M.alloc(&lam0, (size_t)bs * n6));
     HIPCHK(M.alloc(&lam1, (size_t)bs * n6));
-    HIPCHK(M.alloc(&w0, (size_t)bs * np));
-    HIPCHK(M.alloc(&w1, (size_t)bs * np));
     HIPCHK(M.alloc(&seg, (size_t)leaf * bs * n6));
     for (int l = 1; l < plan.L; ++l) HIPCHK(M.alloc(&ckl[l], (size_t)plan.slots[l] * bs * n6));
     HIPCHK(M.alloc(&gpart, (size_t)bs * nblk * ADJ_NT));
@@ -785,8 +771,6 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
       if (e == hipSuccess) e = hipMemsetAsync(gpart, 0, (size_t)nb * nblk * ADJ_NT * sizeof(double), c->stream);
       double* lam = lam0;   // lambda_{n+1}
       double* lout = lam1;  // lambda_n
-      d4* wcur = w0;        // w of step n
-      d4* wnext = w1;       // w of step n - 1
       ks = n_samples - 1;
       // the loss source of a sample after the last step; the step kernel
       // adds the others to the lambda it produces
@@ -825,21 +809,17 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
         }
         for (int64_t n = s1 - 1; n >= s0 && e2 == hipSuccess; --n) {
           const double* Xn = seg + (size_t)(n - s0) * nb * n6;
-          if (n == s1 - 1)
-            hipLaunchKernelGGL(k_adj_w, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, d_rid, Xn, lam, wcur);
           const bool scan = ks >= 0 && sample_steps[ks] == n;
-          AdjNext nx{n > s0 ? Xn - nb * n6 : nullptr, wnext, S.d_cen, scan ? dg + (size_t)ks * nf : nullptr,
-                     S.geo, nf, n_samples * nf};
+          AdjNext nx{S.d_cen, scan ? dg + (size_t)ks * nf : nullptr, S.geo, nf, n_samples * nf};
           if (scan) --ks;
           auto kern = c->adj_waves == 4
                           ? (c->compact_all ? (scan ? k_adj_step_w4<true, true> : k_adj_step_w4<true, false>)
                                             : (scan ? k_adj_step_w4<false, true> : k_adj_step_w4<false, false>))
                           : (c->compact_all ? (scan ? k_adj_step<true, true> : k_adj_step<true, false>)
                                             : (scan ? k_adj_step<false, true> : k_adj_step<false, false>));
-          hipLaunchKernelGGL(kern, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, d_rid, Xn, lam, wcur, n,
-                             lout, gpart, nx);
+          hipLaunchKernelGGL(kern, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, d_rid, Xn, lam, n, lout,
+                             gpart, nx);
           std::swap(lam, lout);
-          std::swap(wcur, wnext);
           e2 = hipGetLastError();
         }
         if (e2 == hipSuccess) e2 = hipStreamSynchronize(c->stream);
