@@ -112,7 +112,7 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
                                          const int32_t* __restrict__ rid, const double* __restrict__ X,
                                          const double* __restrict__ lam, int64_t step, double* __restrict__ lam_out,
                                          double* __restrict__ gpart, AdjNext nx) {
-  __shared__ double red[ADJ_BS / 64][ADJ_NT];
+  __shared__ double red[ADJ_BS / 16][ADJ_NT];  // gradient partials per row of 16 lanes
   __shared__ d4 tw[ADJ_BS];  // the tile's own w: neighbours inside the tile read it from here
   const int r = rid[blockIdx.y];
   const RealDesc& d = D[r];
@@ -303,19 +303,18 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     if (live_cs) lam_out[5 * np + j] = lC;
   }
   // block reduction of the gradient partials, accumulated per block: DPP
-  // sums within each wavefront (a component that is zero in every lane, as
-  // the channel and SL terms are in most wavefronts, is not summed), then
-  // the four wavefronts in LDS
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // sums within each row of 16 lanes (a component that is zero in every lane
+  // of the wavefront, as the channel and SL terms are in most, is not
+  // summed), then the block's rows in LDS in a fixed order
 #pragma unroll
   for (int k = 0; k < ADJ_NT; ++k) {
-    const double v = __builtin_amdgcn_ballot_w64(gr[k] != 0.0) ? wave_sum_dpp(gr[k]) : 0.0;
-    if (lane == 63) red[wave][k] = v;
+    const double v = __builtin_amdgcn_ballot_w64(gr[k] != 0.0) ? row_sum_dpp(gr[k]) : 0.0;
+    if ((threadIdx.x & 15) == 15) red[threadIdx.x >> 4][k] = v;
   }
   __syncthreads();
   if (threadIdx.x < ADJ_NT) {
     double s = 0.0;
-    for (int q = 0; q < ADJ_BS / 64; ++q) s += red[q][threadIdx.x];
+    for (int q = 0; q < ADJ_BS / 16; ++q) s += red[q][threadIdx.x];
     gpart[(size_t)tile * ADJ_NT + threadIdx.x] = gp_old + s;
   }
 }

@@ -149,11 +149,16 @@ __device__ __forceinline__ double dpp_f64(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, true);
   return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ double wave_sum_dpp(double v) {
+// The first four steps alone: each row's sum in its lane 15.
+__device__ __forceinline__ double row_sum_dpp(double v) {
   v += dpp_f64<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
   v += dpp_f64<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
   v += dpp_f64<0x114, 0xf>(v);  // row_shr:4
   v += dpp_f64<0x118, 0xf>(v);  // row_shr:8
+  return v;
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v = row_sum_dpp(v);
   v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
   v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
   return v;
