@@ -308,6 +308,11 @@ int srs_ctx_create(int device, srs_ctx** out);
                                     wrote last; 0 (default): one order */
 #define SRS_TUNE_ADJ_WAVES 14    /* adjoint reverse step's register budget: 0 (default) the compiler's, 4: at
                                     most 128 VGPRs so that four wavefronts share each SIMD */
+#define SRS_TUNE_ADJ_W 15        /* adjoint reverse step's w = (lambda_Ca / beta, lambda_ATP, lambda_CaM,
+                                    lambda_Fluo) / V0: 0 (default) written by each step for the next on
+                                    launches of fewer than 4096 workgroups (tiles x members), else formed
+                                    in the kernel from lambda; 1: always written (two more state-sized
+                                    buffers per member); 2: always formed.  Same results */
 #define SRS_TUNE_ADJ_LEVELS 11   /* adjoint checkpoint levels (srs_linescan_loss_grad*): 0 (default) the
                                     fewest whose memory fits the sub-batch, 1..6 forced; each level below the
                                     first is one more forward pass and needs about (L+1) n^(1/(L+1)) states */

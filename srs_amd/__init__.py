@@ -69,6 +69,7 @@ TUNE_ADJ_LEVELS = 11
 TUNE_SUBSET_MB = 12
 TUNE_ALT_ORDER = 13
 TUNE_ADJ_WAVES = 14
+TUNE_ADJ_W = 15
 
 
 def _p(a, t):
@@ -356,7 +357,8 @@ class Context:
         keys = dict(tile=TUNE_TILE, nt_geometry=TUNE_NT_GEOMETRY, nt_state=TUNE_NT_STATE, plane_pad=TUNE_PLANE_PAD,
                     compact=TUNE_COMPACT, occupancy=TUNE_OCCUPANCY, group=TUNE_GROUP, lds=TUNE_LDS,
                     drop_finished=TUNE_DROP_FINISHED, gated_sweep=TUNE_GATED_SWEEP, adj_levels=TUNE_ADJ_LEVELS,
-                    subset_mb=TUNE_SUBSET_MB, alt_order=TUNE_ALT_ORDER, adj_waves=TUNE_ADJ_WAVES)
+                    subset_mb=TUNE_SUBSET_MB, alt_order=TUNE_ALT_ORDER, adj_waves=TUNE_ADJ_WAVES,
+                    adj_w=TUNE_ADJ_W)
         for k, v in kw.items():
             check(lib().srs_ctx_tune(self._h, keys[k], int(v)))
         return self

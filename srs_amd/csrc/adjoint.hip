@@ -18,6 +18,9 @@ namespace srs_dev {
 // each, with row y's adjoint, state and weight arrays at y times their size.
 constexpr int ADJ_NT = 13;   // D_CA, D_DYE, kon/koff of CaM, TnC, ATP, Fluo, V_RYR, A_P, K_D_SL_HI
 constexpr int ADJ_BS = 256;
+// Reverse-step launches (tiles x members) below which w is written by each
+// step for the next instead of formed in the kernel (SRS_TUNE_ADJ_W 0).
+constexpr size_t kAdjWrittenWBlocks = 4096;
 __constant__ const int kAdjTheta[ADJ_NT] = {SRS_R_D_CA,     SRS_R_D_DYE,     SRS_R_K_ON_CMDN, SRS_R_K_OFF_CMDN,
                                             SRS_R_K_ON_TRPN, SRS_R_K_OFF_TRPN, SRS_R_K_ON_ATP, SRS_R_K_OFF_ATP,
                                             SRS_R_K_ON_DYE, SRS_R_K_OFF_DYE, SRS_R_V_RYR,     SRS_R_A_P,
@@ -92,9 +95,29 @@ __device__ __forceinline__ d4 adj_weight(const RealDesc& d, int j, double c, uin
   return d4{la_c / sb.beta * iv, la_a * iv, la_m * iv, la_d * iv};
 }
 
+// Written w (small launches): w of the first step of a recomputed segment;
+// the step kernel writes each later one for the step before.
+__global__ __launch_bounds__(ADJ_BS) void k_adj_w(const RealDesc* __restrict__ D, const int32_t* __restrict__ rid,
+                                                  const double* __restrict__ X, const double* __restrict__ lam,
+                                                  d4* __restrict__ w) {
+  const RealDesc& d = D[rid[blockIdx.y]];
+  const int j = blockIdx.x * ADJ_BS + threadIdx.x;
+  if (j >= d.n) return;
+  X += blockIdx.y * 6 * (size_t)d.np;
+  lam += blockIdx.y * 6 * (size_t)d.np;
+  w += blockIdx.y * (size_t)d.np;
+  const d4 m = ((const d4*)lam)[j];
+  const uint8_t f = d.flags[j];
+  w[j] = adj_weight(d, j, (f & F_SL) ? X[4 * (size_t)j] : 0.0, f, m.x, m.y, m.z, m.w);
+}
+
 // What the step kernel does after lambda_n: add the loss source of a scan
-// sample taken after n steps (SCAN).
+// sample taken after n steps (SCAN), and, with written w, form w for step
+// n - 1 from lambda_n and that step's state (Xprev, when it is in the
+// recomputed segment).
 struct AdjNext {
+  const double* Xprev;  // state of step n - 1, or null
+  d4* wout;             // its w
   const double* cen;    // scan cell centres [3][np]
   const double* gcoef;  // the sample's loss coefficients per point
   ScanGeo g;
@@ -103,15 +126,17 @@ struct AdjNext {
 
 // lambda_n and the theta-gradient partials of one step.  The transposed
 // diffusion gathers w = (lambda_Ca / beta, lambda_ATP, lambda_CaM,
-// lambda_Fluo) / V0 of lambda_{n+1} and this step's state: each element forms
-// its own and stages it in LDS for the tile, and a neighbour outside the tile
-// forms it from its lambda (the same operations, so the same bits as a w
-// written by the step before, with no w buffer in memory).
-template <bool COMPACT, bool SCAN>
+// lambda_Fluo) / V0 of lambda_{n+1} and this step's state.  WMAT: w was
+// written by the step before (or k_adj_w) and is gathered as it is, the
+// shortest dependency chain for a launch too small to fill the GPU.  Else
+// each element forms its own w and stages it in LDS for the tile, and a
+// neighbour outside the tile forms it from its lambda: no w buffer is read
+// or written.  The operations are the same, so are the bits.
+template <bool COMPACT, bool SCAN, bool WMAT>
 __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
                                          const int32_t* __restrict__ rid, const double* __restrict__ X,
-                                         const double* __restrict__ lam, int64_t step, double* __restrict__ lam_out,
-                                         double* __restrict__ gpart, AdjNext nx) {
+                                         const double* __restrict__ lam, const d4* __restrict__ w, int64_t step,
+                                         double* __restrict__ lam_out, double* __restrict__ gpart, AdjNext nx) {
   __shared__ double red[ADJ_BS / 16][ADJ_NT];  // gradient partials per row of 16 lanes
   __shared__ d4 tw[ADJ_BS];  // the tile's own w: neighbours inside the tile read it from here
   const int r = rid[blockIdx.y];
@@ -125,6 +150,7 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
   X += blockIdx.y * 6 * np;
   lam += blockIdx.y * 6 * np;
   lam_out += blockIdx.y * 6 * np;
+  if (WMAT) w += blockIdx.y * np;
   gpart += (size_t)blockIdx.y * gridDim.x * ADJ_NT;
   // the tile's running partials, loaded now so the final update waits on no load
   const double gp_old = threadIdx.x < ADJ_NT ? gpart[(size_t)tile * ADJ_NT + threadIdx.x] : 0.0;
@@ -148,11 +174,17 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     mu = Lm[j];
     V0 = d.V0[j];
     sb = sl_buffer(d, j, m0.x, g.f);
-    const double iv = 1.0 / V0;
-    w0 = d4{mu.x / sb.beta * iv, mu.y * iv, mu.z * iv, mu.w * iv};
+    if constexpr (WMAT) {
+      w0 = w[j];
+    } else {
+      const double iv = 1.0 / V0;
+      w0 = d4{mu.x / sb.beta * iv, mu.y * iv, mu.z * iv, mu.w * iv};
+    }
   }
-  tw[threadIdx.x] = w0;
-  __syncthreads();
+  if constexpr (!WMAT) {
+    tw[threadIdx.x] = w0;
+    __syncthreads();
+  }
   // the scan sample's loss source (all lanes of the wavefront take part)
   const double src =
       SCAN ? scan_adj_sum(nx.cen, nx.g, nx.n_f, nx.gcoef + (size_t)blockIdx.y * nx.gstride, np, j, in) : 0.0;
@@ -162,11 +194,15 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     const uint8_t f = g.f;
     const int base = tile * ADJ_BS;
     auto nw = [&](int nb) -> d4 {
-      const unsigned k = (unsigned)(nb - base);
-      if (k < (unsigned)ADJ_BS) return tw[k];
-      const uint8_t fn = d.flags[nb];
-      const d4 ln = Lm[nb];
-      return adj_weight(d, nb, (fn & F_SL) ? Q[nb].x : 0.0, fn, ln.x, ln.y, ln.z, ln.w);
+      if constexpr (WMAT) {
+        return w[nb];
+      } else {
+        const unsigned k = (unsigned)(nb - base);
+        if (k < (unsigned)ADJ_BS) return tw[k];
+        const uint8_t fn = d.flags[nb];
+        const d4 ln = Lm[nb];
+        return adj_weight(d, nb, (fn & F_SL) ? Q[nb].x : 0.0, fn, ln.x, ln.y, ln.z, ln.w);
+      }
     };
     const d4 w1 = nw(g.n1), w2 = nw(g.n2), w3 = nw(g.n3), w4 = nw(g.n4);
     const bool live_tr = (f & (F_JSR | F_TROPC)) == F_TROPC;
@@ -301,6 +337,10 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
     ((d4*)lam_out)[j] = d4{lc, la, lm, ld};
     if (live_tr) lam_out[4 * np + j] = lT;
     if (live_cs) lam_out[5 * np + j] = lC;
+    if (WMAT && nx.Xprev) {
+      const double* Xp = nx.Xprev + blockIdx.y * 6 * np;
+      nx.wout[blockIdx.y * np + j] = adj_weight(d, j, (f & F_SL) ? Xp[4 * (size_t)j] : 0.0, f, lc, la, lm, ld);
+    }
   }
   // block reduction of the gradient partials, accumulated per block: DPP
   // sums within each row of 16 lanes (a component that is zero in every lane
@@ -321,20 +361,35 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
 
 // The compiler's register budget (126 VGPRs: four wavefronts per SIMD), or
 // at most 128 (four) asked for explicitly; the two compile alike.
-template <bool COMPACT, bool SCAN>
+template <bool COMPACT, bool SCAN, bool WMAT>
 __global__ __launch_bounds__(ADJ_BS) void k_adj_step(const RealDesc* __restrict__ D, const RealLive* __restrict__ L,
                                                      const int32_t* __restrict__ rid, const double* __restrict__ X,
-                                                     const double* __restrict__ lam, int64_t step,
-                                                     double* __restrict__ lam_out, double* __restrict__ gpart,
-                                                     AdjNext nx) {
-  adj_step<COMPACT, SCAN>(D, L, rid, X, lam, step, lam_out, gpart, nx);
+                                                     const double* __restrict__ lam, const d4* __restrict__ w,
+                                                     int64_t step, double* __restrict__ lam_out,
+                                                     double* __restrict__ gpart, AdjNext nx) {
+  adj_step<COMPACT, SCAN, WMAT>(D, L, rid, X, lam, w, step, lam_out, gpart, nx);
 }
-template <bool COMPACT, bool SCAN>
+template <bool COMPACT, bool SCAN, bool WMAT>
 __global__ __launch_bounds__(ADJ_BS) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_step_w4(
     const RealDesc* __restrict__ D, const RealLive* __restrict__ L, const int32_t* __restrict__ rid,
-    const double* __restrict__ X, const double* __restrict__ lam, int64_t step, double* __restrict__ lam_out,
-    double* __restrict__ gpart, AdjNext nx) {
-  adj_step<COMPACT, SCAN>(D, L, rid, X, lam, step, lam_out, gpart, nx);
+    const double* __restrict__ X, const double* __restrict__ lam, const d4* __restrict__ w, int64_t step,
+    double* __restrict__ lam_out, double* __restrict__ gpart, AdjNext nx) {
+  adj_step<COMPACT, SCAN, WMAT>(D, L, rid, X, lam, w, step, lam_out, gpart, nx);
+}
+
+using AdjStepFn = void (*)(const RealDesc*, const RealLive*, const int32_t*, const double*, const double*,
+                           const d4*, int64_t, double*, double*, AdjNext);
+template <bool COMPACT, bool SCAN, bool WMAT>
+static AdjStepFn adj_step_fn(bool waves4) {
+  return waves4 ? k_adj_step_w4<COMPACT, SCAN, WMAT> : k_adj_step<COMPACT, SCAN, WMAT>;
+}
+static AdjStepFn adj_step_kernel(bool compact, bool scan, bool wmat, bool waves4) {
+  if (compact) {
+    if (scan) return wmat ? adj_step_fn<true, true, true>(waves4) : adj_step_fn<true, true, false>(waves4);
+    return wmat ? adj_step_fn<true, false, true>(waves4) : adj_step_fn<true, false, false>(waves4);
+  }
+  if (scan) return wmat ? adj_step_fn<false, true, true>(waves4) : adj_step_fn<false, true, false>(waves4);
+  return wmat ? adj_step_fn<false, false, true>(waves4) : adj_step_fn<false, false, false>(waves4);
 }
 
 // The loss gradient of one scan sample in the Fluo adjoint:
@@ -589,10 +644,13 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
   size_t mem_free = 0, mem_total = 0;
   HIPCHK(hipMemGetInfo(&mem_free, &mem_total));
   const size_t budget = mem_free / 10 * 9;
+  // written w (two buffers) on launches of fewer than kAdjWrittenWBlocks
+  // workgroups, unless SRS_TUNE_ADJ_W forces one form
+  const bool wmat_possible = want_grad && (c->adj_w == 1 || (c->adj_w == 0 && (size_t)nblk < kAdjWrittenWBlocks));
   // device bytes per member of a sub-batch under a plan (a loss alone keeps
   // only the start state)
   auto per_member = [&](const AdjPlan& p) {
-    return 8 * n6 * (size_t)(want_grad ? p.kept() + p.K[p.L - 1] + 2 : 1) +
+    return 8 * n6 * (size_t)(want_grad ? p.kept() + p.K[p.L - 1] + 2 : 1) + (wmat_possible ? 64 * np : 0) +
            8 * (size_t)n_samples * nout + 8 * (size_t)S.n_blocks * nf + 8 * (size_t)nblk * ADJ_NT +
            8 * (size_t)n_samples * nf + 4 * max_sched + 64;
   };
@@ -622,6 +680,7 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
   c->adj_last_levels = plan.L;
   for (int l = 0; l < ADJ_MAX_LEVELS; ++l) c->adj_last_strides[l] = l < plan.L ? plan.K[l] : 0;
   c->adj_last_batch = bs;
+  const bool wmat = wmat_possible && (c->adj_w == 1 || (size_t)nblk * bs < kAdjWrittenWBlocks);
 
   // everything restored at the end
   std::vector<RealLive> live0(n_real);
@@ -638,6 +697,7 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
   double *ck = nullptr, *samp = nullptr, *part = nullptr, *dg = nullptr, *gpart = nullptr;
   double *lam0 = nullptr, *lam1 = nullptr, *seg = nullptr;
   std::vector<double*> ckl(plan.L, nullptr);  // checkpoint levels; ckl[0] = ck, the forward's
+  d4 *w0 = nullptr, *w1 = nullptr;            // written w of steps n and n - 1
   int32_t *d_sched = nullptr, *d_rid = nullptr;
   int2* d_sub_tiles = nullptr;
   const int nt_mesh = (hm.n + c->tile - 1) / c->tile;
@@ -651,6 +711,10 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
   if (want_grad) {
     HIPCHK(M.alloc(&lam0, (size_t)bs * n6));
     HIPCHK(M.alloc(&lam1, (size_t)bs * n6));
+    if (wmat) {
+      HIPCHK(M.alloc(&w0, (size_t)bs * np));
+      HIPCHK(M.alloc(&w1, (size_t)bs * np));
+    }
     HIPCHK(M.alloc(&seg, (size_t)leaf * bs * n6));
     for (int l = 1; l < plan.L; ++l) HIPCHK(M.alloc(&ckl[l], (size_t)plan.slots[l] * bs * n6));
     HIPCHK(M.alloc(&gpart, (size_t)bs * nblk * ADJ_NT));
@@ -770,6 +834,8 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
       if (e == hipSuccess) e = hipMemsetAsync(gpart, 0, (size_t)nb * nblk * ADJ_NT * sizeof(double), c->stream);
       double* lam = lam0;   // lambda_{n+1}
       double* lout = lam1;  // lambda_n
+      d4* wcur = w0;        // written w of step n
+      d4* wnext = w1;       // written w of step n - 1
       ks = n_samples - 1;
       // the loss source of a sample after the last step; the step kernel
       // adds the others to the lambda it produces
@@ -808,17 +874,17 @@ static int loss_grad_run(srs_ctx* c, const std::vector<int32_t>& ids, int32_t sc
         }
         for (int64_t n = s1 - 1; n >= s0 && e2 == hipSuccess; --n) {
           const double* Xn = seg + (size_t)(n - s0) * nb * n6;
+          if (wmat && n == s1 - 1)
+            hipLaunchKernelGGL(k_adj_w, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, d_rid, Xn, lam, wcur);
           const bool scan = ks >= 0 && sample_steps[ks] == n;
-          AdjNext nx{S.d_cen, scan ? dg + (size_t)ks * nf : nullptr, S.geo, nf, n_samples * nf};
+          AdjNext nx{wmat && n > s0 ? Xn - nb * n6 : nullptr, wnext, S.d_cen, scan ? dg + (size_t)ks * nf : nullptr,
+                     S.geo, nf, n_samples * nf};
           if (scan) --ks;
-          auto kern = c->adj_waves == 4
-                          ? (c->compact_all ? (scan ? k_adj_step_w4<true, true> : k_adj_step_w4<true, false>)
-                                            : (scan ? k_adj_step_w4<false, true> : k_adj_step_w4<false, false>))
-                          : (c->compact_all ? (scan ? k_adj_step<true, true> : k_adj_step<true, false>)
-                                            : (scan ? k_adj_step<false, true> : k_adj_step<false, false>));
-          hipLaunchKernelGGL(kern, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, d_rid, Xn, lam, n, lout,
-                             gpart, nx);
+          const AdjStepFn kern = adj_step_kernel(c->compact_all, scan, wmat, c->adj_waves == 4);
+          hipLaunchKernelGGL(kern, g_adj, dim3(ADJ_BS), 0, c->stream, c->d_desc, c->d_live, d_rid, Xn, lam, wcur, n,
+                             lout, gpart, nx);
           std::swap(lam, lout);
+          std::swap(wcur, wnext);
           e2 = hipGetLastError();
         }
         if (e2 == hipSuccess) e2 = hipStreamSynchronize(c->stream);

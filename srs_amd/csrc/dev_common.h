@@ -419,6 +419,7 @@ struct srs_ctx {
   int gpu_layout = 0;     // states records in the GPU program's layout (Ca_cyt at each RyR appended)
   int adj_levels = 0;     // adjoint checkpoint levels, 0: the fewest that fit the sub-batch
   int adj_waves = 0;      // adjoint reverse-step register budget: 0 compiler's choice, 4 wavefronts per SIMD
+  int adj_w = 0;          // adjoint w: 0 by launch size, 1 written by each step, 2 formed in the kernel
   int adj_last_levels = 0, adj_last_batch = 0;  // the plan of the last loss/gradient call
   int64_t adj_last_strides[6] = {};
   bool compact_all = false;

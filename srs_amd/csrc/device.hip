@@ -1279,6 +1279,10 @@ int srs_ctx_tune(srs_ctx* c, int32_t key, int32_t value) {
       if (value != 0 && value != 4) return srs::fail(SRS_E_ARG, "adjoint register budget must be 0 or 4");
       c->adj_waves = value;
       return SRS_OK;
+    case SRS_TUNE_ADJ_W:
+      if (value < 0 || value > 2) return srs::fail(SRS_E_ARG, "adjoint w form must be 0, 1 or 2");
+      c->adj_w = value;
+      return SRS_OK;
     case SRS_TUNE_ALT_ORDER:
       c->alt_order = value != 0;
       return c->finalized ? drop_graphs(c) : SRS_OK;

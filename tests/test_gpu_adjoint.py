@@ -655,6 +655,27 @@ def test_adjoint_register_budget_does_not_change_the_gradient():
     assert out[1][0] == out[3][0] and np.array_equal(out[1][1], out[3][1])
 
 
+def test_adjoint_w_forms_give_the_same_gradient():
+    """SRS_TUNE_ADJ_W: w written by each reverse step for the next (1) and w
+    formed inside the kernel from lambda, staged in LDS for the tile (2),
+    give the same losses and gradients bit for bit, batched and single."""
+    pr, members, samples = _batch_setup()
+    tg, _ = _targets(pr, members, samples)
+    ctx, sid, nf, _ = _batch_ctx(pr, members)
+    n = len(members)
+    scheds = [m[2] for m in members]
+    out = []
+    for form in (1, 2):
+        ctx.tune(adj_w=form)
+        out.append(ctx.loss_grad_batch(range(n), sid, scheds, samples, tg, ckpt=31))
+        out.append(ctx.loss_grad(2, sid, scheds[2], samples, tg[2], 64))
+    ctx.tune(adj_w=0)
+    ctx.close()
+    assert np.array_equal(out[0][0], out[2][0]) and np.array_equal(out[0][1], out[2][1])
+    assert out[1][0] == out[3][0] and np.array_equal(out[1][1], out[3][1])
+    assert np.all(np.isfinite(out[0][1])) and np.any(out[0][1] != 0.0)
+
+
 @pytest.mark.parametrize("levels", [2, 3])
 def test_checkpoint_levels_do_not_change_the_single_gradient(levels):
     """A single realization with a multi-level checkpoint plan (its leaf
