@@ -159,8 +159,8 @@ __device__ __forceinline__ void adj_step(const RealDesc* __restrict__ D, const R
   double gr[ADJ_NT];
 #pragma unroll
   for (int k = 0; k < ADJ_NT; ++k) gr[k] = 0.0;
-  // the element's operator row, state, lambda_{n+1} and w; the tile's w
-  // staged in LDS
+  // the element's operator row, state, lambda_{n+1} and w; with w formed
+  // here, the tile's w staged in LDS
   const bool in = j < d.n;
   const d4* Q = (const d4*)X;
   const d4* Lm = (const d4*)lam;
