@@ -276,7 +276,7 @@ def bench_adjoint(t_sim=5.0, n_points=64, n_samples=50, ckpt=256, adj_waves=0, t
 
 
 def bench_adjoint_batch(n_real=256, n_steps=256, n_points=64, n_samples=8, max_batch=128, adj_waves=0,
-                        sweep_tune=None):
+                        sweep_tune=None, adj_w=0):
     """Batched adjoint workload: n_real realizations on the dyad-box geometry
     at h = 30 nm, each with its own RyR trajectory (Philox stream r) recorded
     by one gated run of all of them, frozen for the loss; a 64-point line
@@ -285,13 +285,14 @@ def bench_adjoint_batch(n_real=256, n_steps=256, n_points=64, n_samples=8, max_b
     every member's gradient (the library's checkpoint plan, at most max_batch
     members stepped at once).  sweep_tune: sweep tunables set after the set-up
     (nt_geometry, nt_state, occupancy, lds), for the forward and recomputed
-    steps."""
+    steps; adj_w: the reverse step's w form (0 by launch size, 1 written by
+    each step, 2 formed in the kernel)."""
     from srs_amd import workloads as W
 
     t0 = time.perf_counter()
     ctx, sid, scheds, samples, tgt, p, info = W.dyad_adjoint_batch(n_real, n_steps, n_points, n_samples, max_batch,
                                                                    log=log)
-    ctx.tune(adj_waves=adj_waves, **(sweep_tune or {}))
+    ctx.tune(adj_waves=adj_waves, adj_w=adj_w, **(sweep_tune or {}))
     t_setup = time.perf_counter() - t0
     idx = __import__("srs_amd").adjoint_theta()
     t0 = time.perf_counter()
