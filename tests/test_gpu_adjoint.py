@@ -664,6 +664,8 @@ def test_adjoint_w_forms_give_the_same_gradient():
     ctx, sid, nf, _ = _batch_ctx(pr, members)
     n = len(members)
     scheds = [m[2] for m in members]
+    with pytest.raises(S.SrsError, match="w form"):
+        ctx.tune(adj_w=3)
     out = []
     for form in (1, 2):
         ctx.tune(adj_w=form)
