@@ -508,6 +508,14 @@ int srs_ctx_add_scan(srs_ctx* ctx, int32_t mesh_id, const double* centers, const
  * padding cells included).  dF/F0 = cafb / f0 - 1. */
 int srs_scan_sample(srs_ctx* ctx, int32_t real, int32_t scan_id, double* ca, double* caf, double* cafb);
 
+/* Replaces the linescan program's WriteOutputFile (linescan/main.cpp:316-336),
+ * which writes scan_output_Ca.txt, scan_output_CaF.txt and scan_output.txt
+ * (:482-485, :252-254): rows[j * n_cols + i] as printf "%g " and a newline
+ * after each of the n_rows rows; one row per scan (the ca, caf or cafb of an
+ * srs_scan_sample call), one column per point.  SRS_E_IO if the file cannot
+ * be written (the reference prints a warning). */
+int srs_scan_write(const char* path, const double* rows, int32_t n_rows, int32_t n_cols);
+
 /* ====================================================== adjoint (fitting) */
 /* These calls have no counterpart in the reference: they serve parameter
  * fitting against line scans with the channel trajectory held fixed. */

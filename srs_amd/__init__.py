@@ -329,6 +329,13 @@ def adjoint_theta() -> np.ndarray:
     return idx
 
 
+def scan_write(path: str, rows) -> None:
+    """One of the linescan program's output files (WriteOutputFile,
+    linescan/main.cpp:316-336): one row per scan, one column per point."""
+    a = np.ascontiguousarray(np.atleast_2d(np.asarray(rows, np.float64)))
+    check(lib().srs_scan_write(str(path).encode(), _f64(a), a.shape[0], a.shape[1]))
+
+
 def dff0(cafb: np.ndarray, f0: float) -> np.ndarray:
     """dF/F0 of a blurred fluorescence scan line against the resting signal."""
     return np.asarray(cafb) / f0 - 1.0
@@ -626,6 +633,7 @@ class Context:
         return a.value, b.value
 
 
-__all__ = ["Params", "Grid", "BoxSpec", "Context", "SrsError", "read_parameters", "read_scan_parameters", "dff0", "grid_read", "box_grid",
+__all__ = ["Params", "Grid", "BoxSpec", "Context", "SrsError", "read_parameters", "read_scan_parameters", "scan_write",
+           "dff0", "grid_read", "box_grid",
            "box_mesh_write", "bfs_order", "assemble_operator", "equilibrium_state", "boundary_rates",
            "states_record", "run_cru3d", "I", "R"]

@@ -136,6 +136,7 @@ _PROTOS = {
     "srs_scan_read_parameters": (C.c_int, [C.c_char_p, C.POINTER(SrsScanDesc)]),
     "srs_ctx_add_scan": (C.c_int, [C.c_void_p, C.c_int32, f64p, C.POINTER(SrsScanDesc), i32p, i32p, i32p]),
     "srs_scan_sample": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p, f64p, f64p]),
+    "srs_scan_write": (C.c_int, [C.c_char_p, f64p, C.c_int32, C.c_int32]),
     "srs_adjoint_theta": (C.c_int, [i32p]),
     "srs_params_set": (C.c_int, [C.c_void_p, C.c_int32, f64p]),
     "srs_record_schedule": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, i32p]),

@@ -6,6 +6,7 @@
 // reader implements the subset of XML those files use (elements, attributes,
 // comments, processing instructions, CDATA, character and the five named
 // entities) without a library dependency.
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -408,5 +409,25 @@ extern "C" int srs_scan_read_parameters(const char* path, srs_scan_desc* d) {
       else return srs::fail(SRS_E_ARG, "invalid scan axis: %s", v);
     }
   }
+  return SRS_OK;
+}
+
+// One of the linescan program's output files (WriteOutputFile,
+// linescan/main.cpp:316-336): each row as printf "%g " per value, then a
+// newline.  The reference warns and goes on when the file cannot be opened;
+// here that is SRS_E_IO.
+extern "C" int srs_scan_write(const char* path, const double* rows, int32_t n_rows, int32_t n_cols) {
+  srs::clear_error();
+  if (!path) return srs::fail(SRS_E_ARG, "null path");
+  if (n_rows < 0 || n_cols < 0) return srs::fail(SRS_E_ARG, "negative size");
+  if (!rows && n_rows > 0 && n_cols > 0) return srs::fail(SRS_E_ARG, "null rows");
+  FILE* f = fopen(path, "w");
+  if (!f) return srs::fail(SRS_E_IO, "cannot open output file %s", path);
+  for (int32_t j = 0; j < n_rows; ++j) {
+    for (int32_t i = 0; i < n_cols; ++i) fprintf(f, "%g ", rows[(size_t)j * n_cols + i]);
+    fputc('\n', f);
+  }
+  const bool failed = ferror(f) != 0;
+  if (fclose(f) != 0 || failed) return srs::fail(SRS_E_IO, "write error on %s", path);
   return SRS_OK;
 }

@@ -78,6 +78,28 @@ def test_scan_parameter_errors(tmp_path):
         S.read_scan_parameters(str(f))
 
 
+def test_scan_output_file_format(tmp_path):
+    """srs_scan_write gives the linescan program's output files byte for byte:
+    WriteOutputFile (linescan/main.cpp:316-336) prints every value of a row
+    with "%g " and ends the row with a newline; Python's %g is C's printf %g
+    for finite doubles (exponent forms, six significant digits, -0)."""
+    rng = np.random.default_rng(5)
+    rows = rng.random((4, 7)) * 10.0 ** rng.integers(-7, 8, (4, 7))
+    rows[0, :4] = [0.0, -0.0, 1e-5, 123456789.0]
+    rows[1, :3] = [4.1667, 0.0999999999, -2.5e-300]
+    f = tmp_path / "scan_output.txt"
+    S.scan_write(str(f), rows)
+    expect = "".join("".join("%g " % v for v in row) + "\n" for row in rows)
+    assert f.read_bytes() == expect.encode()
+    # one row, and no rows at all (an empty file, as the reference writes for 0 scans)
+    S.scan_write(str(f), rows[2])
+    assert f.read_text() == "".join("%g " % v for v in rows[2]) + "\n"
+    S.scan_write(str(f), np.zeros((0, 5)))
+    assert f.read_bytes() == b""
+    with pytest.raises(S.SrsError, match="cannot open"):
+        S.scan_write(str(tmp_path / "no_such_dir" / "x.txt"), rows)
+
+
 def _scan(**kw):
     d = S.read_scan_parameters(None)
     for k, v in kw.items():
